@@ -1,0 +1,661 @@
+"""FP64 NumPy restatement of the reference phase-screen hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the oracle the HIP library is
+checked against; it is imported by ``tests/``, ``__graft_entry__.smoke`` and
+the ``cpu_baseline`` leg of ``bench.py`` and by nothing in ``pyturb_amd``.
+
+Every function names the reference file:line it restates (paths relative to
+the reference checkout, ``src/pyturb/...``).  All arithmetic is float64 /
+complex128; noise and innovations are *injected* as arrays so the GPU and
+the oracle consume identical inputs.
+
+Pinning (how each function is checked, see ``tests/test_oracle.py``):
+
+* ``psd``, ``band_amplitude``, ``build_filters``: golden vectors written from
+  the reference (``tests/golden/make_golden.py``) plus the closed form of the
+  PSD and the flat-PSD limit of the band amplitude.
+* ``ifft2_unnormalised``: brute-force DFT on tiny grids.
+* ``generate``: reference golden screens (seed=1, n=256 subset + checksums),
+  and the real/imag pairing identity.
+* ``integrate`` / ``integrate_dirs`` / ``flow_translate``: reference golden
+  frames (9-layer n=64 atmosphere), the integer-pixel ``np.roll`` identity and
+  the layer-sum == sum of single-layer translates identity.
+* ``phase_covariance``: C(0) = 0.0863 (L0/r0)^(5/3) closed form and golden.
+* ``build_extrusion``: ``A C_zz = C_xz`` and ``B B^T = C_xx - A C_xz^T``
+  identities and golden A/B at W=32.
+* ``compact_row_ring`` / ``OracleExtruder`` / ``readout``: golden readouts
+  from a 2-layer n=32 extruder, exactness at integer travel and
+  "one big jump == many small steps".
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+from scipy import linalg as _sla
+from scipy.special import gamma as _gamma
+from scipy.special import kv as _kv
+
+ARCSEC = np.pi / (180.0 * 3600.0)
+
+__all__ = [
+    "ARCSEC",
+    "psd",
+    "band_amplitude",
+    "build_filters",
+    "draw_generate_noise",
+    "ifft2_unnormalised",
+    "generate",
+    "subharmonic_field",
+    "draw_flow",
+    "flow_translate",
+    "integrate",
+    "integrate_dirs",
+    "OracleAtmosphere",
+    "phase_covariance",
+    "build_extrusion",
+    "compact_row_ring",
+    "catmull_rom_weights",
+    "lanczos_weights",
+    "readout",
+    "OracleExtruder",
+    "OracleInfinite",
+    "structure_function",
+    "r0_from_seeing",
+    "rel_rms",
+]
+
+
+# ---------------------------------------------------------------------------
+# small helpers
+# ---------------------------------------------------------------------------
+def rel_rms(test, ref) -> float:
+    """Relative RMS error ``||test-ref||_2 / ||ref||_2`` in float64."""
+    test = np.asarray(test, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    den = float(np.sqrt(np.mean(ref * ref)))
+    return float(np.sqrt(np.mean((test - ref) ** 2))) / (den if den > 0 else 1.0)
+
+
+def r0_from_seeing(seeing: float, wavelength: float = 500e-9) -> float:
+    """utils.py:137-142 -- FWHM = 0.98 lambda / r0."""
+    return 0.98 * wavelength / (seeing * ARCSEC)
+
+
+def structure_function(phase, pixel_scale=1.0, max_separation=None):
+    """utils.py:159-201 -- axis-averaged structure function estimate."""
+    ph = np.asarray(phase, dtype=np.float64)
+    if ph.ndim == 2:
+        ph = ph[None]
+    n = min(ph.shape[-2:])
+    smax = n // 4 if max_separation is None else int(max_separation)
+    out = np.empty(smax)
+    for k, s in enumerate(range(1, smax + 1)):
+        d0 = ph[..., s:, :] - ph[..., :-s, :]
+        d1 = ph[..., :, s:] - ph[..., :, :-s]
+        out[k] = 0.5 * (np.mean(d0 * d0) + np.mean(d1 * d1))
+    return np.arange(1, smax + 1) * pixel_scale, out
+
+
+# ---------------------------------------------------------------------------
+# PSD, band amplitudes, filters  (fourier.py:131-228)
+# ---------------------------------------------------------------------------
+def psd(f_squared, r0, L0, power_law=11.0 / 3.0, inner_scale=0.0):
+    """fourier.py:131-149 -- modified von Karman phase PSD [rad^2 m^2]."""
+    f2 = np.asarray(f_squared, dtype=np.float64)
+    k0 = 0.0 if np.isinf(L0) else L0 ** -2.0
+    val = 0.023 * r0 ** (-5.0 / 3.0) * (f2 + k0) ** (-0.5 * power_law)
+    if inner_scale > 0.0:
+        fm = 5.92 / (2.0 * np.pi * inner_scale)
+        val = val * np.exp(-f2 / (fm * fm))
+    return val
+
+
+def band_amplitude(fx, fy, width, r0, L0, power_law=11.0 / 3.0,
+                   inner_scale=0.0, samples=33):
+    """fourier.py:151-172 -- cell-integrated mode amplitude.
+
+    Midpoint quadrature of the PSD over a ``width``-wide square cell centred
+    on ``(fx, fy)``: ``amp^2 = mean(PSD) w^2 * <f^2>_PSD / |f_c|^2``.
+    """
+    u = (np.arange(samples) + 0.5) / samples - 0.5
+    ax = fx + u * width
+    ay = fy + u * width
+    f2 = ax[:, None] ** 2 + ay[None, :] ** 2
+    p = psd(f2, r0, L0, power_law, inner_scale)
+    power = p.mean() * width * width
+    f2_eff = float(np.sum(p * f2) / np.sum(p))
+    return math.sqrt(power * f2_eff / (fx * fx + fy * fy))
+
+
+def build_filters(n, dx, r0, L0, subharmonics=8, power_law=11.0 / 3.0,
+                  inner_scale=0.0):
+    """fourier.py:174-228 -- main-grid amplitude and subharmonic bases.
+
+    Returns a dict with ``amplitude`` (n,n), ``f`` (n,), ``sh_amp`` (P,3,3),
+    ``sh_freq`` (P,3) and ``sh_basis`` (P,3,n) complex, all float64.
+    """
+    df = 1.0 / (n * dx)
+    f = np.fft.fftfreq(n, d=dx)
+    f2 = f[:, None] ** 2 + f[None, :] ** 2
+    f2[0, 0] = 1.0
+    amp = np.sqrt(psd(f2, r0, L0, power_law, inner_scale)) * df
+    amp[0, 0] = 0.0
+    kmax = min(8, (n - 1) // 2)
+    for a in range(-kmax, kmax + 1):
+        for b in range(-kmax, kmax + 1):
+            if a or b:
+                amp[a, b] = band_amplitude(f[a], f[b], df, r0, L0, power_law,
+                                           inner_scale)
+    x = (np.arange(n) - 0.5 * n) * dx
+    P = int(subharmonics)
+    sh_amp = np.zeros((P, 3, 3))
+    sh_freq = np.zeros((P, 3))
+    sh_basis = np.zeros((P, 3, n), dtype=np.complex128)
+    for p in range(P):
+        dfp = df / 3.0 ** (p + 1)
+        fp = np.array([-dfp, 0.0, dfp])
+        sh_freq[p] = fp
+        for a in range(3):
+            for b in range(3):
+                if a != 1 or b != 1:
+                    sh_amp[p, a, b] = band_amplitude(fp[a], fp[b], dfp, r0, L0,
+                                                     power_law, inner_scale)
+        sh_basis[p] = np.exp(2j * np.pi * fp[:, None] * x[None, :])
+    return dict(amplitude=amp, f=f, sh_amp=sh_amp, sh_freq=sh_freq,
+                sh_basis=sh_basis, n=n, dx=dx)
+
+
+# ---------------------------------------------------------------------------
+# generate  (fourier.py:233-282)
+# ---------------------------------------------------------------------------
+def draw_generate_noise(rng, n, count, levels, dtype="float64"):
+    """The reference's draw order for one ``generate(count)`` call.
+
+    fourier.py:256-257 (main noise (2, ceil(count/2), n, n)) then
+    fourier.py:271-272 (one (2, count, 3, 3) block per subharmonic level).
+    """
+    n_fft = (count + 1) // 2
+    main = rng.standard_normal((2, n_fft, n, n), dtype=dtype)
+    sh = np.stack([rng.standard_normal((2, count, 3, 3), dtype=dtype)
+                   for _ in range(levels)]) if levels else np.zeros((0, 2, count, 3, 3))
+    return main, sh
+
+
+def ifft2_unnormalised(x):
+    """``ifft2 * n0*n1``: the +i, unnormalised 2-D DFT over the last two axes."""
+    x = np.asarray(x, dtype=np.complex128)
+    return np.fft.ifft2(x, axes=(-2, -1)) * (x.shape[-1] * x.shape[-2])
+
+
+def subharmonic_field(coeffs, basis):
+    """Real low-frequency field of stacked 3x3 coefficients, mean removed.
+
+    ``coeffs`` (..., P, 3, 3) complex, ``basis`` (P, 3, n) complex ->
+    (..., n, n) real: ``Re(sum_p basis_p^T c_p basis_p)`` minus its mean over
+    the grid (fourier.py:274-278; atmosphere.py:931-934).
+    """
+    low = np.einsum("pax,...pab,pby->...xy", basis, coeffs, basis).real
+    return low - low.mean(axis=(-2, -1), keepdims=True)
+
+
+def generate(filters, noise, sh_noise, count=None):
+    """fourier.py:233-282 -- screens from injected noise, float64.
+
+    ``noise`` (2, n_fft, n, n); ``sh_noise`` (P, 2, count, 3, 3).
+    Returns (count, n, n) radians.
+    """
+    noise = np.asarray(noise, dtype=np.float64)
+    if count is None:
+        count = 2 * noise.shape[1]
+    field = ifft2_unnormalised((noise[0] + 1j * noise[1]) * filters["amplitude"])
+    screens = np.concatenate((field.real, field.imag))[:count]
+    P = filters["sh_amp"].shape[0]
+    if P:
+        sh = np.asarray(sh_noise, dtype=np.float64)
+        cn = (sh[:, 0] + 1j * sh[:, 1]) * filters["sh_amp"][:, None]  # (P,count,3,3)
+        screens = screens + subharmonic_field(np.moveaxis(cn, 0, 1),
+                                              filters["sh_basis"])
+    return screens
+
+
+# ---------------------------------------------------------------------------
+# spectral frozen flow  (flow.py:71-129, atmosphere.py:882-998)
+# ---------------------------------------------------------------------------
+def draw_flow(rng, filters, dtype="float64"):
+    """flow.py:71-94 -- fixed spectrum and per-level 3x3 coefficients."""
+    n = filters["n"]
+    w = rng.standard_normal((2, n, n), dtype=dtype).astype(np.float64)
+    spectrum = (w[0] + 1j * w[1]) * filters["amplitude"]
+    coeffs = []
+    for p in range(filters["sh_amp"].shape[0]):
+        v = rng.standard_normal((2, 3, 3), dtype=dtype).astype(np.float64)
+        coeffs.append((v[0] + 1j * v[1]) * filters["sh_amp"][p])
+    P = len(coeffs)
+    return spectrum, (np.stack(coeffs) if P else np.zeros((0, 3, 3), complex))
+
+
+def _shift_phasor(s, freqs):
+    return np.exp(2j * np.pi * np.multiply.outer(np.asarray(s, float), freqs))
+
+
+def integrate(spectra, sh_coeffs, f, sh_freq, sh_basis, disp_x, disp_y,
+              crop=None):
+    """atmosphere.py:882-938 -- one on-axis frame, float64.
+
+    ``spectra`` (L,n,n); ``sh_coeffs`` (P,L,3,3); ``disp_x/y`` (L,) metres.
+    """
+    px = _shift_phasor(disp_x, f)  # (L, n)
+    py = _shift_phasor(disp_y, f)
+    spec = np.einsum("lij,li,lj->ij", spectra, px, py)
+    total = ifft2_unnormalised(spec).real
+    P = sh_coeffs.shape[0]
+    if P:
+        qx = np.exp(2j * np.pi * np.asarray(disp_x)[None, :, None] * sh_freq[:, None, :])
+        qy = np.exp(2j * np.pi * np.asarray(disp_y)[None, :, None] * sh_freq[:, None, :])
+        shifted = np.einsum("plab,pla,plb->pab", sh_coeffs, qx, qy)
+        total = total + subharmonic_field(shifted, sh_basis)
+    if crop is not None:
+        total = total[crop, crop]
+    return total
+
+
+def integrate_dirs(spectra, sh_coeffs, f, sh_freq, sh_basis, disp_x, disp_y,
+                   crop=None):
+    """atmosphere.py:940-998 -- (D,L) displacements -> (D, n, n)."""
+    return np.stack([
+        integrate(spectra, sh_coeffs, f, sh_freq, sh_basis, dxs, dys, crop)
+        for dxs, dys in zip(np.asarray(disp_x), np.asarray(disp_y))
+    ])
+
+
+def flow_translate(spectrum, coeffs, filters, sx, sy):
+    """flow.py:96-129 -- single-layer screen blown by (sx, sy) metres."""
+    return integrate(spectrum[None], coeffs[:, None], filters["f"],
+                     filters["sh_freq"], filters["sh_basis"], [sx], [sy])
+
+
+class OracleAtmosphere:
+    """atmosphere.py:305-579 + 1191-1219 -- spectral-engine state in FP64.
+
+    Reproduces the reference's SeedSequence tree (atmosphere.py:393-437,
+    448-470) so that, for a given ``seed`` and ``dtype``, the layer spectra,
+    subharmonic coefficients and sample generators are the reference's own.
+    ``layers`` is a sequence of (altitude, cn2_fraction, wind_speed,
+    wind_direction_deg, L0) tuples.
+    """
+
+    def __init__(self, layers, r0, wavelength=500e-9, zenith_angle=0.0,
+                 diameter=8.0, n=512, L0=None, power_law=11.0 / 3.0,
+                 inner_scale=0.0, subharmonics=8, field_of_view=0.0,
+                 dtype="float32", seed=None):
+        lay = [tuple(float(v) for v in ly) for ly in layers]
+        frac = np.array([ly[1] for ly in lay])
+        frac = frac / frac.sum()
+        if L0 is not None:
+            lay = [(a, f, s, d, float(L0)) for (a, f, s, d, _) in lay]
+        self.layers = lay
+        self.n = int(n)
+        self.dx = diameter / n
+        self.wavelength = wavelength
+        cz = math.cos(math.radians(zenith_angle))
+        self.airmass = 1.0 / cz
+        self.r0_los = r0 * cz ** 0.6
+        alt = np.array([ly[0] for ly in lay]) * self.airmass
+        margin = int(np.ceil(alt.max() * np.tan(field_of_view * ARCSEC) / self.dx))
+        self.margin = margin
+        self.ns = self.n + 2 * margin
+        self.crop = slice(margin, margin + self.n)
+        self.alt = alt
+        theta = np.deg2rad([ly[3] for ly in lay])
+        speed = np.array([ly[2] for ly in lay])
+        self.vx = speed * np.cos(theta)
+        self.vy = speed * np.sin(theta)
+        self.speed = speed
+        self.r0_layer = self.r0_los * frac ** (-0.6)
+        self.period = self.ns * self.dx
+        master = np.random.SeedSequence(seed)
+        children = master.spawn(len(lay))
+        master.spawn(1)  # spectral boiling stream (atmosphere.py:395)
+        master.spawn(1)  # extruder boiling stream (atmosphere.py:397)
+        self.filters, spectra, coeffs = [], [], []
+        self.gen_seeds, self.ext_seeds = [], []
+        kw = dict(subharmonics=subharmonics, power_law=power_law,
+                  inner_scale=inner_scale)
+        for ly, r0_i, child in zip(lay, self.r0_layer, children):
+            gseed, fseed, eseed = child.spawn(3)
+            filt = build_filters(self.ns, self.dx, r0_i, ly[4], **kw)
+            self.filters.append(filt)
+            self.gen_seeds.append(int(gseed.generate_state(1)[0]))
+            self.ext_seeds.append(int(eseed.generate_state(1)[0]))
+            rng = np.random.default_rng(int(fseed.generate_state(1)[0]))
+            spec, cf = draw_flow(rng, filt, dtype)
+            spectra.append(spec)
+            coeffs.append(cf)
+        self.spectra = np.stack(spectra)
+        self.sh_coeffs = np.stack(coeffs, axis=1)  # (P, L, 3, 3)
+        self.f = self.filters[0]["f"]
+        self.sh_freq = self.filters[0]["sh_freq"]
+        self.sh_basis = self.filters[0]["sh_basis"]
+        # sample() generators, one per distinct round(L0, 9)
+        groups: Dict[float, List[int]] = {}
+        for i, ly in enumerate(lay):
+            groups.setdefault(round(ly[4], 9), []).append(i)
+        self.sample_groups = []
+        for idx, child in zip(groups.values(), master.spawn(len(groups))):
+            if len(idx) == 1:
+                i = idx[0]
+                self.sample_groups.append((self.filters[i], self.gen_seeds[i]))
+                continue
+            r0_agg = float(np.sum(self.r0_layer[idx] ** (-5.0 / 3.0)) ** -0.6)
+            filt = build_filters(self.ns, self.dx, r0_agg, lay[idx[0]][4], **kw)
+            self.sample_groups.append((filt, int(child.generate_state(1)[0])))
+
+    def displacements(self, t, ox=0.0, oy=0.0):
+        return self.vx * t + self.alt * ox, self.vy * t + self.alt * oy
+
+    def phase(self, t, ox=0.0, oy=0.0):
+        sx, sy = self.displacements(t, ox, oy)
+        return integrate(self.spectra, self.sh_coeffs, self.f, self.sh_freq,
+                         self.sh_basis, sx, sy, self.crop)
+
+    def opd(self, t=0.0, directions=None):
+        scale = self.wavelength / (2.0 * np.pi)
+        if directions is None:
+            return self.phase(t) * scale
+        return np.stack([self.phase(t, np.tan(a * ARCSEC), np.tan(b * ARCSEC))
+                         for a, b in directions]) * scale
+
+    def sample(self, count, noises=None, dtype="float64"):
+        """atmosphere.py:1191-1219 -- summed group screens, OPD metres.
+
+        ``noises``: optional list of ``(main, sh)`` per group; otherwise drawn
+        from each group's own seed (fresh generator, first call).
+        """
+        total = 0.0
+        for g, (filt, gseed) in enumerate(self.sample_groups):
+            if noises is None:
+                rng = np.random.default_rng(gseed)
+                main, sh = draw_generate_noise(rng, self.ns, count,
+                                               filt["sh_amp"].shape[0], dtype)
+            else:
+                main, sh = noises[g]
+            total = total + generate(filt, main, sh, count)
+        return total[:, self.crop, self.crop] * (self.wavelength / (2 * np.pi))
+
+
+# ---------------------------------------------------------------------------
+# covariance and extrusion matrices  (infinite.py:54-117, extrude.py:250-304)
+# ---------------------------------------------------------------------------
+def phase_covariance(r, r0, L0):
+    """infinite.py:88-117 -- von Karman phase covariance [rad^2]."""
+    r = np.asarray(r, dtype=np.float64)
+    c = ((L0 / r0) ** (5.0 / 3.0) * 2.0 ** (-5.0 / 6.0) * _gamma(11.0 / 6.0)
+         / np.pi ** (8.0 / 3.0) * (4.8 * _gamma(1.2)) ** (5.0 / 6.0))
+    x = 2.0 * np.pi * r / L0
+    core = np.empty_like(x)
+    zero = x <= 0
+    core[zero] = 2.0 ** (-1.0 / 6.0) * _gamma(5.0 / 6.0)
+    xs = x[~zero]
+    core[~zero] = xs ** (5.0 / 6.0) * _kv(5.0 / 6.0, xs)
+    return c * core
+
+
+def _spd_solve(spd, rhs):
+    """infinite.py:54-64 -- Cholesky solve with least-squares fallback."""
+    try:
+        return _sla.cho_solve(_sla.cho_factor(spd), rhs)
+    except np.linalg.LinAlgError:
+        return np.linalg.lstsq(spd, rhs, rcond=None)[0]
+
+
+def build_extrusion(width, stencil_rows, dx, L0, r0=1.0):
+    """extrude.py:250-304 / infinite.py:231-269 -- (A, B) in float64.
+
+    Stencil rows at y = 0..m-1, new row at y = m, columns x = 0..W-1;
+    ``A = C_xz C_zz^-1`` and ``B = V sqrt(max(lambda, 0))`` of the
+    symmetrised residual covariance.
+    """
+    m, W = int(stencil_rows), int(width)
+    yy, xx = np.meshgrid(np.arange(m, dtype=float), np.arange(W, dtype=float),
+                         indexing="ij")
+    zx, zy = xx.ravel(), yy.ravel()
+    nx, ny = np.arange(W, dtype=float), np.full(W, float(m))
+
+    def cov(ax, ay, bx, by):
+        d = np.hypot(ax[:, None] - bx[None, :], ay[:, None] - by[None, :])
+        return phase_covariance(d * dx, r0, L0)
+
+    czz = cov(zx, zy, zx, zy)
+    cxz = cov(nx, ny, zx, zy)
+    cxx = cov(nx, ny, nx, ny)
+    A = _spd_solve(czz, cxz.T).T
+    res = cxx - A @ cxz.T
+    res = 0.5 * (res + res.T)
+    lam, vec = np.linalg.eigh(res)
+    B = vec * np.sqrt(np.clip(lam, 0.0, None))
+    return A, B
+
+
+# ---------------------------------------------------------------------------
+# ring buffer, interpolation, readout  (ring.py, extrude.py, infinite.py)
+# ---------------------------------------------------------------------------
+def compact_row_ring(buf, base, fill, stencil_rows, first_readable):
+    """ring.py:9-29 -- drop consumed rows, keep the stencil tail."""
+    keep_from = min(int(math.floor(first_readable)) - 1,
+                    base + fill - stencil_rows) - base
+    keep_from = max(1, keep_from)
+    keep = fill - keep_from
+    buf[:keep] = buf[keep_from:fill].copy()
+    return base + keep_from, keep
+
+
+def catmull_rom_weights(t):
+    """extrude.py:74-83 -- taps at offsets (-1, 0, 1, 2)."""
+    t2 = t * t
+    t3 = t2 * t
+    return (0.5 * (-t + 2 * t2 - t3), 0.5 * (2 - 5 * t2 + 3 * t3),
+            0.5 * (t + 4 * t2 - 3 * t3), 0.5 * (t3 - t2))
+
+
+def lanczos_weights(t, a=3):
+    """infinite.py:67-85 -- normalised Lanczos-a taps at offsets -a+1..a."""
+    offs = tuple(range(-a + 1, a + 1))
+    raw = [np.sinc(t - d) * np.sinc((t - d) / a) for d in offs]
+    tot = sum(raw[1:], raw[0])
+    return offs, tuple(r / tot for r in raw)
+
+
+def _taps(interp, frac):
+    if interp == "linear":
+        return (0, 1), (1.0 - frac, frac)
+    if interp == "lanczos":
+        return lanczos_weights(frac)
+    return (-1, 0, 1, 2), catmull_rom_weights(frac)
+
+
+def readout(buf, fill, along, perp, shift_along, shift_perp, interp="cubic"):
+    """extrude.py:883-915 -- multi-layer 2-D interpolated gather, summed.
+
+    ``buf`` (L, cap, W); ``fill`` (L,) valid rows; ``along``/``perp``
+    (L, n, n) grids; shifts (L,).  Float64 accumulation.
+    """
+    L, _, W = buf.shape
+    total = 0.0
+    for l in range(L):
+        row = along[l] + shift_along[l]
+        col = perp[l] + shift_perp[l]
+        r0 = np.floor(row).astype(np.int64)
+        c0 = np.floor(col).astype(np.int64)
+        ro, rw = _taps(interp, row - r0)
+        co, cw = _taps(interp, col - c0)
+        b = buf[l].astype(np.float64)
+        acc = 0.0
+        for dr, wr in zip(ro, rw):
+            rr = np.clip(r0 + dr, 0, int(fill[l]) - 1)
+            racc = 0.0
+            for dc, wc in zip(co, cw):
+                cc = np.clip(c0 + dc, 0, W - 1)
+                racc = racc + wc * b[rr, cc]
+            acc = acc + wr * racc
+        total = total + acc
+    return total
+
+
+def pupil_grids(n, cos_w, sin_w, W, magnification=1.0):
+    """extrude.py:376-383 -- rotated pupil grid in wind-frame pixels."""
+    g = (np.arange(n, dtype=np.float64) - 0.5 * (n - 1)) * magnification
+    along = g[:, None] * cos_w + g[None, :] * sin_w
+    perp = -g[:, None] * sin_w + g[None, :] * cos_w + 0.5 * W
+    return along, perp
+
+
+def along_extent_and_span(n, vx, vy, mag, fov):
+    """extrude.py:222-247."""
+    sp = math.hypot(vx, vy)
+    c = vx / sp if sp > 0 else 1.0
+    s = vy / sp if sp > 0 else 0.0
+    ext = 0.5 * (n - 1) * mag * (abs(c) + abs(s))
+    span = int(math.ceil(ext + fov)) - int(math.floor(-ext - fov)) + 3
+    return ext, span
+
+
+class OracleExtruder:
+    """extrude.py:307-469, 520-659 -- frozen-flow multi-layer extruder, FP64.
+
+    Innovations are injected: ``innovation(layer, k)`` must return the k-th
+    (W,) white-noise row for that layer (the reference draws it from the
+    layer's generator after the seed screen, extrude.py:428).  ``seed_rows``
+    (L, span_l, W) gives each layer's initial rows (extrude.py:408-420).
+    """
+
+    def __init__(self, n, dx, layer_r0, layer_L0, layer_wind, layer_alt,
+                 fov_pix=0.0, stencil_rows=2, lgs_alt=None, A=None, B_unit=None):
+        self.n, self.dx, self.m = int(n), float(dx), int(stencil_rows)
+        L = len(layer_r0)
+        fov = [float(fov_pix)] * L if np.ndim(fov_pix) == 0 else list(fov_pix)
+        mags = [1.0 - a / lgs_alt for a in layer_alt] if lgs_alt else [1.0] * L
+        ext, spans = zip(*[along_extent_and_span(self.n, vx, vy, mg, fv)
+                           for (vx, vy), mg, fv in zip(layer_wind, mags, fov)])
+        self.W = int(math.ceil(2.0 * max(ext) + 2.0 * max(fov))) + 4
+        self.cap = max(spans) + self.n + self.m + 32
+        self.spans = list(spans)
+        self.L = L
+        if A is None:
+            A, B_unit = build_extrusion(self.W, self.m, self.dx, layer_L0[0])
+        self.A = np.asarray(A, np.float64)
+        self.B = [np.asarray(B_unit, np.float64) * r ** (-5.0 / 6.0) for r in layer_r0]
+        self.buf = np.zeros((L, self.cap, self.W))
+        self.base = np.zeros(L, np.int64)
+        self.fill = np.zeros(L, np.int64)
+        self.travel = np.zeros(L)
+        self.speed, self.cos, self.sin = [], [], []
+        self.along, self.perp = [], []
+        self.along_min, self.along_max, self.fov = [], [], fov
+        self.alt_pix = [a / self.dx for a in layer_alt]
+        for l, ((vx, vy), mg) in enumerate(zip(layer_wind, mags)):
+            sp = math.hypot(vx, vy)
+            c = vx / sp if sp > 0 else 1.0
+            s = vy / sp if sp > 0 else 0.0
+            self.speed.append(sp)
+            self.cos.append(c)
+            self.sin.append(s)
+            a, p = pupil_grids(self.n, c, s, self.W, mg)
+            self.along.append(a)
+            self.perp.append(p)
+            self.along_min.append(float(a.min()))
+            self.along_max.append(float(a.max()))
+            self.base[l] = int(math.floor(a.min() - fov[l])) - 2
+        self.innov_count = np.zeros(L, np.int64)
+
+    def seed(self, seed_rows):
+        for l in range(self.L):
+            k = self.spans[l]
+            self.buf[l, :k] = np.asarray(seed_rows[l], np.float64)[:k]
+            self.fill[l] = k
+
+    def _extrude_one(self, l, innovation):
+        if self.fill[l] == self.cap:
+            self.base[l], self.fill[l] = compact_row_ring(
+                self.buf[l], int(self.base[l]), int(self.fill[l]), self.m,
+                self.along_min[l] - self.fov[l] + self.travel[l])
+        f = int(self.fill[l])
+        z = self.buf[l, f - self.m:f].ravel()
+        beta = np.asarray(innovation(l, int(self.innov_count[l])), np.float64)
+        self.innov_count[l] += 1
+        self.buf[l, f] = self.A @ z + self.B[l] @ beta
+        self.fill[l] = f + 1
+
+    def rows_needed(self, l, travel):
+        top = int(math.ceil(self.along_max[l] + self.fov[l] + travel)) + 2
+        return max(0, top - int(self.base[l] + self.fill[l] - 1))
+
+    def set_time(self, t, innovation):
+        for l in range(self.L):
+            tr = self.speed[l] * t / self.dx
+            if tr < self.travel[l]:
+                raise ValueError("wind travel is monotonic; travel cannot decrease")
+            self.travel[l] = tr
+            top = int(math.ceil(self.along_max[l] + self.fov[l] + tr)) + 2
+            while self.base[l] + self.fill[l] - 1 < top:
+                self._extrude_one(l, innovation)
+
+    def shifts(self, thx=0.0, thy=0.0):
+        sa = np.empty(self.L)
+        sp = np.empty(self.L)
+        for l in range(self.L):
+            dxp, dyp = self.alt_pix[l] * thx, self.alt_pix[l] * thy
+            oa = dxp * self.cos[l] + dyp * self.sin[l]
+            op = -dxp * self.sin[l] + dyp * self.cos[l]
+            sa[l] = (self.travel[l] - self.base[l]) + oa
+            sp[l] = op
+        return sa, sp
+
+    def integrate(self, thx=0.0, thy=0.0, interp="cubic"):
+        sa, sp = self.shifts(thx, thy)
+        return readout(self.buf, self.fill, np.stack(self.along),
+                       np.stack(self.perp), sa, sp, interp)
+
+
+class OracleInfinite:
+    """infinite.py:120-348 -- single-layer extruder along axis 0, FP64."""
+
+    def __init__(self, n, dx, r0, L0, seed_screen, stencil_rows=2,
+                 interp="cubic", A=None, B=None):
+        self.n, self.m, self.interp = int(n), int(stencil_rows), interp
+        if A is None:
+            A, B = build_extrusion(self.n, self.m, dx, L0, r0)
+        self.A, self.B = np.asarray(A, float), np.asarray(B, float)
+        self.cap = self.n + self.m + 6
+        self.buf = np.zeros((self.cap, self.n))
+        self.buf[:self.n] = seed_screen
+        self.base, self.fill, self.travel = 0, self.n, 0.0
+        self.count = 0
+
+    def _advance_to(self, travel, innovation):
+        if travel < self.travel:
+            raise ValueError("wind travel is monotonic; travel cannot decrease")
+        self.travel = float(travel)
+        top = int(np.floor(self.travel)) + self.n + 2
+        while self.base + self.fill - 1 < top:
+            if self.fill == self.cap:
+                self.base, self.fill = compact_row_ring(
+                    self.buf, self.base, self.fill, self.m, self.travel)
+            z = self.buf[self.fill - self.m:self.fill].ravel()
+            beta = np.asarray(innovation(self.count), float)
+            self.count += 1
+            self.buf[self.fill] = self.A @ z + self.B @ beta
+            self.fill += 1
+        return self.sample()
+
+    def sample(self):
+        pos = (self.travel - self.base) + np.arange(self.n)
+        i0 = np.floor(pos).astype(np.int64)
+        t = (pos - i0)[:, None]
+        offs, ws = _taps(self.interp, t)
+        out = 0.0
+        for d, w in zip(offs, ws):
+            out = out + w * self.buf[np.clip(i0 + d, 0, self.fill - 1)]
+        return out

@@ -1,0 +1,294 @@
+"""Pin the FP64 oracle: golden vectors from the reference, closed forms,
+identities and brute force.  CPU only."""
+
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import config2_layer_tuples, load_golden
+
+
+# ---------------------------------------------------------------- filters
+def test_psd_closed_form_and_limits():
+    # 0.023 r0^-5/3 (f^2 + L0^-2)^(-11/6): check two hand-evaluated points.
+    r0, L0 = 0.15, 25.0
+    f2 = np.array([0.0, 4.0])
+    want = 0.023 * r0 ** (-5 / 3) * (f2 + 1 / 625.0) ** (-11 / 6)
+    np.testing.assert_allclose(O.psd(f2, r0, L0), want, rtol=1e-14)
+    # Kolmogorov: pure power law, slope -11/3 in |f|
+    p1, p2 = O.psd(np.array([1.0, 4.0]), 1.0, np.inf)
+    assert math.isclose(math.log(p2 / p1) / math.log(2.0), -11 / 3, rel_tol=1e-12)
+    # inner scale multiplies by exp(-f^2/fm^2), fm = 5.92/(2 pi l0)
+    fm = 5.92 / (2 * np.pi * 0.2)
+    assert math.isclose(O.psd(1.0, 1.0, np.inf, inner_scale=0.2),
+                        O.psd(1.0, 1.0, np.inf) * math.exp(-1 / fm ** 2), rel_tol=1e-13)
+
+
+def test_band_amplitude_flat_limit_and_centre_rule():
+    # A cell far from DC has a nearly flat PSD: amp -> sqrt(PSD(fc) * w^2).
+    fx, fy, w = 40.0, 13.0, 0.01
+    amp = O.band_amplitude(fx, fy, w, 0.15, 25.0)
+    flat = math.sqrt(O.psd(fx * fx + fy * fy, 0.15, 25.0)) * w
+    assert abs(amp / flat - 1) < 1e-5
+    # Near DC the integrated amplitude differs strongly from the centre rule.
+    amp0 = O.band_amplitude(0.125, 0.0, 0.125, 0.15, 25.0)
+    centre = math.sqrt(O.psd(0.125 ** 2, 0.15, 25.0)) * 0.125
+    assert abs(amp0 / centre - 1) > 0.01
+
+
+def test_filters_match_reference_golden():
+    g = load_golden("generate_n256_seed1.npz")
+    filt = O.build_filters(256, 0.03125, 0.15, 25.0, subharmonics=3)
+    idx = np.r_[0:9, 248:256]
+    np.testing.assert_allclose(filt["amplitude"][np.ix_(idx, idx)], g["amp_block"],
+                               rtol=1e-13, atol=0)
+    np.testing.assert_allclose(filt["amplitude"][5], g["amp_row5"], rtol=1e-13)
+    np.testing.assert_allclose(filt["sh_amp"], g["sh_amp"], rtol=1e-13)
+    np.testing.assert_allclose(filt["sh_basis"], g["sh_basis"], rtol=0, atol=1e-14)
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(4, 4), (8, 2), (3, 5), (16, 16)])
+def test_ifft2_unnormalised_equals_brute_force(shape):
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    n0, n1 = shape
+    j0 = np.arange(n0)
+    j1 = np.arange(n1)
+    F0 = np.exp(2j * np.pi * np.outer(j0, j0) / n0)
+    F1 = np.exp(2j * np.pi * np.outer(j1, j1) / n1)
+    np.testing.assert_allclose(O.ifft2_unnormalised(x), F0 @ x @ F1, atol=1e-11)
+
+
+# ---------------------------------------------------------------- generate
+def test_generate_matches_reference_golden():
+    g = load_golden("generate_n256_seed1.npz")
+    filt = O.build_filters(256, 0.03125, 0.15, 25.0, subharmonics=3)
+    rng = np.random.default_rng(1)
+    main, sh = O.draw_generate_noise(rng, 256, 64, 3)
+    scr = O.generate(filt, main, sh, 64)
+    np.testing.assert_allclose(scr[:2], g["screens01"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(scr[63, 100:132, 40:72], g["screen63_patch"], atol=1e-10)
+    np.testing.assert_allclose(scr.sum(axis=(1, 2)), g["sums"], rtol=1e-9, atol=1e-7)
+    np.testing.assert_allclose((scr * scr).sum(axis=(1, 2)), g["sumsq"], rtol=1e-11)
+
+
+def test_generate_pairs_real_and_imag_parts():
+    # screens[k] and screens[n_fft + k] are Re/Im of one coloured FFT
+    filt = O.build_filters(16, 0.1, 0.2, 10.0, subharmonics=0)
+    rng = np.random.default_rng(3)
+    main, sh = O.draw_generate_noise(rng, 16, 6, 0)
+    scr = O.generate(filt, main, sh, 6)
+    field = O.ifft2_unnormalised((main[0] + 1j * main[1]) * filt["amplitude"])
+    np.testing.assert_allclose(scr[:3], field.real, atol=1e-12)
+    np.testing.assert_allclose(scr[3:], field.imag, atol=1e-12)
+
+
+def test_subharmonic_field_is_zero_mean_and_matches_direct_sum():
+    rng = np.random.default_rng(5)
+    filt = O.build_filters(12, 0.2, 0.1, 30.0, subharmonics=2)
+    c = rng.standard_normal((2, 3, 3)) + 1j * rng.standard_normal((2, 3, 3))
+    low = O.subharmonic_field(c, filt["sh_basis"])
+    assert abs(low.mean()) < 1e-12
+    x = (np.arange(12) - 6.0) * 0.2
+    direct = np.zeros((12, 12))
+    for p in range(2):
+        fp = filt["sh_freq"][p]
+        for a in range(3):
+            for b in range(3):
+                direct += (c[p, a, b] * np.exp(2j * np.pi * (fp[a] * x[:, None]
+                                                              + fp[b] * x[None, :]))).real
+    np.testing.assert_allclose(low, direct - direct.mean(), atol=1e-11)
+
+
+# ---------------------------------------------------------------- spectral
+@pytest.fixture(scope="module")
+def atm64():
+    return O.OracleAtmosphere(config2_layer_tuples(), r0=0.15, diameter=8.0, n=64,
+                              L0=25.0, field_of_view=30.0, dtype="float64", seed=2)
+
+
+def test_atmosphere_seed_tree_matches_reference(atm64):
+    g = load_golden("atmosphere_n64_seed2.npz")
+    assert atm64.ns == int(g["n_screen"])
+    np.testing.assert_allclose(np.abs(atm64.spectra).sum(axis=(1, 2)),
+                               g["spectra_abs_sum"], rtol=1e-12)
+    np.testing.assert_allclose(atm64.sh_coeffs, g["sh_coeffs"], rtol=1e-12, atol=0)
+
+
+def test_spectral_frames_match_reference_golden(atm64):
+    g = load_golden("atmosphere_n64_seed2.npz")
+    for t, want in zip(g["times"], g["frames"]):
+        np.testing.assert_allclose(atm64.opd(float(t)), want, rtol=0,
+                                   atol=1e-12 * np.abs(want).max())
+    got = atm64.opd(0.01, directions=[tuple(d) for d in g["dirs"]])
+    np.testing.assert_allclose(got, g["dframes"], atol=1e-12 * np.abs(got).max())
+    nofov = O.OracleAtmosphere(config2_layer_tuples(), r0=0.15, diameter=8.0, n=64,
+                               L0=25.0, dtype="float64", seed=2)
+    want = g["frame_nofov_t002"]
+    np.testing.assert_allclose(nofov.opd(0.02), want, atol=1e-12 * np.abs(want).max())
+
+
+def test_integer_pixel_shift_is_a_roll():
+    # atmosphere test_frozen_flow_shifts_screen_by_wind: +s rolls by -k.
+    filt = O.build_filters(32, 0.25, 0.15, 25.0, subharmonics=0)
+    rng = np.random.default_rng(9)
+    spec, cf = O.draw_flow(rng, filt)
+    base = O.flow_translate(spec, cf, filt, 0.0, 0.0)
+    moved = O.flow_translate(spec, cf, filt, 5 * 0.25, -3 * 0.25)
+    np.testing.assert_allclose(moved, np.roll(np.roll(base, -5, 0), 3, 1), atol=1e-10)
+
+
+def test_layer_sum_equals_sum_of_translates(atm64):
+    sx, sy = atm64.displacements(0.3)
+    total = O.integrate(atm64.spectra, atm64.sh_coeffs, atm64.f, atm64.sh_freq,
+                        atm64.sh_basis, sx, sy)
+    parts = sum(O.flow_translate(atm64.spectra[l], atm64.sh_coeffs[:, l],
+                                 atm64.filters[l], sx[l], sy[l])
+                for l in range(len(sx)))
+    np.testing.assert_allclose(total, parts, rtol=1e-10, atol=1e-10)
+
+
+def test_sample_groups_match_reference_golden():
+    g = load_golden("sample_n64_seed4.npz")
+    layers = [(0, 0.4, 10.0, 0.0, 25.0), (1000, 0.2, 10.0, 0.0, 50.0),
+              (5000, 0.3, 10.0, 0.0, 25.0), (10000, 0.1, 10.0, 0.0, 50.0)]
+    atm = O.OracleAtmosphere(layers, r0=0.15, diameter=8.0, n=64, dtype="float64", seed=4)
+    assert len(atm.sample_groups) == 2
+    got = atm.sample(4)
+    np.testing.assert_allclose(got, g["sample"], rtol=0, atol=1e-12 * np.abs(got).max())
+
+
+def test_r0_scaling_is_exact_power():
+    # amplitude ~ r0^(-5/6) for every cell, integrated or not
+    a = O.build_filters(32, 0.1, 0.1, 25.0, subharmonics=2)
+    b = O.build_filters(32, 0.1, 0.3, 25.0, subharmonics=2)
+    k = (0.3 / 0.1) ** (5 / 6)
+    np.testing.assert_allclose(a["amplitude"], b["amplitude"] * k, rtol=1e-12)
+    np.testing.assert_allclose(a["sh_amp"], b["sh_amp"] * k, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- covariance
+def test_phase_covariance_variance_and_decay():
+    r0, L0 = 0.15, 25.0
+    c0 = O.phase_covariance(np.array([0.0]), r0, L0)[0]
+    assert c0 == pytest.approx(0.0863 * (L0 / r0) ** (5 / 3), rel=2e-3)
+    c = O.phase_covariance(np.linspace(0, 50, 200), r0, L0)
+    assert np.all(np.diff(c) < 0)
+    # continuity at r -> 0
+    assert O.phase_covariance(np.array([1e-9]), r0, L0)[0] == pytest.approx(c0, rel=1e-5)
+
+
+def test_extrusion_matrices_identities_and_golden():
+    g = load_golden("extrusion_W32.npz")
+    A, B = O.build_extrusion(32, 2, 0.125, 25.0)
+    np.testing.assert_allclose(A, g["A"], rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(B, g["B"], rtol=1e-8, atol=1e-10)
+    # A C_zz = C_xz and B B^T = C_xx - A C_xz^T
+    yy, xx = np.meshgrid(np.arange(2.0), np.arange(32.0), indexing="ij")
+    zx, zy = xx.ravel(), yy.ravel()
+    nx, ny = np.arange(32.0), np.full(32, 2.0)
+    cov = lambda ax, ay, bx, by: O.phase_covariance(
+        np.hypot(ax[:, None] - bx[None], ay[:, None] - by[None]) * 0.125, 1.0, 25.0)
+    czz, cxz, cxx = cov(zx, zy, zx, zy), cov(nx, ny, zx, zy), cov(nx, ny, nx, ny)
+    np.testing.assert_allclose(A @ czz, cxz, atol=1e-6 * np.abs(cxz).max())
+    np.testing.assert_allclose(B @ B.T, cxx - A @ cxz.T, atol=1e-8 * np.abs(cxx).max())
+
+
+# ---------------------------------------------------------------- extruder
+def _reference_like_extruder(interp_seeds=(11, 12)):
+    """Oracle extruder fed with the reference's own seed screens/innovations."""
+    kw = dict(n=32, dx=0.25, layer_r0=[0.2, 0.35], layer_L0=[25.0, 25.0],
+              layer_wind=[(6.0, 2.5), (-3.0, 11.0)], layer_alt=[0.0, 8000.0],
+              fov_pix=[0.0, 3.0])
+    ext = O.OracleExtruder(**kw)
+    rngs = [np.random.default_rng(s) for s in interp_seeds]
+    seeds = []
+    for l, rng in enumerate(rngs):
+        filt = O.build_filters(ext.W, 0.25, kw["layer_r0"][l], 25.0, subharmonics=8)
+        main, sh = O.draw_generate_noise(rng, ext.W, 1, 8)
+        seeds.append(O.generate(filt, main, sh, 1)[0])
+    ext.seed(seeds)
+    innov = lambda l, k: rngs[l].standard_normal(ext.W)
+    return ext, innov
+
+
+@pytest.mark.parametrize("interp", ["linear", "cubic", "lanczos"])
+def test_extruder_readout_matches_reference_golden(interp):
+    g = load_golden("extrude_n32.npz")[interp]
+    ext, innov = _reference_like_extruder()
+    k = 0
+    for t in (0.0, 0.01, 0.037, 0.4):
+        ext.set_time(t, innov)
+        for th in ((0.0, 0.0), (2e-5, -1e-5)):
+            got = ext.integrate(*th, interp=interp)
+            np.testing.assert_allclose(got, g[k], rtol=0, atol=1e-9 * np.abs(g[k]).max())
+            k += 1
+
+
+def test_extruder_integer_travel_is_exact_translation():
+    # Along-wind integer travel shifts the readout by whole rows (wind along axis 0).
+    ext = O.OracleExtruder(n=16, dx=0.5, layer_r0=[0.2], layer_L0=[20.0],
+                           layer_wind=[(10.0, 0.0)], layer_alt=[0.0])
+    rng = np.random.default_rng(1)
+    ext.seed([rng.standard_normal((ext.spans[0], ext.W))])
+    innov = lambda l, k: rng.standard_normal(ext.W)
+    ext.set_time(0.0, innov)
+    a = ext.integrate()
+    ext.set_time(3 * 0.5 / 10.0, innov)  # exactly 3 pixels
+    b = ext.integrate()
+    np.testing.assert_allclose(b[:-3], a[3:], atol=1e-12)
+
+
+def test_extruder_big_jump_equals_small_steps():
+    def run(times):
+        ext = O.OracleExtruder(n=16, dx=0.5, layer_r0=[0.2, 0.3], layer_L0=[20.0, 20.0],
+                               layer_wind=[(7.0, 3.0), (-2.0, -9.0)], layer_alt=[0.0, 3000.0])
+        rng = np.random.default_rng(4)
+        ext.seed([rng.standard_normal((s, ext.W)) for s in ext.spans])
+        streams = [np.random.default_rng(100 + l) for l in range(2)]
+        innov = lambda l, k: streams[l].standard_normal(ext.W)
+        for t in times:
+            ext.set_time(t, innov)
+        return ext.integrate()
+    # The recurrence is identical; only compaction bookkeeping (the ring base)
+    # differs, which moves the last bits of the fractional readout position.
+    np.testing.assert_allclose(run([3.0]), run(np.linspace(0, 3.0, 37)),
+                               rtol=0, atol=1e-12 * 40)
+
+
+def test_readout_weights_partition_unity():
+    t = np.linspace(0, 1, 17, endpoint=False)
+    assert np.allclose(sum(O.catmull_rom_weights(t)), 1.0)
+    assert np.allclose(sum(O.lanczos_weights(t)[1]), 1.0)
+
+
+@pytest.mark.parametrize("interp", ["cubic", "linear"])
+def test_infinite_matches_reference_golden(interp):
+    g = load_golden("infinite_n32_seed5.npz")[interp]
+    rng = np.random.default_rng(5)
+    filt = O.build_filters(32, 0.1, 0.15, 25.0, subharmonics=8)
+    main, sh = O.draw_generate_noise(rng, 32, 1, 8)
+    seed = O.generate(filt, main, sh, 1)[0]
+    inf = O.OracleInfinite(32, 0.1, 0.15, 25.0, seed, interp=interp)
+    innov = lambda k: rng.standard_normal(32)
+    seq = [inf._advance_to(0.0, innov)]
+    for tr in (0.3, 1.3, 4.0, 44.0):
+        seq.append(inf._advance_to(tr, innov))
+    np.testing.assert_allclose(np.stack(seq), g, atol=1e-10 * np.abs(g).max())
+
+
+def test_structure_function_of_oracle_screens_matches_theory():
+    # Kolmogorov ensemble, D(r) = 6.88 (r/r0)^(5/3) within 15% (fourier tests)
+    filt = O.build_filters(128, 0.02, 0.1, np.inf, subharmonics=8)
+    rng = np.random.default_rng(7)
+    main, sh = O.draw_generate_noise(rng, 128, 16, 8)
+    scr = O.generate(filt, main, sh, 16)
+    r, d = O.structure_function(scr, 0.02, max_separation=20)
+    ratio = d[3:] / (6.88 * (r[3:] / 0.1) ** (5 / 3))
+    assert np.all((ratio > 0.8) & (ratio < 1.2))
