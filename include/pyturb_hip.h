@@ -1,0 +1,104 @@
+/*
+ * pyturb_hip.h -- C ABI of the MI355X (gfx950) phase-screen library.
+ *
+ * Every array argument is a DEVICE pointer (HBM) owned by the caller; sizes
+ * are plain integers; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Calls are asynchronous on `stream` and return a status
+ * code; on failure pt_last_error() returns a message.  No entry point
+ * allocates device memory: workspaces are caller-owned and their sizes are
+ * stated below.  Calls sharing a workspace on one stream serialise
+ * correctly; concurrent calls must not share a workspace.
+ *
+ * Complex arrays are interleaved (re, im) float32 ("c64") or float64
+ * ("c128").  Transforms are the reference's unnormalised inverse DFT,
+ * ifft2(x) * n0 * n1 (sign +1); sign -1 is the forward DFT.
+ *
+ * Each entry point replaces the array-module call chain the reference runs
+ * for the same step; the replaced code is cited (paths are relative to the
+ * reference checkout).
+ */
+#ifndef PYTURB_HIP_H
+#define PYTURB_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+#define PT_OK 0
+#define PT_ERR_ARG 1         /* invalid argument (shape, count, NULL)      */
+#define PT_ERR_HIP 2         /* HIP runtime error (launch, copy)           */
+#define PT_ERR_UNSUPPORTED 3 /* size outside what the kernels implement    */
+
+/* Last error message of the calling thread ("" if none). */
+const char* pt_last_error(void);
+
+/* PT_ABI_VERSION of the loaded library. */
+int pt_abi_version(void);
+
+/*
+ * Unnormalised 2-D DFT over the last two axes of `in` (batch, n0, n1) c64,
+ * written to `out` (same shape; may equal `in`).  Powers of two from 32 to
+ * 2048 per axis run the LDS Stockham passes; other sizes run the direct
+ * DFT path and then need `work` of batch*n0*n1 c64 (else may be NULL).
+ * Replaces: src/pyturb/fourier.py:259, src/pyturb/atmosphere.py:918,978,
+ *           src/pyturb/flow.py:110 (xp.fft.ifft2 * n^2).
+ */
+int pt_fft2(const void* in, void* out, void* work, int batch, int n0, int n1,
+            int sign, void* stream);
+
+/*
+ * Fused Monte-Carlo screen generation, one PhaseScreen.generate(count) call
+ * on injected noise (src/pyturb/fourier.py:233-282, and the crop/scale of
+ * src/pyturb/atmosphere.py:1213-1219 when summing L0 groups).
+ *
+ *   noise_re, noise_im : (n_fft, n, n) f32, n_fft = (count+1)/2, the real and
+ *                        imaginary white-noise planes (fourier.py:257-258)
+ *   amplitude          : (n, n) f32 PSD filter (fourier.py:174-191)
+ *   sh_noise           : (levels, 2, count, 3, 3) f32 subharmonic noise
+ *                        (fourier.py:271-273), NULL when levels == 0
+ *   sh_amp             : (levels, 3, 3) f64 subharmonic amplitudes
+ *   screen_scale       : (count,) f32 per-screen factor or NULL; output is
+ *                        (screen + low) * scale * screen_scale[s]
+ *   out                : (count, n_out, n_out) f32; screen s pixel (x, y) is
+ *                        full-grid pixel (x + crop_offset, y + crop_offset);
+ *                        accumulate != 0 adds into out
+ *   work               : n_fft * n * n c64
+ * Screen k < n_fft is the real part of FFT k, screen n_fft + k its
+ * imaginary part (fourier.py:260).
+ */
+int pt_generate(const float* noise_re, const float* noise_im,
+                const float* amplitude, const float* sh_noise,
+                const double* sh_amp, int levels, int count, int n,
+                int crop_offset, int n_out, float scale,
+                const float* screen_scale, int accumulate, float* out,
+                void* work, void* stream);
+
+/*
+ * Spectral frozen-flow frames (src/pyturb/atmosphere.py:882-938 on-axis,
+ * :940-998 for directions; src/pyturb/flow.py:96-129 for one layer).
+ *
+ *   spectra   : (layers, n, n) c64 fixed layer spectra (atmosphere.py:545)
+ *   disp      : (frames, layers, 2) f64 displacement (s_x, s_y) in metres of
+ *               each layer for each output frame (v*t + h*tan(theta))
+ *   sh_coeffs : (levels, layers, 3, 3) c128 subharmonic coefficients, NULL
+ *               when levels == 0
+ *   out       : (frames, n_out, n_out) f32,
+ *               out = scale * Re(ifft2(sum_l S_l P_x P_y) * n^2 + low)
+ *               cropped at crop_offset
+ *   work      : frames * n * n c64
+ * Shift-theorem phases are reduced exactly (32-bit fixed-point cycles), so
+ * large travel keeps full FP32 phase accuracy.
+ */
+int pt_spectral_frames(const void* spectra, int layers, int n,
+                       double pixel_scale, const double* disp, int frames,
+                       const double* sh_coeffs, int levels, int crop_offset,
+                       int n_out, float scale, float* out, void* work,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYTURB_HIP_H */

@@ -1,0 +1,125 @@
+"""ctypes binding of ``libpyturb_hip.so`` and device-array plumbing.
+
+The library is the only compute path of this package: every hot call goes
+through one of the ``pt_*`` C entry points declared in
+``include/pyturb_hip.h``.  If the shared object is missing or no ROCm device
+is visible, the calls raise -- there is no CPU fallback.
+
+PyTorch supplies device memory and the current HIP stream; tensors cross the
+boundary as raw device pointers.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpyturb_hip.so")
+
+PT_OK = 0
+PT_ERR_ARG = 1
+PT_ERR_HIP = 2
+PT_ERR_UNSUPPORTED = 3
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_f = ctypes.c_float
+
+# name -> (restype, argtypes); mirrors include/pyturb_hip.h
+SIGNATURES = {
+    "pt_last_error": (ctypes.c_char_p, []),
+    "pt_abi_version": (_i, []),
+    "pt_fft2": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "pt_generate": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp]),
+    "pt_spectral_frames": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class HipError(RuntimeError):
+    """A HIP-library call failed (launch error, unsupported size, ...)."""
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Open the shared library and bind every exported entry point."""
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `python -m pyturb_amd.build` "
+            "(hipcc, --offload-arch=gfx950). This package has no CPU fallback."
+        )
+    so = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(so, name)
+        fn.restype = res
+        fn.argtypes = args
+    return so
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc == PT_OK:
+        return
+    msg = lib().pt_last_error().decode(errors="replace")
+    if rc == PT_ERR_ARG:
+        raise ValueError(f"{what}: {msg}")
+    if rc == PT_ERR_UNSUPPORTED:
+        raise NotImplementedError(f"{what}: {msg}")
+    raise HipError(f"{what}: {msg}")
+
+
+# ---------------------------------------------------------------------------
+# torch plumbing
+# ---------------------------------------------------------------------------
+def torch():
+    import torch as _t
+
+    return _t
+
+
+def device(index: int = 0):
+    t = torch()
+    if not t.cuda.is_available():
+        raise RuntimeError(
+            "no ROCm GPU is visible: the MI355X backend needs a gfx950 device "
+            "(this package has no CPU compute path)"
+        )
+    return t.device("cuda", index)
+
+
+def stream_ptr() -> int:
+    return torch().cuda.current_stream().cuda_stream
+
+
+def ptr(tensor) -> int:
+    """Device pointer of a contiguous device tensor (None -> NULL)."""
+    if tensor is None:
+        return None
+    if not tensor.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not tensor.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return tensor.data_ptr()
+
+
+def to_device(array, dtype=None, dev=None):
+    """Host array -> contiguous device tensor (one H2D copy)."""
+    t = torch()
+    a = np.ascontiguousarray(array if dtype is None else np.asarray(array, dtype=dtype))
+    return t.from_numpy(a).to(dev or device(), non_blocking=False)
+
+
+def empty(shape, dtype, dev=None):
+    t = torch()
+    return t.empty(tuple(int(s) for s in shape), dtype=dtype, device=dev or device())

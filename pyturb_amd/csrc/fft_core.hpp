@@ -1,0 +1,216 @@
+// LDS-resident Stockham FFT engine for gfx950 (CDNA4, wave64).
+//
+// One workgroup transforms M independent complex sequences of length N
+// (N a power of two, 32 <= N <= 2048).  Every thread owns exactly eight
+// complex values per stage: one radix-8 butterfly, two radix-4 or four
+// radix-2 butterflies, so M * N == 8 * T for a T-thread workgroup.
+//
+// The first stage reads its inputs through a caller-supplied prologue
+// functor straight from HBM (so layer sums, PSD filtering and noise
+// colouring fuse into the load), and the last stage hands its outputs to an
+// epilogue functor that writes HBM (so real/imag split, subharmonic add,
+// scaling and cropping fuse into the store).  Intermediate stages exchange
+// through LDS in place: read, barrier, write, barrier.
+//
+// Two data layouts are supported:
+//   ROWS  (COLS=false): sequence m occupies LDS[m*N + i]; butterfly b of a
+//         stage maps to (m = b / (N/R), j = b % (N/R)), so consecutive lanes
+//         walk along a sequence -> coalesced row access in HBM.
+//   COLS  (COLS=true):  element i of sequence m sits at LDS[i*M + m];
+//         butterfly b maps to (m = b % M, j = b / M), so consecutive lanes
+//         walk across M adjacent columns -> M*8-byte segments in HBM.
+//
+// Stockham autosort step (radix R, stride Ns = product of earlier radices):
+//   v[r] = x[j + r*N/R] * W_{Ns*R}^{(j mod Ns)*r},  v = DFT_R(v),
+//   y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns] = v[r].
+// Twiddles come from an N-entry table tw[t] = exp(S*2*pi*i*t/N) in LDS.
+// S = +1 is the unnormalised inverse DFT (the reference's ifft2 * n^2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pt {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by S*i
+template <int S>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+  return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int S>
+__device__ __forceinline__ void dft2(float2& a0, float2& a1) {
+  float2 t = a0;
+  a0 = cadd(t, a1);
+  a1 = csub(t, a1);
+}
+
+template <int S>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <int S>
+__device__ __forceinline__ void dft8(float2* v) {
+  // even/odd split into two 4-point DFTs, recombined with W_8^k
+  dft4<S>(v[0], v[2], v[4], v[6]);
+  dft4<S>(v[1], v[3], v[5], v[7]);
+  const float c = 0.70710678118654752440f;
+  float2 o1 = make_float2(c * (v[3].x - S * v[3].y), c * (S * v[3].x + v[3].y));
+  float2 o2 = mul_si<S>(v[5]);
+  float2 o3 = make_float2(c * (-v[7].x - S * v[7].y), c * (S * v[7].x - v[7].y));
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+template <int R, int S>
+__device__ __forceinline__ void dftR(float2* v) {
+  if constexpr (R == 8) {
+    dft8<S>(v);
+  } else if constexpr (R == 4) {
+    dft4<S>(v[0], v[1], v[2], v[3]);
+  } else {
+    dft2<S>(v[0], v[1]);
+  }
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+template <int N>
+struct Plan {
+  static_assert((N & (N - 1)) == 0 && N >= 32 && N <= 2048, "power-of-two 32..2048");
+  static constexpr int LOG = ilog2(N);
+  static constexpr int NFULL = LOG / 3;
+  static constexpr int REM = LOG % 3;
+  static constexpr int NST = NFULL + (REM ? 1 : 0);
+  static constexpr int radix(int s) { return s < NFULL ? 8 : (1 << REM); }
+  static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * radix(s - 1); }
+};
+
+// Twiddle table exp(S*2*pi*i*t/N), t < N, written cooperatively into LDS.
+template <int N, int S>
+__device__ __forceinline__ void init_twiddles(float2* tw, int tid, int nthreads) {
+  for (int t = tid; t < N; t += nthreads) {
+    float s, c;
+    sincospif(2.0f * (float)t / (float)N, &s, &c);  // exact argument for power-of-two N
+    tw[t] = make_float2(c, S * s);
+  }
+}
+
+template <int N, int T, int M, bool COLS>
+struct Layout {
+  static_assert(M * N == 8 * T, "eight values per thread");
+  __device__ static __forceinline__ int addr(int m, int i) { return COLS ? i * M + m : m * N + i; }
+};
+
+// One Stockham stage ST.  For ST == 0 inputs come from `load(m, i)`; for the
+// last stage outputs go to `store(m, i, v)`; otherwise LDS.
+template <int N, int T, int M, bool COLS, int S, int ST, class Load, class Store>
+__device__ __forceinline__ void fft_stage(float2* lds, const float2* tw, int tid, Load& load, Store& store) {
+  using P = Plan<N>;
+  using Lo = Layout<N, T, M, COLS>;
+  constexpr int R = P::radix(ST);
+  constexpr int NS = P::ns(ST);
+  constexpr int Q = 8 / R;
+  constexpr int NR = N / R;
+  constexpr bool LAST = (ST == P::NST - 1);
+  float2 v[8];
+  int mq[Q], jq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    int b = q * T + tid;
+    mq[q] = COLS ? (b % M) : (b / NR);
+    jq[q] = COLS ? (b / M) : (b % NR);
+  }
+  if constexpr (ST == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q * R + r] = load(mq[q], jq[q] + r * NR);
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q * R + r] = lds[Lo::addr(mq[q], jq[q] + r * NR)];
+    __syncthreads();  // every read of this stage precedes any write
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if constexpr (NS > 1) {
+      const int k = jq[q] % NS;
+      constexpr int STEP = N / (NS * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q * R + r] = cmul(v[q * R + r], tw[k * r * STEP]);
+    }
+    dftR<R, S>(&v[q * R]);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = jq[q] % NS;
+    const int d = (jq[q] / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if constexpr (LAST) {
+        store(mq[q], d + r * NS, v[q * R + r]);
+      } else {
+        lds[Lo::addr(mq[q], d + r * NS)] = v[q * R + r];
+      }
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+    fft_stage<N, T, M, COLS, S, ST + 1>(lds, tw, tid, load, store);
+  }
+}
+
+template <int N, int T, int M, bool COLS, int S, class Load, class Store>
+__device__ __forceinline__ void wg_fft(float2* lds, const float2* tw, int tid, Load& load, Store& store) {
+  fft_stage<N, T, M, COLS, S, 0>(lds, tw, tid, load, store);
+}
+
+// ---------------------------------------------------------------------------
+// Shift-theorem phasors with exact argument reduction.  A displacement s
+// [m] on an N-point grid of pitch dx gives, for signed frequency index k,
+// the phase 2*pi*k*u with u = s/(N*dx) cycles.  The host-independent device
+// code keeps u as a 32-bit fixed-point fraction of a cycle, so k*U wraps
+// modulo 2^32 exactly and only the final conversion rounds.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t cycles_fixed(double u) {
+  double f = u - floor(u);                   // [0, 1)
+  double s = f * 4294967296.0;               // 2^32
+  uint64_t q = (uint64_t)llrint(s);
+  return (uint32_t)q;                        // 2^32 wraps to 0
+}
+
+__device__ __forceinline__ float2 phasor_fixed(uint32_t phase) {
+  // phase * 2^-32 cycles -> signed half-cycles in [-1, 1)
+  float h = (float)(int32_t)phase * 4.656612873077393e-10f;  // 2^-31
+  float s, c;
+  sincospif(h, &s, &c);
+  return make_float2(c, s);
+}
+
+__device__ __forceinline__ int signed_freq(int k, int n) { return k < (n + 1) / 2 ? k : k - n; }
+
+}  // namespace pt

@@ -1,0 +1,536 @@
+// Fused FFT passes for Monte-Carlo screens (PhaseScreen.generate) and the
+// spectral frozen-flow frame engine, gfx950.
+//
+// A 2-D transform of an N x N grid is two passes over HBM:
+//   pass A (rows):    256-thread workgroups, M = 2048/N rows each; the
+//                     prologue builds the spectrum on the fly (noise x PSD
+//                     amplitude, or the phasor-weighted layer sum) and the
+//                     rows are transformed along axis 1 in LDS;
+//   pass B (columns): up to 1024-thread workgroups, a tile of M adjacent
+//                     columns transformed along axis 0 in LDS; the epilogue
+//                     takes the real/imag parts, adds the subharmonic field,
+//                     scales, crops and writes float32 screens.
+// The intermediate (one c64 grid per transform) is the only extra traffic.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "common.hpp"
+#include "fft_core.hpp"
+
+namespace pt {
+
+template <int N>
+struct RowsCfg {
+  static constexpr int T = 256;
+  static constexpr int M = 8 * T / N;
+};
+template <int N>
+struct ColsCfg {
+  static constexpr int T = (N * N / 8 < 1024) ? N * N / 8 : 1024;
+  static constexpr int M = 8 * T / N;
+};
+
+// ----------------------------------------------------------------------------
+// plain 2-D transform passes (pt_fft2)
+// ----------------------------------------------------------------------------
+template <int N, int S>
+__global__ __launch_bounds__(256) void k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, int n1) {
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
+  __shared__ float2 lds[M * N];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  init_twiddles<N, S>(tw, tid, T);
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.y * n1 * N + (size_t)blockIdx.x * M * N;
+  auto load = [&](int m, int i) { return in[base + (size_t)m * N + i]; };
+  auto store = [&](int m, int i, float2 v) { out[base + (size_t)m * N + i] = v; };
+  wg_fft<N, T, M, false, S>(lds, tw, tid, load, store);
+}
+
+template <int N, int S>
+__global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __restrict__ in, float2* __restrict__ out, int n1) {
+  constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
+  __shared__ float2 lds[M * N];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  init_twiddles<N, S>(tw, tid, T);
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.y * N * n1 + (size_t)blockIdx.x * M;
+  auto load = [&](int m, int i) { return in[base + (size_t)i * n1 + m]; };
+  auto store = [&](int m, int i, float2 v) { out[base + (size_t)i * n1 + m] = v; };
+  wg_fft<N, T, M, true, S>(lds, tw, tid, load, store);
+}
+
+// ----------------------------------------------------------------------------
+// subharmonic epilogue tables for a column tile
+// ----------------------------------------------------------------------------
+// Dynamic LDS of a column pass: U[P][N] basis values (when it fits), then
+// H[NT][M][P] float2, KP[NT][M][P] float, K[NT][M] float, MEAN[NT][P] double.
+struct ShLds {
+  float2* U;
+  float2* H;
+  float* KP;
+  float* K;
+  double* MEAN;
+  bool u_in_lds;
+};
+
+__host__ __device__ inline size_t sh_lds_bytes(int n, int P, int NT, int M, bool u_in_lds) {
+  if (P == 0) return 0;
+  size_t b = u_in_lds ? (size_t)n * P * 8 : 0;
+  b += (size_t)NT * M * P * 8 + (size_t)NT * M * P * 4 + (size_t)NT * M * 4;
+  b = (b + 7) & ~(size_t)7;
+  b += (size_t)NT * P * 8;
+  return (b + 15) & ~(size_t)15;
+}
+
+__device__ inline ShLds sh_carve(unsigned char* dyn, int n, int P, int NT, int M, bool u_in_lds) {
+  ShLds s;
+  size_t o = 0;
+  s.u_in_lds = u_in_lds;
+  s.U = (float2*)(dyn + o);
+  if (u_in_lds) o += (size_t)n * P * 8;
+  s.H = (float2*)(dyn + o);
+  o += (size_t)NT * M * P * 8;
+  s.KP = (float*)(dyn + o);
+  o += (size_t)NT * M * P * 4;
+  s.K = (float*)(dyn + o);
+  o += (size_t)NT * M * 4;
+  o = (o + 7) & ~(size_t)7;
+  s.MEAN = (double*)(dyn + o);
+  return s;
+}
+
+// Fill U (if in LDS), H, KP and MEAN for NT coefficient sets over columns
+// c0..c0+M-1.  `block(t, p, C)` writes the 3x3 block of set t, level p.
+// `valid(t)` says whether set t is used.  Ends with K filled; callers
+// __syncthreads() before reading.
+template <class Block, class Valid>
+__device__ inline void sh_setup(ShLds& s, int n, int P, int NT, int M, int c0, int tid, int nthreads,
+                                Block& block, Valid& valid) {
+  if (s.u_in_lds) {
+    for (int idx = tid; idx < P * n; idx += nthreads) s.U[idx] = sh_basis_e(idx / n, idx % n, n);
+  }
+  const int ntask = NT * M * P;
+  for (int task = tid; task < ntask + NT * P; task += nthreads) {
+    if (task < ntask) {
+      const int p = task % P;
+      const int m = (task / P) % M;
+      const int t = task / (P * M);
+      if (!valid(t)) continue;
+      double2 C[9];
+      block(t, p, C);
+      int y = c0 + m;
+      if (y >= n) y = n - 1;  // tile overhang (never stored)
+      double2 H;
+      double g1;
+      sh_column(C, sh_basis_e(p, y, n), H, g1);
+      s.H[(t * M + m) * P + p] = make_float2((float)H.x, (float)H.y);
+      s.KP[(t * M + m) * P + p] = (float)g1;
+    } else {
+      const int q = task - ntask;
+      const int p = q % P, t = q / P;
+      if (!valid(t)) continue;
+      double2 C[9];
+      block(t, p, C);
+      s.MEAN[t * P + p] = sh_block_mean(C, sh_mean_e(p, n));
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < NT * M; q += nthreads) {
+    const int t = q / M;
+    if (!valid(t)) continue;
+    double acc = 0.0;
+    for (int p = 0; p < P; ++p) acc += (double)s.KP[q * P + p] - s.MEAN[t * P + p];
+    s.K[q] = (float)acc;
+  }
+}
+
+__device__ __forceinline__ float sh_eval(const ShLds& s, int t, int m, int M, int x, int n, int P) {
+  float acc = s.K[t * M + m];
+  const float2* H = s.H + (size_t)(t * M + m) * P;
+  for (int p = 0; p < P; ++p) {
+    float2 e = s.u_in_lds ? s.U[p * n + x] : sh_basis_e(p, x, n);
+    float2 h = H[p];
+    acc += e.x * h.x - e.y * h.y;
+  }
+  return acc;
+}
+
+// ----------------------------------------------------------------------------
+// PhaseScreen.generate
+// ----------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre, const float* __restrict__ nim,
+                                                  const float* __restrict__ amp, float2* __restrict__ work) {
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
+  __shared__ float2 lds[M * N];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  init_twiddles<N, 1>(tw, tid, T);
+  __syncthreads();
+  const size_t plane = (size_t)N * N;
+  const size_t b = blockIdx.y;
+  const size_t r0 = (size_t)blockIdx.x * M * N;
+  const float* re = nre + b * plane + r0;
+  const float* im = nim + b * plane + r0;
+  const float* am = amp + r0;
+  float2* out = work + b * plane + r0;
+  auto load = [&](int m, int i) {
+    const size_t o = (size_t)m * N + i;
+    const float a = am[o];
+    return make_float2(re[o] * a, im[o] * a);
+  };
+  auto store = [&](int m, int i, float2 v) { out[(size_t)m * N + i] = v; };
+  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
+}
+
+struct GenArgs {
+  const float2* work;
+  float* out;
+  const float* sh_noise;
+  const double* sh_amp;
+  const float* screen_scale;
+  int count, n_fft, levels, off, n_out, accumulate, u_in_lds;
+  float scale;
+};
+
+template <int N>
+__global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
+  constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
+  __shared__ float2 lds[M * N];
+  __shared__ float2 tw[N];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  const int tid = threadIdx.x;
+  const int b = blockIdx.y;
+  const int c0 = blockIdx.x * M;
+  if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;  // tile entirely outside the crop
+  const int P = a.levels;
+  const int s0 = b, s1 = b + a.n_fft;
+  const bool has1 = s1 < a.count;
+  ShLds sh = sh_carve(dyn, N, P, 2, M, a.u_in_lds != 0);
+  init_twiddles<N, 1>(tw, tid, T);
+  if (P > 0) {
+    auto valid = [&](int t) { return t == 0 || has1; };
+    auto block = [&](int t, int p, double2* C) {
+      const int s = t == 0 ? s0 : s1;
+      const float* re = a.sh_noise + ((size_t)(p * 2 + 0) * a.count + s) * 9;
+      const float* im = a.sh_noise + ((size_t)(p * 2 + 1) * a.count + s) * 9;
+      for (int k = 0; k < 9; ++k) {
+        const double am = a.sh_amp[p * 9 + k];
+        C[k] = make_double2((double)re[k] * am, (double)im[k] * am);
+      }
+    };
+    sh_setup(sh, N, P, 2, M, c0, tid, T, block, valid);
+  }
+  __syncthreads();
+  const size_t plane = (size_t)N * N;
+  const float2* src = a.work + (size_t)b * plane + c0;
+  const size_t oplane = (size_t)a.n_out * a.n_out;
+  const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
+  const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
+  auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
+  auto store = [&](int m, int i, float2 v) {
+    const int x = i - a.off, y = c0 + m - a.off;
+    if ((unsigned)x >= (unsigned)a.n_out || (unsigned)y >= (unsigned)a.n_out) return;
+    float l0 = 0.0f, l1 = 0.0f;
+    if (P > 0) {
+      l0 = sh_eval(sh, 0, m, M, i, N, P);
+      if (has1) l1 = sh_eval(sh, 1, m, M, i, N, P);
+    }
+    const size_t o = (size_t)x * a.n_out + y;
+    float* d0 = a.out + (size_t)s0 * oplane + o;
+    const float v0 = (v.x + l0) * f0;
+    *d0 = a.accumulate ? *d0 + v0 : v0;
+    if (has1) {
+      float* d1 = a.out + (size_t)s1 * oplane + o;
+      const float v1 = (v.y + l1) * f1;
+      *d1 = a.accumulate ? *d1 + v1 : v1;
+    }
+  };
+  wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
+}
+
+// ----------------------------------------------------------------------------
+// spectral frozen-flow frames
+// ----------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void k_frame_rows(const float2* __restrict__ spectra, int L,
+                                                    const double* __restrict__ disp, double inv_period,
+                                                    float2* __restrict__ work) {
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
+  __shared__ float2 lds[M * N];
+  __shared__ float2 tw[N];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  float2* py = (float2*)dyn;          // [L][N]
+  float2* px = py + (size_t)L * N;    // [L][M]
+  uint32_t* UX = (uint32_t*)(px + (size_t)L * M);
+  uint32_t* UY = UX + L;
+  const int tid = threadIdx.x;
+  const int f = blockIdx.y;
+  const int row0 = blockIdx.x * M;
+  for (int l = tid; l < L; l += T) {
+    UX[l] = cycles_fixed(disp[((size_t)f * L + l) * 2 + 0] * inv_period);
+    UY[l] = cycles_fixed(disp[((size_t)f * L + l) * 2 + 1] * inv_period);
+  }
+  init_twiddles<N, 1>(tw, tid, T);
+  __syncthreads();
+  for (int idx = tid; idx < L * N; idx += T) {
+    const int l = idx / N, j = idx % N;
+    py[idx] = phasor_fixed((uint32_t)signed_freq(j, N) * UY[l]);
+  }
+  for (int idx = tid; idx < L * M; idx += T) {
+    const int l = idx / M, m = idx % M;
+    px[idx] = phasor_fixed((uint32_t)signed_freq(row0 + m, N) * UX[l]);
+  }
+  __syncthreads();
+  const size_t plane = (size_t)N * N;
+  const float2* sp = spectra + (size_t)row0 * N;
+  auto load = [&](int m, int i) {
+    float2 acc = make_float2(0.0f, 0.0f);
+    for (int l = 0; l < L; ++l) {
+      const float2 s = sp[(size_t)l * plane + (size_t)m * N + i];
+      const float2 ph = cmul(px[l * M + m], py[l * N + i]);
+      const float2 t = cmul(s, ph);
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    return acc;
+  };
+  float2* out = work + (size_t)f * plane + (size_t)row0 * N;
+  auto store = [&](int m, int i, float2 v) { out[(size_t)m * N + i] = v; };
+  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
+}
+
+struct FrameArgs {
+  const float2* work;
+  float* out;
+  const double* disp;     // (F, L, 2)
+  const double2* coeffs;  // (P, L, 9)
+  double inv_period;
+  int L, levels, off, n_out, u_in_lds;
+  float scale;
+};
+
+template <int N>
+__global__ __launch_bounds__(ColsCfg<N>::T) void k_frame_cols(FrameArgs a) {
+  constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
+  __shared__ float2 lds[M * N];
+  __shared__ float2 tw[N];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  const int tid = threadIdx.x;
+  const int f = blockIdx.y;
+  const int c0 = blockIdx.x * M;
+  if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;
+  const int P = a.levels, L = a.L;
+  ShLds sh = sh_carve(dyn, N, P, 1, M, a.u_in_lds != 0);
+  // shifted 3x3 blocks of this frame: after the epilogue tables in LDS
+  double2* Q = (double2*)(dyn + sh_lds_bytes(N, P, 1, M, a.u_in_lds != 0));  // [P][L] (+df_p phasor, x then y)
+  double2* SB = Q + 2 * (size_t)P * L;                                         // [P][9]
+  init_twiddles<N, 1>(tw, tid, T);
+  if (P > 0) {
+    for (int q = tid; q < 2 * P * L; q += T) {
+      const int axis = q & 1, pl = q >> 1, p = pl / L, l = pl % L;
+      const double u = a.disp[((size_t)f * L + l) * 2 + axis] * a.inv_period / pow3(p + 1);
+      double s, c;
+      sincospi(2.0 * (u - floor(u)), &s, &c);
+      Q[q] = make_double2(c, s);
+    }
+    __syncthreads();
+    for (int q = tid; q < 9 * P; q += T) {
+      const int p = q / 9, k = q % 9, ia = k / 3, ib = k % 3;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int l = 0; l < L; ++l) {
+        const double2 qx = Q[2 * (p * L + l) + 0], qy = Q[2 * (p * L + l) + 1];
+        double2 wx = ia == 0 ? dconj(qx) : (ia == 1 ? make_double2(1.0, 0.0) : qx);
+        double2 wy = ib == 0 ? dconj(qy) : (ib == 1 ? make_double2(1.0, 0.0) : qy);
+        const double2 t = dcmul(dcmul(a.coeffs[((size_t)p * L + l) * 9 + k], wx), wy);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      SB[q] = acc;
+    }
+    __syncthreads();
+    auto valid = [&](int t) { return true; };
+    auto block = [&](int t, int p, double2* C) {
+      for (int k = 0; k < 9; ++k) C[k] = SB[p * 9 + k];
+    };
+    sh_setup(sh, N, P, 1, M, c0, tid, T, block, valid);
+  }
+  __syncthreads();
+  const size_t plane = (size_t)N * N;
+  const float2* src = a.work + (size_t)f * plane + c0;
+  float* dst = a.out + (size_t)f * a.n_out * a.n_out;
+  auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
+  auto store = [&](int m, int i, float2 v) {
+    const int x = i - a.off, y = c0 + m - a.off;
+    if ((unsigned)x >= (unsigned)a.n_out || (unsigned)y >= (unsigned)a.n_out) return;
+    const float lo = P > 0 ? sh_eval(sh, 0, m, M, i, N, P) : 0.0f;
+    dst[(size_t)x * a.n_out + y] = (v.x + lo) * a.scale;
+  };
+  wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
+}
+
+// ----------------------------------------------------------------------------
+// host dispatch
+// ----------------------------------------------------------------------------
+static bool is_pow2_fast(int n) { return n >= 32 && n <= 2048 && (n & (n - 1)) == 0; }
+
+// LDS budget per workgroup on gfx950 is 160 KiB.
+static const size_t kLdsMax = 160 * 1024;
+
+template <class K>
+static int set_dyn_lds(K kernel, size_t bytes) {
+  if (bytes > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return fail(PT_ERR_HIP, "hipFuncSetAttribute(%zu B LDS): %s", bytes, hipGetErrorString(e));
+  }
+  return PT_OK;
+}
+
+template <int N>
+static int fft2_pow2(const float2* in, float2* out, int batch, int sign, hipStream_t st) {
+  dim3 gr(N / RowsCfg<N>::M, batch), gc(N / ColsCfg<N>::M, batch);
+  if (sign > 0) {
+    k_fft_rows<N, 1><<<gr, RowsCfg<N>::T, 0, st>>>(in, out, N);
+    k_fft_cols<N, 1><<<gc, ColsCfg<N>::T, 0, st>>>(out, out, N);
+  } else {
+    k_fft_rows<N, -1><<<gr, RowsCfg<N>::T, 0, st>>>(in, out, N);
+    k_fft_cols<N, -1><<<gc, ColsCfg<N>::T, 0, st>>>(out, out, N);
+  }
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+static size_t static_lds_cols(int n, int M) { return (size_t)M * n * 8 + (size_t)n * 8; }
+
+template <int N>
+static int generate_pow2(const float* nre, const float* nim, const float* amp, GenArgs a, hipStream_t st) {
+  constexpr int MC = ColsCfg<N>::M;
+  const size_t stat = static_lds_cols(N, MC);
+  a.u_in_lds = a.levels > 0 && stat + sh_lds_bytes(N, a.levels, 2, MC, true) <= kLdsMax;
+  const size_t dyn = sh_lds_bytes(N, a.levels, 2, MC, a.u_in_lds != 0);
+  if (stat + dyn > kLdsMax) return fail(PT_ERR_UNSUPPORTED, "generate: %d subharmonic levels exceed LDS at n=%d", a.levels, N);
+  int rc = set_dyn_lds(k_gen_cols<N>, dyn);
+  if (rc) return rc;
+  k_gen_rows<N><<<dim3(N / RowsCfg<N>::M, a.n_fft), RowsCfg<N>::T, 0, st>>>(nre, nim, amp, (float2*)a.work);
+  k_gen_cols<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, dyn, st>>>(a);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+template <int N>
+static int frames_pow2(const float2* spectra, int L, const double* disp, int F, FrameArgs a, hipStream_t st) {
+  constexpr int MR = RowsCfg<N>::M, MC = ColsCfg<N>::M;
+  const size_t dyn_r = (size_t)L * N * 8 + (size_t)L * MR * 8 + (size_t)2 * L * 4;
+  if (dyn_r + (size_t)MR * N * 8 + (size_t)N * 8 > kLdsMax)
+    return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor table at n=%d", L, N);
+  const size_t stat = static_lds_cols(N, MC);
+  const size_t extra = a.levels > 0 ? (size_t)a.levels * L * 16 * 2 + (size_t)a.levels * 9 * 16 : 0;
+  a.u_in_lds = a.levels > 0 && stat + sh_lds_bytes(N, a.levels, 1, MC, true) + extra <= kLdsMax;
+  const size_t dyn_c = sh_lds_bytes(N, a.levels, 1, MC, a.u_in_lds != 0) + extra;
+  if (stat + dyn_c > kLdsMax) return fail(PT_ERR_UNSUPPORTED, "frames: subharmonic tables exceed LDS at n=%d", N);
+  int rc = set_dyn_lds(k_frame_rows<N>, dyn_r);
+  if (rc) return rc;
+  rc = set_dyn_lds(k_frame_cols<N>, dyn_c);
+  if (rc) return rc;
+  k_frame_rows<N><<<dim3(N / MR, F), RowsCfg<N>::T, dyn_r, st>>>(spectra, L, disp, a.inv_period, (float2*)a.work);
+  k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, dyn_c, st>>>(a);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+#define PT_POW2_SWITCH(n, CALL) \
+  switch (n) {                  \
+    case 32: CALL(32);          \
+    case 64: CALL(64);          \
+    case 128: CALL(128);        \
+    case 256: CALL(256);        \
+    case 512: CALL(512);        \
+    case 1024: CALL(1024);      \
+    case 2048: CALL(2048);      \
+    default: break;             \
+  }
+
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" int pt_fft2(const void* in, void* out, void* work, int batch, int n0, int n1, int sign, void* stream) {
+  if (!in || !out) return fail(PT_ERR_ARG, "pt_fft2: NULL array");
+  if (batch < 0 || n0 < 1 || n1 < 1 || (sign != 1 && sign != -1)) return fail(PT_ERR_ARG, "pt_fft2: bad shape/sign");
+  if (batch == 0) return PT_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (n0 == n1 && is_pow2_fast(n0)) {
+#define CALL(N) return fft2_pow2<N>((const float2*)in, (float2*)out, batch, sign, st);
+    PT_POW2_SWITCH(n0, CALL)
+#undef CALL
+  }
+  (void)work;
+  return fail(PT_ERR_UNSUPPORTED, "pt_fft2: %dx%d not implemented (square powers of two 32..2048)", n0, n1);
+}
+
+extern "C" int pt_generate(const float* noise_re, const float* noise_im, const float* amplitude,
+                           const float* sh_noise, const double* sh_amp, int levels, int count, int n,
+                           int crop_offset, int n_out, float scale, const float* screen_scale, int accumulate,
+                           float* out, void* work, void* stream) {
+  if (count < 1) return fail(PT_ERR_ARG, "pt_generate: count must be >= 1");
+  if (!noise_re || !noise_im || !amplitude || !out || !work) return fail(PT_ERR_ARG, "pt_generate: NULL array");
+  if (levels < 0 || (levels > 0 && (!sh_noise || !sh_amp))) return fail(PT_ERR_ARG, "pt_generate: bad subharmonics");
+  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "pt_generate: crop outside grid");
+  GenArgs a;
+  a.work = (const float2*)work;
+  a.out = out;
+  a.sh_noise = sh_noise;
+  a.sh_amp = sh_amp;
+  a.screen_scale = screen_scale;
+  a.count = count;
+  a.n_fft = (count + 1) / 2;
+  a.levels = levels;
+  a.off = crop_offset;
+  a.n_out = n_out;
+  a.accumulate = accumulate ? 1 : 0;
+  a.u_in_lds = 0;
+  a.scale = scale;
+  if (a.n_fft > 65535) return fail(PT_ERR_ARG, "pt_generate: count > 131070 per call");
+  hipStream_t st = (hipStream_t)stream;
+  if (is_pow2_fast(n)) {
+#define CALL(N) return generate_pow2<N>(noise_re, noise_im, amplitude, a, st);
+    PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  }
+  return fail(PT_ERR_UNSUPPORTED, "pt_generate: n=%d not implemented (powers of two 32..2048)", n);
+}
+
+extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
+                                  int frames, const double* sh_coeffs, int levels, int crop_offset, int n_out,
+                                  float scale, float* out, void* work, void* stream) {
+  if (!spectra || !disp || !out || !work) return fail(PT_ERR_ARG, "pt_spectral_frames: NULL array");
+  if (layers < 1 || frames < 0 || levels < 0 || (levels > 0 && !sh_coeffs))
+    return fail(PT_ERR_ARG, "pt_spectral_frames: bad layers/frames/levels");
+  if (!(pixel_scale > 0.0)) return fail(PT_ERR_ARG, "pt_spectral_frames: pixel_scale must be > 0");
+  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "pt_spectral_frames: crop outside grid");
+  if (frames == 0) return PT_OK;
+  if (frames > 65535) return fail(PT_ERR_ARG, "pt_spectral_frames: frames > 65535 per call");
+  FrameArgs a;
+  a.work = (const float2*)work;
+  a.out = out;
+  a.disp = disp;
+  a.coeffs = (const double2*)sh_coeffs;
+  a.inv_period = 1.0 / ((double)n * pixel_scale);
+  a.L = layers;
+  a.levels = levels;
+  a.off = crop_offset;
+  a.n_out = n_out;
+  a.u_in_lds = 0;
+  a.scale = scale;
+  hipStream_t st = (hipStream_t)stream;
+  if (is_pow2_fast(n)) {
+#define CALL(N) return frames_pow2<N>((const float2*)spectra, layers, disp, frames, a, st);
+    PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  }
+  return fail(PT_ERR_UNSUPPORTED, "pt_spectral_frames: n=%d not implemented (powers of two 32..2048)", n);
+}

@@ -1,0 +1,118 @@
+"""Thin typed wrappers over the C entry points (device tensors in and out).
+
+Each wrapper checks shapes/dtypes on the host before launching (the
+kernels assume them), allocates the output and workspace with the torch
+caching allocator when the caller does not pass them, and launches on the
+current torch stream.  No wrapper computes anything on the host.
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from . import _hip
+
+
+def _t():
+    return _hip.torch()
+
+
+def _require(t, dtype, shape, name):
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: expected {dtype}, got {t.dtype}")
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous device tensor")
+
+
+def fft2(x, sign: int = 1, out=None):
+    """Unnormalised 2-D DFT over the last two axes of a complex64 tensor.
+
+    ``sign=+1`` is ``ifft2(x) * n0 * n1`` (the reference's convention),
+    ``sign=-1`` the forward DFT.
+    """
+    t = _t()
+    if x.dtype != t.complex64:
+        raise ValueError("fft2: complex64 input required")
+    x = x.contiguous()
+    n0, n1 = x.shape[-2:]
+    batch = int(np.prod(x.shape[:-2])) if x.dim() > 2 else 1
+    if out is None:
+        out = t.empty_like(x)
+    rc = _hip.lib().pt_fft2(_hip.ptr(x), _hip.ptr(out), None, batch, n0, n1, int(sign),
+                            _hip.stream_ptr())
+    _hip.check(rc, "pt_fft2")
+    return out
+
+
+def generate(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offset: int = 0,
+             n_out: Optional[int] = None, scale: float = 1.0, screen_scale=None,
+             out=None, accumulate: bool = False, work=None):
+    """Fused ``PhaseScreen.generate`` on injected noise (see pt_generate).
+
+    ``noise`` (2, ceil(count/2), n, n) float32; ``sh_noise`` (P, 2, count, 3, 3)
+    float32 or None; ``amplitude`` (n, n) float32; ``sh_amp`` (P, 3, 3)
+    float64.  Returns ``out`` (count, n_out, n_out) float32.
+    """
+    t = _t()
+    n = int(amplitude.shape[-1])
+    n_fft = (count + 1) // 2
+    _require(noise, t.float32, (2, n_fft, n, n), "noise")
+    _require(amplitude, t.float32, (n, n), "amplitude")
+    P = 0 if sh_amp is None else int(sh_amp.shape[0])
+    if P:
+        _require(sh_amp, t.float64, (P, 3, 3), "sh_amp")
+        _require(sh_noise, t.float32, (P, 2, count, 3, 3), "sh_noise")
+    n_out = n if n_out is None else int(n_out)
+    if out is None:
+        out = t.zeros((count, n_out, n_out), dtype=t.float32, device=noise.device) if accumulate \
+            else t.empty((count, n_out, n_out), dtype=t.float32, device=noise.device)
+    _require(out, t.float32, (count, n_out, n_out), "out")
+    if screen_scale is not None:
+        _require(screen_scale, t.float32, (count,), "screen_scale")
+    if work is None:
+        work = t.empty((n_fft, n, n), dtype=t.complex64, device=noise.device)
+    elif work.numel() < n_fft * n * n or work.dtype != t.complex64:
+        raise ValueError("work: need n_fft*n*n complex64")
+    plane = n_fft * n * n * 4
+    base = _hip.ptr(noise)
+    rc = _hip.lib().pt_generate(base, base + plane, _hip.ptr(amplitude),
+                                _hip.ptr(sh_noise) if P else None,
+                                _hip.ptr(sh_amp) if P else None, P, int(count), n,
+                                int(crop_offset), n_out, float(scale),
+                                _hip.ptr(screen_scale), int(bool(accumulate)),
+                                _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "pt_generate")
+    return out
+
+
+def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset: int = 0,
+                    n_out: Optional[int] = None, scale: float = 1.0, out=None, work=None):
+    """Frozen-flow frames (see pt_spectral_frames).
+
+    ``spectra`` (L, n, n) complex64; ``disp`` (F, L, 2) float64 metres;
+    ``sh_coeffs`` (P, L, 3, 3) complex128 or None.  Returns (F, n_out, n_out).
+    """
+    t = _t()
+    L, n = int(spectra.shape[0]), int(spectra.shape[-1])
+    _require(spectra, t.complex64, (L, n, n), "spectra")
+    F = int(disp.shape[0])
+    _require(disp, t.float64, (F, L, 2), "disp")
+    P = 0 if sh_coeffs is None else int(sh_coeffs.shape[0])
+    if P:
+        _require(sh_coeffs, t.complex128, (P, L, 3, 3), "sh_coeffs")
+    n_out = n if n_out is None else int(n_out)
+    if out is None:
+        out = t.empty((F, n_out, n_out), dtype=t.float32, device=spectra.device)
+    _require(out, t.float32, (F, n_out, n_out), "out")
+    if work is None or work.numel() < F * n * n:
+        work = t.empty((F, n, n), dtype=t.complex64, device=spectra.device)
+    rc = _hip.lib().pt_spectral_frames(_hip.ptr(spectra), L, n, float(pixel_scale),
+                                       _hip.ptr(disp), F, _hip.ptr(sh_coeffs) if P else None,
+                                       P, int(crop_offset), n_out, float(scale),
+                                       _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "pt_spectral_frames")
+    return out

@@ -1,0 +1,210 @@
+"""GPU parity: FFT passes, fused generate and spectral frames vs the oracle.
+
+Tolerances: FP32 device arithmetic against the FP64 oracle on identical
+noise, relative RMS <= 1e-5 (the stated bar), max-abs checks where noted.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import config2_layer_tuples, load_golden
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_fft2_pow2_matches_numpy(hip, n, sign):
+    import torch
+    from pyturb_amd import ops
+
+    print(f"fft2 n={n} sign={sign}", flush=True)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    got = _np(ops.fft2(torch.from_numpy(x.astype(np.complex64)).cuda(), sign))
+    ref = np.fft.ifft2(x) * n * n if sign > 0 else np.fft.fft2(x)
+    err = np.sqrt(np.mean(np.abs(got - ref) ** 2) / np.mean(np.abs(ref) ** 2))
+    assert err < 2e-6, err
+
+
+def test_fft2_inplace(hip):
+    import torch
+    from pyturb_amd import ops
+
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal((3, 256, 256)) + 1j * rng.standard_normal((3, 256, 256)))
+    t = torch.from_numpy(x.astype(np.complex64)).cuda()
+    ops.fft2(t, 1, out=t)
+    ref = np.fft.ifft2(x) * 256 * 256
+    assert O.rel_rms(np.abs(_np(t) - ref), np.abs(ref)) < 2e-6
+
+
+def _gen_case(n, dx, r0, L0, P, count, seed):
+    filt = O.build_filters(n, dx, r0, L0, subharmonics=P)
+    rng = np.random.default_rng(seed)
+    main, sh = O.draw_generate_noise(rng, n, count, P)
+    return filt, main, sh
+
+
+def _run_generate(filt, main, sh, count, **kw):
+    import torch
+    from pyturb_amd import ops
+
+    P = filt["sh_amp"].shape[0]
+    dev = torch.device("cuda")
+    noise = torch.from_numpy(main.astype(np.float32)).to(dev)
+    shn = torch.from_numpy(sh.astype(np.float32)).to(dev) if P else None
+    amp = torch.from_numpy(filt["amplitude"].astype(np.float32)).to(dev)
+    sha = torch.from_numpy(filt["sh_amp"]).to(dev) if P else None
+    return ops.generate(noise, shn, amp, sha, count, **kw)
+
+
+def test_generate_config1_matches_oracle(hip):
+    """n=256, dx=1/32, r0=0.15, L0=25, 3 levels, 64 screens, seed=1 noise."""
+    print("generate config1", flush=True)
+    filt, main, sh = _gen_case(256, 0.03125, 0.15, 25.0, 3, 64, 1)
+    ref = O.generate(filt, main.astype(np.float32).astype(np.float64),
+                     sh.astype(np.float32).astype(np.float64), 64)
+    got = _np(_run_generate(filt, main, sh, 64))
+    assert O.rel_rms(got, ref) < REL
+    # and against the reference's own float64 realisation (golden)
+    g = load_golden("generate_n256_seed1.npz")
+    assert O.rel_rms(got[:2], g["screens01"]) < REL
+
+
+@pytest.mark.parametrize("n,count,P", [(64, 5, 0), (64, 7, 8), (128, 1, 2), (512, 4, 8),
+                                       (1024, 2, 8), (32, 3, 1), (2048, 2, 8)])
+def test_generate_shapes_levels_odd_counts(hip, n, count, P):
+    print(f"generate n={n} count={count} P={P}", flush=True)
+    filt, main, sh = _gen_case(n, 8.0 / n, 0.12, 30.0, P, count, 7)
+    ref = O.generate(filt, main.astype(np.float32).astype(np.float64),
+                     sh.astype(np.float32).astype(np.float64) if P else sh, count)
+    got = _np(_run_generate(filt, main, sh, count))
+    assert got.shape == (count, n, n)
+    assert O.rel_rms(got, ref) < REL
+
+
+def test_generate_crop_scale_accumulate(hip):
+    import torch
+
+    n, count, P = 128, 6, 4
+    filt, main, sh = _gen_case(n, 0.0625, 0.1, 20.0, P, count, 3)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    ref = O.generate(filt, f32(main), f32(sh), count)[:, 10:110, 10:110]
+    ss = np.linspace(0.5, 2.0, count).astype(np.float32)
+    out = _run_generate(filt, main, sh, count, crop_offset=10, n_out=100, scale=0.25,
+                        screen_scale=torch.from_numpy(ss).cuda())
+    got = _np(out)
+    want = ref * 0.25 * ss[:, None, None]
+    assert O.rel_rms(got, want) < REL
+    _run_generate(filt, main, sh, count, crop_offset=10, n_out=100, scale=0.25,
+                  screen_scale=torch.from_numpy(ss).cuda(), out=out, accumulate=True)
+    assert O.rel_rms(_np(out), 2 * want) < REL
+
+
+def _frames_case(n, times, P=8, seed=2):
+    atm = O.OracleAtmosphere(config2_layer_tuples(), r0=0.15, diameter=8.0, n=n, L0=25.0,
+                             subharmonics=P, dtype="float64", seed=seed)
+    return atm
+
+
+def _run_frames(atm, disp, scale=1.0):
+    import torch
+    from pyturb_amd import ops
+
+    dev = torch.device("cuda")
+    spectra = torch.from_numpy(atm.spectra.astype(np.complex64)).to(dev)
+    P = atm.sh_coeffs.shape[0]
+    coeffs = torch.from_numpy(np.ascontiguousarray(atm.sh_coeffs)).to(dev) if P else None
+    d = torch.from_numpy(np.ascontiguousarray(disp, dtype=np.float64)).to(dev)
+    return _np(ops.spectral_frames(spectra, d, coeffs, atm.dx, crop_offset=atm.margin,
+                                   n_out=atm.n, scale=scale))
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_spectral_frames_config2_match_oracle(hip, n):
+    """9-layer profile, winds from seed 2, up to 2 s of travel (wrapped)."""
+    print(f"frames n={n}", flush=True)
+    atm = _frames_case(n, None)
+    times = [0.0, 0.001, 0.0137, 0.5, 1.999]
+    disp = np.stack([np.stack(atm.displacements(t), axis=-1) for t in times])
+    got = _run_frames(atm, disp)
+    for k, t in enumerate(times):
+        ref = atm.phase(t)
+        assert O.rel_rms(got[k], ref) < REL, (t, O.rel_rms(got[k], ref))
+
+
+def test_spectral_frame_matches_reference_golden(hip):
+    g = load_golden("atmosphere_n64_seed2.npz")
+    atm = _frames_case(64, None)
+    disp = np.stack(atm.displacements(0.02), axis=-1)[None]
+    got = _run_frames(atm, disp, scale=500e-9 / (2 * np.pi))
+    assert O.rel_rms(got[0], g["frame_nofov_t002"]) < REL
+
+
+def test_spectral_integer_shift_is_roll(hip):
+    atm = _frames_case(128, None, P=0)
+    dx = atm.dx
+    disp = np.zeros((2, 9, 2))
+    disp[1, :, 0] = 5 * dx
+    disp[1, :, 1] = -3 * dx
+    got = _run_frames(atm, disp)
+    np.testing.assert_allclose(got[1], np.roll(np.roll(got[0], -5, 0), 3, 1),
+                               atol=2e-5 * np.abs(got[0]).max())
+
+
+def test_phase_reduction_at_large_travel(hip):
+    """Exact fixed-point argument reduction: a shift by whole periods is exact."""
+    atm = _frames_case(256, None, P=0)
+    period = atm.ns * atm.dx
+    base = np.stack(atm.displacements(0.01), axis=-1)
+    disp = np.stack([base, base + 1000 * period])
+    got = _run_frames(atm, disp)
+    assert O.rel_rms(got[1], got[0]) < 1e-6
+
+
+# ------------------------------------------------------------------ drop-in API
+def test_atmosphere_api_matches_reference_golden(hip):
+    import pyturb_amd as pa
+
+    g = load_golden("atmosphere_n64_seed2.npz")
+    layers = [pa.Layer(*ly) for ly in config2_layer_tuples()]
+    atm = pa.Atmosphere(layers, r0=0.15, diameter=8.0, n=64, L0=25.0, dtype="float64",
+                        seed=2)
+    got = _np(atm.opd(0.02))
+    assert got.dtype == np.float64
+    assert O.rel_rms(got, g["frame_nofov_t002"]) < REL
+    ts = [t for t, _ in atm.frames(dt=1e-3, steps=3)]
+    assert ts == [0.0, 1e-3, 2e-3]
+
+
+def test_sample_api_matches_reference_golden(hip):
+    import pyturb_amd as pa
+
+    g = load_golden("sample_n64_seed4.npz")
+    layers = [pa.Layer(0, 0.4, 10.0, 0.0, 25.0), pa.Layer(1000, 0.2, 10.0, 0.0, 50.0),
+              pa.Layer(5000, 0.3, 10.0, 0.0, 25.0), pa.Layer(10000, 0.1, 10.0, 0.0, 50.0)]
+    atm = pa.Atmosphere(layers, r0=0.15, diameter=8.0, n=64, dtype="float64", seed=4)
+    got = _np(atm.sample(4))
+    assert O.rel_rms(got, g["sample"]) < REL
+
+
+def test_phase_screen_api_matches_reference_golden(hip):
+    import pyturb_amd as pa
+
+    g = load_golden("generate_n256_seed1.npz")
+    gen = pa.PhaseScreen(n=256, pixel_scale=0.03125, r0=0.15, L0=25.0, subharmonics=3,
+                         seed=1, dtype="float64")
+    got = _np(gen.generate(64))
+    assert O.rel_rms(got[:2], g["screens01"]) < REL
+    np.testing.assert_allclose(got.sum(axis=(1, 2)), g["sums"],
+                               atol=REL * np.abs(got).sum() / 64)
