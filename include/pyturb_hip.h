@@ -64,7 +64,8 @@ int pt_fft2(const void* in, void* out, void* work, int batch, int n0, int n1,
  *   out                : (count, n_out, n_out) f32; screen s pixel (x, y) is
  *                        full-grid pixel (x + crop_offset, y + crop_offset);
  *                        accumulate != 0 adds into out
- *   work               : n_fft * n * n c64
+ *   work               : n_fft * n * n c64 (2 * n_fft * n * n when n is not
+ *                        a power of two in 32..2048: general-size path)
  * Screen k < n_fft is the real part of FFT k, screen n_fft + k its
  * imaginary part (fourier.py:260).
  */
@@ -87,7 +88,8 @@ int pt_generate(const float* noise_re, const float* noise_im,
  *   out       : (frames, n_out, n_out) f32,
  *               out = scale * Re(ifft2(sum_l S_l P_x P_y) * n^2 + low)
  *               cropped at crop_offset
- *   work      : frames * n * n c64
+ *   work      : frames * n * n c64 (twice that for the general-size path,
+ *               n not a power of two in 32..2048)
  * Shift-theorem phases are reduced exactly (32-bit fixed-point cycles), so
  * large travel keeps full FP32 phase accuracy.
  */
@@ -96,6 +98,55 @@ int pt_spectral_frames(const void* spectra, int layers, int n,
                        const double* sh_coeffs, int levels, int crop_offset,
                        int n_out, float scale, float* out, void* work,
                        void* stream);
+
+/*
+ * Innovation terms of the Assemat-Wilson recurrence for a batch of new rows:
+ *   out[k][w] = sum_j B_unit[w][j] * beta[k][j]      (out = beta * B^T)
+ *   beta (rows, width) f32 white noise, B_unit (width, width) f32, out
+ *   (rows, width) f32.  The per-layer r0^(-5/6) factor is applied by
+ *   pt_extrude_rows.  Replaces the B @ beta half of
+ *   src/pyturb/extrude.py:427-429 and src/pyturb/infinite.py:278-280,
+ *   batched over every row of a step (as extrude.py:686-689 does for
+ *   boiling).
+ */
+int pt_innovation_gemm(const float* beta, const float* B_unit, int rows,
+                       int width, float* out, void* stream);
+
+/*
+ * One round of the row recurrence for `nact` ring buffers:
+ *   buf[layer[e]][trow[e]][:] = A @ concat(buf[layer[e]][zrows[e][0..m-1]])
+ *                              + bscale[e] * bb[bbrow[e]][:]
+ *   A (width, m*width) f32 (shared by the layers of one outer scale);
+ *   buf (layers, cap, width) f32 ring buffers (physical row indices);
+ *   bb from pt_innovation_gemm.  HOST arrays: layer, trow, bbrow (nact)
+ *   int, zrows (nact, m) int, bscale (nact) f32.  m <= 4.
+ * Replaces src/pyturb/extrude.py:424-430 (and infinite.py:274-281) for all
+ * layers that extrude a row in this round.
+ */
+int pt_extrude_rows(const float* A, int width, int stencil_rows,
+                    const float* bb, float* buf, long long cap, int nact,
+                    const int* layer, const int* zrows, const int* trow,
+                    const int* bbrow, const float* bscale, void* stream);
+
+/*
+ * Fused multi-layer, multi-direction readout of the ring buffers
+ * (src/pyturb/extrude.py:754-915 and the CUDA readout kernels
+ * extrude.py:91-198; one launch instead of one per direction):
+ *   out[d][i][j] = scale * sum_l interp(buf[l], row, col) (+ out if accumulate)
+ *   g = (i - (n-1)/2) * mag[d][l]  (likewise for j),
+ *   row = g_i cos_l + g_j sin_l + sa[d][l],
+ *   col = -g_i sin_l + g_j cos_l + width/2 + sp[d][l],
+ *   rows clamped to [0, fill_l - 1] (local ring rows, physical row
+ *   (start_l + r) mod cap), columns to [0, width - 1].
+ *   HOST arrays: start (layers) int, fill (layers) int64, wind_cs
+ *   (layers, 2) f64 = (cos, sin), dir_geom (dirs, layers, 3) f64 =
+ *   (sa, sp, mag).  interp: 0 linear, 1 Catmull-Rom cubic, 2 Lanczos-3.
+ *   out (dirs, n, n) f32 device.
+ */
+int pt_readout(const float* buf, int layers, long long cap, int width,
+               const int* start, const long long* fill, const double* wind_cs,
+               const double* dir_geom, int dirs, int n, int interp, float scale,
+               int accumulate, float* out, void* stream);
 
 #ifdef __cplusplus
 }

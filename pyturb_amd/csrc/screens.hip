@@ -2,7 +2,8 @@
 // spectral frozen-flow frame engine, gfx950.
 //
 // A 2-D transform of an N x N grid is two passes over HBM:
-//   pass A (rows):    256-thread workgroups, M = 2048/N rows each; the
+//   pass A (rows):    256-thread workgroups (N*N/8 when smaller), M = 8T/N
+//                     rows each; the
 //                     prologue builds the spectrum on the fly (noise x PSD
 //                     amplitude, or the phasor-weighted layer sum) and the
 //                     rows are transformed along axis 1 in LDS;
@@ -18,12 +19,13 @@
 
 #include "common.hpp"
 #include "fft_core.hpp"
+#include "sh_epilogue.hpp"
 
 namespace pt {
 
 template <int N>
 struct RowsCfg {
-  static constexpr int T = 256;
+  static constexpr int T = (N * N / 8 < 256) ? N * N / 8 : 256;
   static constexpr int M = 8 * T / N;
 };
 template <int N>
@@ -61,102 +63,6 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __rest
   auto load = [&](int m, int i) { return in[base + (size_t)i * n1 + m]; };
   auto store = [&](int m, int i, float2 v) { out[base + (size_t)i * n1 + m] = v; };
   wg_fft<N, T, M, true, S>(lds, tw, tid, load, store);
-}
-
-// ----------------------------------------------------------------------------
-// subharmonic epilogue tables for a column tile
-// ----------------------------------------------------------------------------
-// Dynamic LDS of a column pass: U[P][N] basis values (when it fits), then
-// H[NT][M][P] float2, KP[NT][M][P] float, K[NT][M] float, MEAN[NT][P] double.
-struct ShLds {
-  float2* U;
-  float2* H;
-  float* KP;
-  float* K;
-  double* MEAN;
-  bool u_in_lds;
-};
-
-__host__ __device__ inline size_t sh_lds_bytes(int n, int P, int NT, int M, bool u_in_lds) {
-  if (P == 0) return 0;
-  size_t b = u_in_lds ? (size_t)n * P * 8 : 0;
-  b += (size_t)NT * M * P * 8 + (size_t)NT * M * P * 4 + (size_t)NT * M * 4;
-  b = (b + 7) & ~(size_t)7;
-  b += (size_t)NT * P * 8;
-  return (b + 15) & ~(size_t)15;
-}
-
-__device__ inline ShLds sh_carve(unsigned char* dyn, int n, int P, int NT, int M, bool u_in_lds) {
-  ShLds s;
-  size_t o = 0;
-  s.u_in_lds = u_in_lds;
-  s.U = (float2*)(dyn + o);
-  if (u_in_lds) o += (size_t)n * P * 8;
-  s.H = (float2*)(dyn + o);
-  o += (size_t)NT * M * P * 8;
-  s.KP = (float*)(dyn + o);
-  o += (size_t)NT * M * P * 4;
-  s.K = (float*)(dyn + o);
-  o += (size_t)NT * M * 4;
-  o = (o + 7) & ~(size_t)7;
-  s.MEAN = (double*)(dyn + o);
-  return s;
-}
-
-// Fill U (if in LDS), H, KP and MEAN for NT coefficient sets over columns
-// c0..c0+M-1.  `block(t, p, C)` writes the 3x3 block of set t, level p.
-// `valid(t)` says whether set t is used.  Ends with K filled; callers
-// __syncthreads() before reading.
-template <class Block, class Valid>
-__device__ inline void sh_setup(ShLds& s, int n, int P, int NT, int M, int c0, int tid, int nthreads,
-                                Block& block, Valid& valid) {
-  if (s.u_in_lds) {
-    for (int idx = tid; idx < P * n; idx += nthreads) s.U[idx] = sh_basis_e(idx / n, idx % n, n);
-  }
-  const int ntask = NT * M * P;
-  for (int task = tid; task < ntask + NT * P; task += nthreads) {
-    if (task < ntask) {
-      const int p = task % P;
-      const int m = (task / P) % M;
-      const int t = task / (P * M);
-      if (!valid(t)) continue;
-      double2 C[9];
-      block(t, p, C);
-      int y = c0 + m;
-      if (y >= n) y = n - 1;  // tile overhang (never stored)
-      double2 H;
-      double g1;
-      sh_column(C, sh_basis_e(p, y, n), H, g1);
-      s.H[(t * M + m) * P + p] = make_float2((float)H.x, (float)H.y);
-      s.KP[(t * M + m) * P + p] = (float)g1;
-    } else {
-      const int q = task - ntask;
-      const int p = q % P, t = q / P;
-      if (!valid(t)) continue;
-      double2 C[9];
-      block(t, p, C);
-      s.MEAN[t * P + p] = sh_block_mean(C, sh_mean_e(p, n));
-    }
-  }
-  __syncthreads();
-  for (int q = tid; q < NT * M; q += nthreads) {
-    const int t = q / M;
-    if (!valid(t)) continue;
-    double acc = 0.0;
-    for (int p = 0; p < P; ++p) acc += (double)s.KP[q * P + p] - s.MEAN[t * P + p];
-    s.K[q] = (float)acc;
-  }
-}
-
-__device__ __forceinline__ float sh_eval(const ShLds& s, int t, int m, int M, int x, int n, int P) {
-  float acc = s.K[t * M + m];
-  const float2* H = s.H + (size_t)(t * M + m) * P;
-  for (int p = 0; p < P; ++p) {
-    float2 e = s.u_in_lds ? s.U[p * n + x] : sh_basis_e(p, x, n);
-    float2 h = H[p];
-    acc += e.x * h.x - e.y * h.y;
-  }
-  return acc;
 }
 
 // ----------------------------------------------------------------------------
@@ -331,28 +237,7 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_frame_cols(FrameArgs a) {
   double2* SB = Q + 2 * (size_t)P * L;                                         // [P][9]
   init_twiddles<N, 1>(tw, tid, T);
   if (P > 0) {
-    for (int q = tid; q < 2 * P * L; q += T) {
-      const int axis = q & 1, pl = q >> 1, p = pl / L, l = pl % L;
-      const double u = a.disp[((size_t)f * L + l) * 2 + axis] * a.inv_period / pow3(p + 1);
-      double s, c;
-      sincospi(2.0 * (u - floor(u)), &s, &c);
-      Q[q] = make_double2(c, s);
-    }
-    __syncthreads();
-    for (int q = tid; q < 9 * P; q += T) {
-      const int p = q / 9, k = q % 9, ia = k / 3, ib = k % 3;
-      double2 acc = make_double2(0.0, 0.0);
-      for (int l = 0; l < L; ++l) {
-        const double2 qx = Q[2 * (p * L + l) + 0], qy = Q[2 * (p * L + l) + 1];
-        double2 wx = ia == 0 ? dconj(qx) : (ia == 1 ? make_double2(1.0, 0.0) : qx);
-        double2 wy = ib == 0 ? dconj(qy) : (ib == 1 ? make_double2(1.0, 0.0) : qy);
-        const double2 t = dcmul(dcmul(a.coeffs[((size_t)p * L + l) * 9 + k], wx), wy);
-        acc.x += t.x;
-        acc.y += t.y;
-      }
-      SB[q] = acc;
-    }
-    __syncthreads();
+    frame_sh_blocks(Q, SB, a.disp, a.coeffs, f, L, P, a.inv_period, tid, T);
     auto valid = [&](int t) { return true; };
     auto block = [&](int t, int p, double2* C) {
       for (int k = 0; k < 9; ++k) C[k] = SB[p * 9 + k];
@@ -458,6 +343,15 @@ static int frames_pow2(const float2* spectra, int L, const double* disp, int F, 
 
 using namespace pt;
 
+// general-size path (generic.hip)
+int pt_fft2_generic(const void* in, void* out, void* work, int batch, int n, int sign, void* stream);
+int pt_generate_generic(const float* nre, const float* nim, const float* amp, const float* sh_noise,
+                        const double* sh_amp, int levels, int count, int n, int off, int n_out, float scale,
+                        const float* screen_scale, int accumulate, float* out, void* work, void* stream);
+int pt_frames_generic(const void* spectra, int L, int n, double pixel_scale, const double* disp, int frames,
+                      const double* coeffs, int levels, int off, int n_out, float scale, float* out, void* work,
+                      void* stream);
+
 extern "C" int pt_fft2(const void* in, void* out, void* work, int batch, int n0, int n1, int sign, void* stream) {
   if (!in || !out) return fail(PT_ERR_ARG, "pt_fft2: NULL array");
   if (batch < 0 || n0 < 1 || n1 < 1 || (sign != 1 && sign != -1)) return fail(PT_ERR_ARG, "pt_fft2: bad shape/sign");
@@ -468,8 +362,8 @@ extern "C" int pt_fft2(const void* in, void* out, void* work, int batch, int n0,
     PT_POW2_SWITCH(n0, CALL)
 #undef CALL
   }
-  (void)work;
-  return fail(PT_ERR_UNSUPPORTED, "pt_fft2: %dx%d not implemented (square powers of two 32..2048)", n0, n1);
+  if (n0 != n1) return fail(PT_ERR_UNSUPPORTED, "pt_fft2: only square grids (got %dx%d)", n0, n1);
+  return pt_fft2_generic(in, out, work, batch, n0, sign, stream);
 }
 
 extern "C" int pt_generate(const float* noise_re, const float* noise_im, const float* amplitude,
@@ -501,7 +395,8 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   }
-  return fail(PT_ERR_UNSUPPORTED, "pt_generate: n=%d not implemented (powers of two 32..2048)", n);
+  return pt_generate_generic(noise_re, noise_im, amplitude, sh_noise, sh_amp, levels, count, n, crop_offset,
+                             n_out, scale, screen_scale, accumulate, out, work, stream);
 }
 
 extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
@@ -532,5 +427,6 @@ extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   }
-  return fail(PT_ERR_UNSUPPORTED, "pt_spectral_frames: n=%d not implemented (powers of two 32..2048)", n);
+  return pt_frames_generic(spectra, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
+                           scale, out, work, stream);
 }

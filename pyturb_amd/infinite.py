@@ -1,0 +1,264 @@
+"""Infinite (extruded) screens: covariance, extrusion matrices and the
+single-layer ``InfinitePhaseScreen`` (src/pyturb/infinite.py:54-395).
+
+The von Karman covariance and the ``A``/``B`` matrices are one-time host
+float64 work with the same numpy/scipy routines as the reference
+(``cho_factor``/``cho_solve`` with a least-squares fallback, ``eigh``), so
+``B``'s eigenvector conventions match.  The ring buffer, the row recurrence
+``A z + B beta`` and the sub-pixel row interpolation run on the device
+(``pt_innovation_gemm``, ``pt_extrude_rows``, ``pt_readout``).
+"""
+
+from __future__ import annotations
+
+from typing import Any, Dict, Tuple, Union
+
+import numpy as np
+from scipy import linalg
+from scipy.special import gamma, kv
+
+from . import _hip
+from .backend import resolve_device
+from .config import ScreenConfig
+from .fourier import PhaseScreen
+from .ring import Ring
+
+__all__ = ["InfinitePhaseScreen", "phase_covariance", "build_extrusion"]
+
+_INTERP_CODE = {"linear": 0, "cubic": 1, "lanczos": 2}
+
+
+def phase_covariance(r, r0: float, L0: float) -> np.ndarray:
+    """Von Karman phase covariance C(r) [rad^2] (infinite.py:88-117).
+
+    ``C(0) = 0.0863 (L0/r0)^(5/3)``; needs a finite ``L0``.
+    """
+    if not np.isfinite(L0):
+        raise ValueError("phase_covariance requires a finite outer scale L0")
+    r = np.asarray(r, dtype=np.float64)
+    pref = ((L0 / r0) ** (5.0 / 3.0) * 2.0 ** (-5.0 / 6.0) * gamma(11.0 / 6.0)
+            / np.pi ** (8.0 / 3.0) * (4.8 * gamma(1.2)) ** (5.0 / 6.0))
+    x = 2.0 * np.pi * r / L0
+    core = np.full_like(x, 2.0 ** (-1.0 / 6.0) * gamma(5.0 / 6.0))
+    pos = x > 0
+    core[pos] = x[pos] ** (5.0 / 6.0) * kv(5.0 / 6.0, x[pos])
+    return pref * core
+
+
+def _spd_solve(spd, rhs):
+    try:
+        return linalg.cho_solve(linalg.cho_factor(spd), rhs)
+    except np.linalg.LinAlgError:
+        return np.linalg.lstsq(spd, rhs, rcond=None)[0]
+
+
+def build_extrusion(width: int, stencil_rows: int, pixel_scale: float, L0: float,
+                    r0: float = 1.0) -> Tuple[np.ndarray, np.ndarray]:
+    """``(A, B)`` float64 for extruding one ``width``-pixel row.
+
+    Stencil rows at y = 0..m-1, new row at y = m; ``A = C_xz C_zz^-1``,
+    ``B = V sqrt(max(lambda, 0))`` of the symmetrised conditional covariance
+    (extrude.py:250-304, infinite.py:231-269).
+    """
+    m, W = int(stencil_rows), int(width)
+    zy, zx = np.divmod(np.arange(m * W, dtype=np.float64), W)  # row-major (y, x)
+    nx = np.arange(W, dtype=np.float64)
+    ny = np.full(W, float(m))
+
+    def cov(ax, ay, bx, by):
+        sep = np.hypot(ax[:, None] - bx[None, :], ay[:, None] - by[None, :])
+        return phase_covariance(sep * pixel_scale, r0, L0)
+
+    c_zz = cov(zx, zy, zx, zy)
+    c_xz = cov(nx, ny, zx, zy)
+    c_xx = cov(nx, ny, nx, ny)
+    a_mat = _spd_solve(c_zz, c_xz.T).T
+    res = c_xx - a_mat @ c_xz.T
+    res = 0.5 * (res + res.T)
+    lam, vec = np.linalg.eigh(res)
+    return a_mat, vec * np.sqrt(np.clip(lam, 0.0, None))
+
+
+_MATRIX_CACHE: Dict[tuple, Tuple[Any, Any]] = {}
+
+
+def device_extrusion(width, stencil_rows, pixel_scale, L0, r0, dev):
+    """(A, B) as float32 device tensors, cached per geometry and device."""
+    key = (int(width), int(stencil_rows), float(pixel_scale), round(float(L0), 9),
+           float(r0), str(dev))
+    hit = _MATRIX_CACHE.get(key)
+    if hit is None:
+        a, b = build_extrusion(width, stencil_rows, pixel_scale, L0, r0)
+        hit = (_hip.to_device(a, np.float32, dev), _hip.to_device(b, np.float32, dev))
+        _MATRIX_CACHE[key] = hit
+    return hit
+
+
+def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, groups=None):
+    """Run the planned rows of several layers on the device.
+
+    ``plans[l]``: list of (target_phys, z_phys) for layer l, in order;
+    ``betas[l]``: (k_l, width) float32 device innovations; ``bscales[l]``:
+    r0^(-5/6) factor.  ``A``/``B`` may be per-layer lists (outer-scale
+    groups) or shared tensors.
+    """
+    t = _hip.torch()
+    L = len(plans)
+    lib = _hip.lib()
+    st = _hip.stream_ptr()
+    As = A if isinstance(A, (list, tuple)) else [A] * L
+    Bs = B if isinstance(B, (list, tuple)) else [B] * L
+    # innovation GEMM per distinct B (one per outer-scale group)
+    bb_of = {}
+    bbrow = [0] * L
+    for key in dict.fromkeys(id(b) for b in Bs):
+        members = [l for l in range(L) if id(Bs[l]) == key and len(plans[l])]
+        if not members:
+            continue
+        beta = t.cat([betas[l] for l in members]).contiguous()
+        bb = t.empty_like(beta)
+        rc = lib.pt_innovation_gemm(_hip.ptr(beta), _hip.ptr(Bs[members[0]]), beta.shape[0],
+                                    width, _hip.ptr(bb), st)
+        _hip.check(rc, "pt_innovation_gemm")
+        off = 0
+        for l in members:
+            bb_of[l] = bb
+            bbrow[l] = off
+            off += len(plans[l])
+    rounds = max((len(p) for p in plans), default=0)
+    for r in range(rounds):
+        active = [l for l in range(L) if len(plans[l]) > r]
+        for key in dict.fromkeys((id(As[l]), id(bb_of[l])) for l in active):
+            grp = [l for l in active if (id(As[l]), id(bb_of[l])) == key]
+            n = len(grp)
+            layer = (ctypes_int * n)(*grp)
+            zr = (ctypes_int * (n * m))(*[z for l in grp for z in plans[l][r][1]])
+            tr = (ctypes_int * n)(*[plans[l][r][0] for l in grp])
+            br = (ctypes_int * n)(*[bbrow[l] + r for l in grp])
+            sc = (ctypes_float * n)(*[float(bscales[l]) for l in grp])
+            rc = lib.pt_extrude_rows(_hip.ptr(As[grp[0]]), width, m, _hip.ptr(bb_of[grp[0]]),
+                                     _hip.ptr(buf), cap, n, layer, zr, tr, br, sc, st)
+            _hip.check(rc, "pt_extrude_rows")
+
+
+import ctypes as _ct  # noqa: E402
+
+ctypes_int = _ct.c_int
+ctypes_float = _ct.c_float
+ctypes_double = _ct.c_double
+ctypes_ll = _ct.c_longlong
+
+
+def readout(buf, cap, width, rings, wind_cs, dir_geom, n, interp, scale=1.0, out=None,
+            accumulate=False):
+    """Fused readout of ring buffers (see pt_readout); returns (D, n, n)."""
+    t = _hip.torch()
+    L = len(rings)
+    D = len(dir_geom)
+    if out is None:
+        out = t.empty((D, n, n), dtype=t.float32, device=buf.device)
+    start = (ctypes_int * L)(*[r.start for r in rings])
+    fill = (ctypes_ll * L)(*[r.fill for r in rings])
+    cs = (ctypes_double * (2 * L))(*[v for c in wind_cs for v in c])
+    geom = (ctypes_double * (3 * D * L))(*np.asarray(dir_geom, dtype=np.float64).ravel())
+    rc = _hip.lib().pt_readout(_hip.ptr(buf), L, cap, width, start, fill, cs, geom, D, n,
+                               _INTERP_CODE[interp], float(scale), int(bool(accumulate)),
+                               _hip.ptr(out), _hip.stream_ptr())
+    _hip.check(rc, "pt_readout")
+    return out
+
+
+class InfinitePhaseScreen:
+    """Endless frozen-flow screen extruded row by row along axis 0.
+
+    Same parameters and semantics as the reference (infinite.py:120-395):
+    ``step(k)``/``advance(pixels)`` move the wind monotonically, the pupil is
+    interpolated at the exact sub-pixel offset; results are device tensors.
+    Innovations are drawn from the same host stream as the reference unless
+    ``innovations`` (a callable ``k -> (k, n) device tensor``) is given.
+    """
+
+    def __init__(self, n: int, pixel_scale: float, r0: float, L0: float = 25.0,
+                 stencil_rows: int = 2, interp: str = "cubic", seed: Any = None,
+                 device: str = "mi355x", dtype: Union[str, np.dtype] = "float32",
+                 innovations=None) -> None:
+        config = ScreenConfig.create(n, pixel_scale, r0, L0, device, dtype,
+                                     finite_outer_scale=True)
+        if (not isinstance(stencil_rows, (int, np.integer)) or not 1 <= stencil_rows < config.n):
+            raise ValueError("stencil_rows must be in [1, n)")
+        if interp not in ("cubic", "linear", "lanczos"):
+            raise ValueError("interp must be 'cubic', 'linear', or 'lanczos'")
+        self.n = config.n
+        self.pixel_scale = config.pixel_scale
+        self.r0 = config.r0
+        self.L0 = config.L0
+        self.stencil_rows = int(stencil_rows)
+        self.interp = interp
+        self.device = config.device
+        self.dtype = config.dtype
+        self._dev = _hip.device(resolve_device(config.device))
+        self._A, self._B = device_extrusion(self.n, self.stencil_rows, self.pixel_scale,
+                                            self.L0, self.r0, self._dev)
+        gen = PhaseScreen(self.n, self.pixel_scale, self.r0, self.L0, seed=seed,
+                          device=device, dtype=dtype)
+        self._rng = gen._rng
+        self._innovations = innovations
+        self._capacity = self.n + self.stencil_rows + 6
+        t = _hip.torch()
+        self._buf = t.empty((1, self._capacity, self.n), dtype=t.float32, device=self._dev)
+        self._buf[0, :self.n] = gen.generate().float()
+        self._ring = Ring(self._capacity, self.stencil_rows, 0, self.n)
+        self._travel = 0.0
+        self._count = 0
+        self._advance_to(0.0)
+
+    def _draw(self, k: int):
+        if self._innovations is not None:
+            beta = self._innovations(k)
+        else:
+            beta = _hip.to_device(self._rng.standard_normal((k, self.n), dtype=self.dtype),
+                                  np.float32, self._dev)
+        self._count += k
+        return beta
+
+    def _advance_to(self, travel: float) -> None:
+        if travel < self._travel:
+            raise ValueError("wind travel is monotonic; travel cannot decrease")
+        self._travel = float(travel)
+        top = int(np.floor(self._travel)) + self.n + 2
+        plan = self._ring.plan(top, self._travel)
+        if plan:
+            extrude_rounds(self._A, self._B, self._buf, self._capacity, self.n,
+                           self.stencil_rows, [plan], [self._draw(len(plan))], [1.0])
+        half = 0.5 * (self.n - 1)
+        geom = [[[(self._travel - self._ring.base) + half, -0.5, 1.0]]]
+        out = readout(self._buf, self._capacity, self.n, [self._ring], [(1.0, 0.0)], geom,
+                      self.n, self.interp)
+        self._current = out[0].double() if self.dtype == np.float64 else out[0]
+
+    @property
+    def screen(self) -> Any:
+        return self._current
+
+    def step(self, steps: int = 1) -> Any:
+        steps = int(steps)
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
+        self._advance_to(self._travel + steps)
+        return self._current
+
+    def advance(self, pixels: float) -> Any:
+        pixels = float(pixels)
+        if pixels < 0:
+            raise ValueError("pixels must be >= 0 (wind travel is monotonic)")
+        self._advance_to(self._travel + pixels)
+        return self._current
+
+    @property
+    def travel(self) -> float:
+        return self._travel
+
+    def __repr__(self) -> str:
+        return (f"InfinitePhaseScreen(n={self.n}, pixel_scale={self.pixel_scale}, "
+                f"r0={self.r0}, L0={self.L0}, stencil_rows={self.stencil_rows}, "
+                f"interp={self.interp!r}, device={self.device!r}, dtype={self.dtype.name!r})")

@@ -19,6 +19,15 @@ def _t():
     return _hip.torch()
 
 
+def fast_size(n: int) -> bool:
+    """Power-of-two grids 32..2048 run the fused LDS Stockham kernels."""
+    return 32 <= n <= 2048 and (n & (n - 1)) == 0
+
+
+def _work_factor(n: int) -> int:
+    return 1 if fast_size(n) else 2
+
+
 def _require(t, dtype, shape, name):
     if tuple(t.shape) != tuple(shape):
         raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
@@ -42,7 +51,8 @@ def fft2(x, sign: int = 1, out=None):
     batch = int(np.prod(x.shape[:-2])) if x.dim() > 2 else 1
     if out is None:
         out = t.empty_like(x)
-    rc = _hip.lib().pt_fft2(_hip.ptr(x), _hip.ptr(out), None, batch, n0, n1, int(sign),
+    work = None if fast_size(n0) and n0 == n1 else t.empty_like(x)
+    rc = _hip.lib().pt_fft2(_hip.ptr(x), _hip.ptr(out), _hip.ptr(work), batch, n0, n1, int(sign),
                             _hip.stream_ptr())
     _hip.check(rc, "pt_fft2")
     return out
@@ -73,10 +83,11 @@ def generate(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offset: int
     _require(out, t.float32, (count, n_out, n_out), "out")
     if screen_scale is not None:
         _require(screen_scale, t.float32, (count,), "screen_scale")
+    need = _work_factor(n) * n_fft * n * n
     if work is None:
-        work = t.empty((n_fft, n, n), dtype=t.complex64, device=noise.device)
-    elif work.numel() < n_fft * n * n or work.dtype != t.complex64:
-        raise ValueError("work: need n_fft*n*n complex64")
+        work = t.empty((need,), dtype=t.complex64, device=noise.device)
+    elif work.numel() < need or work.dtype != t.complex64:
+        raise ValueError(f"work: need {need} complex64 elements")
     plane = n_fft * n * n * 4
     base = _hip.ptr(noise)
     rc = _hip.lib().pt_generate(base, base + plane, _hip.ptr(amplitude),
@@ -108,8 +119,9 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
     if out is None:
         out = t.empty((F, n_out, n_out), dtype=t.float32, device=spectra.device)
     _require(out, t.float32, (F, n_out, n_out), "out")
-    if work is None or work.numel() < F * n * n:
-        work = t.empty((F, n, n), dtype=t.complex64, device=spectra.device)
+    need = _work_factor(n) * F * n * n
+    if work is None or work.numel() < need:
+        work = t.empty((need,), dtype=t.complex64, device=spectra.device)
     rc = _hip.lib().pt_spectral_frames(_hip.ptr(spectra), L, n, float(pixel_scale),
                                        _hip.ptr(disp), F, _hip.ptr(sh_coeffs) if P else None,
                                        P, int(crop_offset), n_out, float(scale),
