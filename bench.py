@@ -1,0 +1,383 @@
+"""Benchmark of the MI355X phase-screen hot path.
+
+Headline workload (the 9-layer spectral frozen-flow atmosphere): 512^2
+pupil, D = 8 m, r0 = 0.15 m, L0 = 25 m, heights [0, .5, 1, 2, 4, 8, 12, 16,
+20] km, Cn2 fractions [.5, .1, .08, .07, .06, .06, .05, .05, .03], winds
+U(5, 35) m/s and directions U(0, 2 pi) from seed 2, 8 subharmonic levels,
+dt = 1 ms, 2000 frames, FP32.  One step = all 2000 frames (two fused HIP
+launches); inputs (layer spectra, displacement table) are resident in HBM.
+
+Also reported on the same line:
+  * the Monte-Carlo ensemble: 2048 x 256^2 + 1536 x 512^2 + 512 x 1024^2
+    screens, two outer-scale groups (L0 = 25 m / 50 m, half the Cn2 each),
+    per-screen r0 ~ U(0.08, 0.25) m (seed 4), white noise resident in HBM;
+  * the plain GPU form of both (torch ops, one kernel chain per layer, FP32
+    everywhere -- the reference's array-module algorithm on ROCm);
+  * roofline of the dominant kernel (pass A of the frame engine) from HIP
+    events on the launch stream;
+  * the CPU baseline: the FP64 NumPy oracle timed on this host (1 thread).
+
+``python bench.py [--gpus 1] [--steps K] [--warmup W]`` prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_FRAMES = 2000
+DT = 1e-3
+N = 512
+HEIGHTS = [0.0, 500.0, 1000.0, 2000.0, 4000.0, 8000.0, 12000.0, 16000.0, 20000.0]
+FRACS = [0.5, 0.1, 0.08, 0.07, 0.06, 0.06, 0.05, 0.05, 0.03]
+MC_SIZES = [(256, 2048), (512, 1536), (1024, 512)]
+MC_L0 = [25.0, 50.0]
+
+
+def profile_layers():
+    import pyturb_amd as pa
+
+    rng = np.random.default_rng(2)
+    speeds = rng.uniform(5.0, 35.0, 9)
+    dirs = rng.uniform(0.0, 2.0 * np.pi, 9)
+    return [pa.Layer(h, f, float(v), float(np.degrees(d)), 25.0)
+            for h, f, v, d in zip(HEIGHTS, FRACS, speeds, dirs)]
+
+
+def build_frames_workload(dev):
+    import pyturb_amd as pa
+    from pyturb_amd import _hip
+
+    atm = pa.Atmosphere(profile_layers(), r0=0.15, diameter=8.0, n=N, L0=25.0, subharmonics=8,
+                        seed=2, device="mi355x")
+    times = np.arange(N_FRAMES) * DT
+    dx = atm._vx[None, :] * times[:, None]
+    dy = atm._vy[None, :] * times[:, None]
+    disp = _hip.to_device(np.stack([dx, dy], axis=-1), np.float64, dev)
+    return atm, times, disp
+
+
+def run_frames(atm, disp, out, work, events=None):
+    from pyturb_amd import _hip
+
+    lib = _hip.lib()
+    st = _hip.stream_ptr()
+    L = len(atm.layers)
+    P = atm.subharmonics
+    scale = atm.wavelength / (2.0 * np.pi)
+    if events:
+        events[0].record()
+    rc = lib.pt_spectral_rows(_hip.ptr(atm._spectra), L, N, atm.pixel_scale, _hip.ptr(disp),
+                              N_FRAMES, _hip.ptr(work), st)
+    _hip.check(rc, "pt_spectral_rows")
+    if events:
+        events[1].record()
+    rc = lib.pt_spectral_cols(_hip.ptr(work), L, N, atm.pixel_scale, _hip.ptr(disp), N_FRAMES,
+                              _hip.ptr(atm._sh_coeffs), P, 0, N, float(scale), _hip.ptr(out), st)
+    _hip.check(rc, "pt_spectral_cols")
+    if events:
+        events[2].record()
+
+
+def plain_frame(atm, t, ctx):
+    """The reference's per-frame algorithm as plain torch ops, per layer."""
+    import torch
+
+    spec, f, fp, basis, coeffs = ctx
+    L = spec.shape[0]
+    sx = torch.tensor(atm._vx * t, dtype=torch.float32, device=spec.device)
+    sy = torch.tensor(atm._vy * t, dtype=torch.float32, device=spec.device)
+    S = torch.zeros_like(spec[0])
+    for l in range(L):
+        phx = torch.exp((2j * np.pi) * sx[l] * f)
+        phy = torch.exp((2j * np.pi) * sy[l] * f)
+        S = S + spec[l] * phx[:, None] * phy[None, :]
+    total = (torch.fft.ifft2(S) * (N * N)).real
+    px = torch.exp((2j * np.pi) * sx[None, :, None] * fp[:, None, :])
+    py = torch.exp((2j * np.pi) * sy[None, :, None] * fp[:, None, :])
+    shifted = (coeffs * px[:, :, :, None] * py[:, :, None, :]).sum(dim=1)
+    m = torch.matmul(shifted, basis).reshape(-1, N)
+    low = torch.matmul(basis.reshape(-1, N).transpose(0, 1).contiguous(), m).real
+    low = low - low.mean()
+    return (total + low) * (atm.wavelength / (2.0 * np.pi))
+
+
+def plain_context(atm):
+    import torch
+
+    dev = atm._spectra.device
+    n = atm.n_screen
+    f = torch.tensor(np.fft.fftfreq(n, d=atm.pixel_scale), dtype=torch.float32, device=dev)
+    df = 1.0 / (n * atm.pixel_scale)
+    P = atm.subharmonics
+    fpn = np.array([-1.0, 0.0, 1.0])[None, :] * (df / 3.0 ** np.arange(1, P + 1))[:, None]
+    x = (np.arange(n) - n / 2.0) * atm.pixel_scale
+    basis = np.exp(2j * np.pi * fpn[:, :, None] * x[None, None, :])
+    return (atm._spectra, f, torch.tensor(fpn, dtype=torch.float32, device=dev),
+            torch.tensor(basis, dtype=torch.complex64, device=dev),
+            atm._sh_coeffs.to(torch.complex64))
+
+
+def build_mc(dev):
+    """Mixed-size ensemble state: amplitudes, HBM-resident noise, scales."""
+    import torch
+
+    from pyturb_amd import _hip
+    from pyturb_amd.spectrum import build_filters
+
+    rng = np.random.default_rng(4)
+    total = sum(c for _, c in MC_SIZES)
+    r0 = rng.uniform(0.08, 0.25, total)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    classes = []
+    first = 0
+    for n, count in MC_SIZES:
+        dx = 8.0 / n
+        groups = []
+        # each L0 group carries half the Cn2: r0_g^(-5/3) = 0.5 r0^(-5/3)
+        scale = torch.tensor((r0[first:first + count] * 0.5 ** -0.6) ** (-5.0 / 6.0),
+                             dtype=torch.float32, device=dev)
+        for L0 in MC_L0:
+            amp, _, sh_amp = build_filters(n, dx, 1.0, L0, 8)
+            n_fft = (count + 1) // 2
+            noise = torch.randn((2, n_fft, n, n), generator=gen, device=dev, dtype=torch.float32)
+            shn = torch.randn((8, 2, count, 3, 3), generator=gen, device=dev, dtype=torch.float32)
+            groups.append(dict(amp=_hip.to_device(amp, np.float32, dev),
+                               sh_amp=_hip.to_device(sh_amp, np.float64, dev),
+                               amp_host=amp, sh_amp_host=sh_amp, noise=noise, shn=shn))
+        out = torch.empty((count, n, n), dtype=torch.float32, device=dev)
+        work = torch.empty(((count + 1) // 2 * n * n,), dtype=torch.complex64, device=dev)
+        classes.append(dict(n=n, count=count, scale=scale, groups=groups, out=out, work=work,
+                            r0=r0[first:first + count]))
+        first += count
+    return classes
+
+
+def run_mc(classes):
+    from pyturb_amd import ops
+
+    for c in classes:
+        for g, grp in enumerate(c["groups"]):
+            ops.generate(grp["noise"], grp["shn"], grp["amp"], grp["sh_amp"], c["count"],
+                         screen_scale=c["scale"], out=c["out"], accumulate=g > 0, work=c["work"])
+
+
+def run_mc_plain(classes):
+    """Plain form: reference generate() algorithm as torch ops per group."""
+    import torch
+
+    for c in classes:
+        n, count = c["n"], c["count"]
+        total = None
+        for grp in c["groups"]:
+            ctx = grp.setdefault("plain", _plain_gen_ctx(grp, n))
+            amp, basis, shamp = ctx
+            nz = grp["noise"]
+            spec = torch.complex(nz[0], nz[1]) * amp
+            field = torch.fft.ifft2(spec) * (n * n)
+            scr = torch.cat((field.real, field.imag))[:count]
+            shn = grp["shn"]
+            cn = torch.complex(shn[:, 0], shn[:, 1]) * shamp  # (P, count, 3, 3)
+            m = torch.matmul(cn, basis[:, None]).transpose(0, 1).reshape(count, -1, n)
+            low = torch.matmul(basis.reshape(-1, n).transpose(0, 1).contiguous()[None], m).real
+            low = low - low.mean(dim=(-2, -1), keepdim=True)
+            scr = (scr + low) * c["scale"][:, None, None]
+            total = scr if total is None else total + scr
+        c["plain_out"] = total
+
+
+def _plain_gen_ctx(grp, n):
+    import torch
+
+    dev = grp["amp"].device
+    P = grp["sh_amp_host"].shape[0]
+    fpn = np.array([-1.0, 0.0, 1.0])[None, :] / (n * 3.0 ** np.arange(1, P + 1))[:, None]
+    x = np.arange(n) - n / 2.0
+    basis = torch.tensor(np.exp(2j * np.pi * fpn[:, :, None] * x[None, None, :]),
+                         dtype=torch.complex64, device=dev)
+    return (grp["amp"], basis, torch.tensor(grp["sh_amp_host"], dtype=torch.complex64,
+                                            device=dev)[:, None])
+
+
+def timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(atm, budget_s=12.0):
+    """FP64 oracle frames on this host, single thread, bounded sample."""
+    import oracle as O
+
+    spectra = atm._spectra.cpu().numpy().astype(np.complex128)
+    coeffs = atm._sh_coeffs.cpu().numpy()
+    f = np.fft.fftfreq(N, d=atm.pixel_scale)
+    filt = O.build_filters(N, atm.pixel_scale, 0.15, 25.0, subharmonics=atm.subharmonics)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        t = done * DT
+        O.integrate(spectra, coeffs, f, filt["sh_freq"], filt["sh_basis"], atm._vx * t,
+                    atm._vy * t)
+        done += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or done >= 400:
+            break
+    return done / el, done, el
+
+
+def parity(atm, out, frames=(0, 1000, 1999)):
+    import oracle as O
+
+    spectra = atm._spectra.cpu().numpy().astype(np.complex128)
+    coeffs = atm._sh_coeffs.cpu().numpy()
+    f = np.fft.fftfreq(N, d=atm.pixel_scale)
+    filt = O.build_filters(N, atm.pixel_scale, 0.15, 25.0, subharmonics=atm.subharmonics)
+    worst = 0.0
+    scale = atm.wavelength / (2 * np.pi)
+    for k in frames:
+        t = k * DT
+        ref = O.integrate(spectra, coeffs, f, filt["sh_freq"], filt["sh_basis"], atm._vx * t,
+                          atm._vy * t) * scale
+        worst = max(worst, O.rel_rms(out[k].cpu().numpy(), ref))
+    return worst
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
+    ap.add_argument("--no-mc", action="store_true", help="skip the Monte-Carlo ensemble")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 3 times (counter scaling)")
+    args = ap.parse_args(argv)
+    if args.copy_calibration:
+        import torch
+
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 4, "copies": 3}))
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the frame and Monte-Carlo workloads shard by frames/screens with no "
+              "exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+    import torch
+
+    from pyturb_amd import _hip
+
+    dev = _hip.device(0)
+    print("bench: building 9-layer 512^2 spectral workload", file=sys.stderr, flush=True)
+    atm, times, disp = build_frames_workload(dev)
+    out = torch.empty((N_FRAMES, N, N), dtype=torch.float32, device=dev)
+    work = torch.empty((N_FRAMES, N, N), dtype=torch.complex64, device=dev)
+    L = len(atm.layers)
+
+    for _ in range(args.warmup):
+        run_frames(atm, disp, out, work)
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run_frames(atm, disp, out, work, evs[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fps = args.steps * N_FRAMES / elapsed
+    t_a = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) * 1e-3
+    t_b = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) * 1e-3
+    bytes_a = 8.0 * L * N * N * N_FRAMES            # layer stack read per frame
+    bytes_b = 4.0 * N * N * N_FRAMES                # OPD written per frame
+    ach_a = bytes_a / t_a / 1e9
+    frame_bytes = (8.0 * L + 4.0) * N * N
+    print(f"bench: frames {fps:.0f}/s pass A {t_a*1e3:.3f} ms pass B {t_b*1e3:.3f} ms",
+          file=sys.stderr, flush=True)
+    par = parity(atm, out)
+
+    extra = {}
+    if not args.no_plain:
+        ctx = plain_context(atm)
+        nplain = 200
+        tp = timed(lambda: [plain_frame(atm, float(t), ctx) for t in times[:nplain]], 1, 1)
+        extra["plain_frames_per_s"] = round(nplain / tp, 1)
+        extra["fused_vs_plain_frames"] = round(fps / (nplain / tp), 1)
+    if not args.no_mc:
+        print("bench: building Monte-Carlo ensemble", file=sys.stderr, flush=True)
+        classes = build_mc(dev)
+        nscr = sum(c for _, c in MC_SIZES)
+        tm = timed(lambda: run_mc(classes), max(1, args.steps // 2), 1)
+        mc_rate = nscr * max(1, args.steps // 2) / tm
+        px = sum(n * n * c for n, c in MC_SIZES)
+        mc_bytes = (4.0 * len(MC_L0) + 4.0) * px
+        extra["mc_screens_per_s"] = round(mc_rate, 1)
+        extra["mc_ms_per_ensemble"] = round(1e3 * tm / max(1, args.steps // 2), 3)
+        extra["mc_algorithmic_GBs"] = round(mc_bytes * mc_rate / nscr / 1e9, 1)
+        if not args.no_plain:
+            tpm = timed(lambda: run_mc_plain(classes), 1, 1)
+            extra["mc_plain_screens_per_s"] = round(nscr / tpm, 1)
+            extra["fused_vs_plain_mc"] = round(mc_rate / (nscr / tpm), 1)
+        del classes
+        torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu:
+        print("bench: CPU baseline (FP64 oracle, 1 thread)", file=sys.stderr, flush=True)
+        rate, done, el = cpu_baseline(atm)
+        cpu = {"value": round(rate, 2), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"{done} frames of the same 9-layer 512^2 workload, FP64 NumPy "
+                         f"oracle (pocketfft), {el:.1f} s"}
+    line = {
+        "metric": "9-layer 512^2 spectral OPD frames/s (fused HIP, FP32)",
+        "value": round(fps, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded host-drawn layer spectra and winds; device-drawn MC noise)",
+        "config": {"workload": "9-layer 512^2 spectral frozen flow, 2000 frames, dt=1 ms",
+                   "layers": L, "n": N, "frames_per_step": N_FRAMES, "subharmonics": 8},
+        "roofline": {"bound": "hbm", "achieved": round(ach_a, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach_a / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_frame_rows<512> (pass A)",
+                     "pass_a_ms": round(t_a * 1e3, 4), "pass_b_ms": round(t_b * 1e3, 4),
+                     "frame_algorithmic_GBs": round(frame_bytes * fps / 1e9, 1),
+                     "pass_b_GBs": round(bytes_b / t_b / 1e9, 1)},
+        "cpu_baseline": cpu,
+        "parity_rel_rms_vs_fp64": float(f"{par:.3g}"),
+        "reference_published_rtx5090_fps_512": 3133,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

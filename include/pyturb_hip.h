@@ -100,6 +100,21 @@ int pt_spectral_frames(const void* spectra, int layers, int n,
                        void* stream);
 
 /*
+ * pt_spectral_frames split into its two passes (power-of-two n only), for
+ * callers that schedule or time them separately:
+ *   pass A: work[f] = row-transformed phasor-weighted layer sum of frame f
+ *   pass B: column transform + subharmonics + crop + scale into out
+ * pt_spectral_rows then pt_spectral_cols on one stream == pt_spectral_frames.
+ */
+int pt_spectral_rows(const void* spectra, int layers, int n,
+                     double pixel_scale, const double* disp, int frames,
+                     void* work, void* stream);
+int pt_spectral_cols(const void* work, int layers, int n, double pixel_scale,
+                     const double* disp, int frames, const double* sh_coeffs,
+                     int levels, int crop_offset, int n_out, float scale,
+                     float* out, void* stream);
+
+/*
  * Innovation terms of the Assemat-Wilson recurrence for a batch of new rows:
  *   out[k][w] = sum_j B_unit[w][j] * beta[k][j]      (out = beta * B^T)
  *   beta (rows, width) f32 white noise, B_unit (width, width) f32, out

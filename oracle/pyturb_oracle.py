@@ -60,6 +60,8 @@ __all__ = [
     "catmull_rom_weights",
     "lanczos_weights",
     "readout",
+    "pupil_grids",
+    "along_extent_and_span",
     "OracleExtruder",
     "OracleInfinite",
     "structure_function",

@@ -307,24 +307,44 @@ static int generate_pow2(const float* nre, const float* nim, const float* amp, G
 }
 
 template <int N>
-static int frames_pow2(const float2* spectra, int L, const double* disp, int F, FrameArgs a, hipStream_t st) {
-  constexpr int MR = RowsCfg<N>::M, MC = ColsCfg<N>::M;
-  const size_t dyn_r = (size_t)L * N * 8 + (size_t)L * MR * 8 + (size_t)2 * L * 4;
+static size_t frames_rows_lds(int L) {
+  return (size_t)L * N * 8 + (size_t)L * RowsCfg<N>::M * 8 + (size_t)2 * L * 4;
+}
+
+template <int N>
+static int frames_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
+                            float2* work, hipStream_t st) {
+  constexpr int MR = RowsCfg<N>::M;
+  const size_t dyn_r = frames_rows_lds<N>(L);
   if (dyn_r + (size_t)MR * N * 8 + (size_t)N * 8 > kLdsMax)
     return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor table at n=%d", L, N);
+  int rc = set_dyn_lds(k_frame_rows<N>, dyn_r);
+  if (rc) return rc;
+  k_frame_rows<N><<<dim3(N / MR, F), RowsCfg<N>::T, dyn_r, st>>>(spectra, L, disp, inv_period, work);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+template <int N>
+static int frames_cols_pow2(int F, FrameArgs a, hipStream_t st) {
+  constexpr int MC = ColsCfg<N>::M;
   const size_t stat = static_lds_cols(N, MC);
-  const size_t extra = a.levels > 0 ? (size_t)a.levels * L * 16 * 2 + (size_t)a.levels * 9 * 16 : 0;
+  const size_t extra = a.levels > 0 ? (size_t)a.levels * a.L * 16 * 2 + (size_t)a.levels * 9 * 16 : 0;
   a.u_in_lds = a.levels > 0 && stat + sh_lds_bytes(N, a.levels, 1, MC, true) + extra <= kLdsMax;
   const size_t dyn_c = sh_lds_bytes(N, a.levels, 1, MC, a.u_in_lds != 0) + extra;
   if (stat + dyn_c > kLdsMax) return fail(PT_ERR_UNSUPPORTED, "frames: subharmonic tables exceed LDS at n=%d", N);
-  int rc = set_dyn_lds(k_frame_rows<N>, dyn_r);
+  int rc = set_dyn_lds(k_frame_cols<N>, dyn_c);
   if (rc) return rc;
-  rc = set_dyn_lds(k_frame_cols<N>, dyn_c);
-  if (rc) return rc;
-  k_frame_rows<N><<<dim3(N / MR, F), RowsCfg<N>::T, dyn_r, st>>>(spectra, L, disp, a.inv_period, (float2*)a.work);
   k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, dyn_c, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
+}
+
+template <int N>
+static int frames_pow2(const float2* spectra, int L, const double* disp, int F, FrameArgs a, hipStream_t st) {
+  int rc = frames_rows_pow2<N>(spectra, L, disp, F, a.inv_period, (float2*)a.work, st);
+  if (rc) return rc;
+  return frames_cols_pow2<N>(F, a, st);
 }
 
 #define PT_POW2_SWITCH(n, CALL) \
@@ -397,6 +417,62 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
   }
   return pt_generate_generic(noise_re, noise_im, amplitude, sh_noise, sh_amp, levels, count, n, crop_offset,
                              n_out, scale, screen_scale, accumulate, out, work, stream);
+}
+
+// The two passes of pt_spectral_frames as separate calls (power-of-two n):
+// pass A writes the row-transformed layer sum to `work`, pass B finishes the
+// column transform and the epilogue.  Same arguments as pt_spectral_frames.
+static int frames_check(const void* spectra, int layers, int n, double pixel_scale, const double* disp, int frames,
+                        const double* sh_coeffs, int levels, int crop_offset, int n_out, float* out, void* work) {
+  if (!disp || !work) return fail(PT_ERR_ARG, "spectral pass: NULL array");
+  if (layers < 1 || frames < 0 || levels < 0 || (levels > 0 && !sh_coeffs))
+    return fail(PT_ERR_ARG, "spectral pass: bad layers/frames/levels");
+  if (!(pixel_scale > 0.0)) return fail(PT_ERR_ARG, "spectral pass: pixel_scale must be > 0");
+  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "spectral pass: crop outside grid");
+  if (frames > 65535) return fail(PT_ERR_ARG, "spectral pass: frames > 65535 per call");
+  if (!is_pow2_fast(n)) return fail(PT_ERR_UNSUPPORTED, "spectral pass: split passes need a power-of-two n");
+  (void)spectra;
+  (void)out;
+  return PT_OK;
+}
+
+extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
+                                int frames, void* work, void* stream) {
+  if (!spectra) return fail(PT_ERR_ARG, "pt_spectral_rows: NULL spectra");
+  int rc = frames_check(spectra, layers, n, pixel_scale, disp, frames, nullptr, 0, 0, n, nullptr, work);
+  if (rc || frames == 0) return rc;
+  const double inv_period = 1.0 / ((double)n * pixel_scale);
+  hipStream_t st = (hipStream_t)stream;
+#define CALL(N) return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
+  PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: n=%d", n);
+}
+
+extern "C" int pt_spectral_cols(const void* work, int layers, int n, double pixel_scale, const double* disp,
+                                int frames, const double* sh_coeffs, int levels, int crop_offset, int n_out,
+                                float scale, float* out, void* stream) {
+  if (!out) return fail(PT_ERR_ARG, "pt_spectral_cols: NULL out");
+  int rc = frames_check(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, out,
+                        (void*)work);
+  if (rc || frames == 0) return rc;
+  FrameArgs a;
+  a.work = (const float2*)work;
+  a.out = out;
+  a.disp = disp;
+  a.coeffs = (const double2*)sh_coeffs;
+  a.inv_period = 1.0 / ((double)n * pixel_scale);
+  a.L = layers;
+  a.levels = levels;
+  a.off = crop_offset;
+  a.n_out = n_out;
+  a.u_in_lds = 0;
+  a.scale = scale;
+  hipStream_t st = (hipStream_t)stream;
+#define CALL(N) return frames_cols_pow2<N>(frames, a, st);
+  PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: n=%d", n);
 }
 
 extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
