@@ -156,7 +156,9 @@ def build_mc(dev):
                                sh_amp=_hip.to_device(sh_amp, np.float64, dev),
                                amp_host=amp, sh_amp_host=sh_amp, noise=noise, shn=shn))
         out = torch.empty((count, n, n), dtype=torch.float32, device=dev)
-        work = torch.empty(((count + 1) // 2 * n * n,), dtype=torch.complex64, device=dev)
+        from pyturb_amd.ops import work_elems
+
+        work = torch.empty((work_elems(0, n, count, 8),), dtype=torch.complex64, device=dev)
         classes.append(dict(n=n, count=count, scale=scale, groups=groups, out=out, work=work,
                             r0=r0[first:first + count]))
         first += count
@@ -294,7 +296,10 @@ def main(argv=None):
     print("bench: building 9-layer 512^2 spectral workload", file=sys.stderr, flush=True)
     atm, times, disp = build_frames_workload(dev)
     out = torch.empty((N_FRAMES, N, N), dtype=torch.float32, device=dev)
-    work = torch.empty((N_FRAMES, N, N), dtype=torch.complex64, device=dev)
+    from pyturb_amd.ops import work_elems
+
+    work = torch.empty((work_elems(1, N, N_FRAMES, atm.subharmonics),), dtype=torch.complex64,
+                       device=dev)
     L = len(atm.layers)
 
     for _ in range(args.warmup):

@@ -38,6 +38,15 @@ const char* pt_last_error(void);
 int pt_abi_version(void);
 
 /*
+ * Workspace (in c64 elements) that pt_generate (kind 0, sets = count) or
+ * pt_spectral_frames / pt_spectral_rows+cols (kind 1, sets = frames) needs
+ * for an n x n grid with `levels` subharmonic levels: the FFT intermediate
+ * (doubled on the general-size path) followed by the subharmonic tables.
+ * -1 on bad arguments.  Host-only, no device work.
+ */
+long long pt_work_elems(int kind, int n, int sets, int levels);
+
+/*
  * Unnormalised 2-D DFT over the last two axes of `in` (batch, n0, n1) c64,
  * written to `out` (same shape; may equal `in`).  Powers of two from 32 to
  * 2048 per axis run the LDS Stockham passes; other sizes run the direct
@@ -64,8 +73,8 @@ int pt_fft2(const void* in, void* out, void* work, int batch, int n0, int n1,
  *   out                : (count, n_out, n_out) f32; screen s pixel (x, y) is
  *                        full-grid pixel (x + crop_offset, y + crop_offset);
  *                        accumulate != 0 adds into out
- *   work               : n_fft * n * n c64 (2 * n_fft * n * n when n is not
- *                        a power of two in 32..2048: general-size path)
+ *   work               : pt_work_elems(0, n, count, levels) c64: the FFT
+ *                        intermediate, then the subharmonic tables
  * Screen k < n_fft is the real part of FFT k, screen n_fft + k its
  * imaginary part (fourier.py:260).
  */
@@ -88,8 +97,7 @@ int pt_generate(const float* noise_re, const float* noise_im,
  *   out       : (frames, n_out, n_out) f32,
  *               out = scale * Re(ifft2(sum_l S_l P_x P_y) * n^2 + low)
  *               cropped at crop_offset
- *   work      : frames * n * n c64 (twice that for the general-size path,
- *               n not a power of two in 32..2048)
+ *   work      : pt_work_elems(1, n, frames, levels) c64
  * Shift-theorem phases are reduced exactly (32-bit fixed-point cycles), so
  * large travel keeps full FP32 phase accuracy.
  */
@@ -100,16 +108,17 @@ int pt_spectral_frames(const void* spectra, int layers, int n,
                        void* stream);
 
 /*
- * pt_spectral_frames split into its two passes (power-of-two n only), for
- * callers that schedule or time them separately:
+ * pt_spectral_frames split into its two passes (power-of-two n, <= 12
+ * subharmonic levels), for callers that schedule or time them separately:
  *   pass A: work[f] = row-transformed phasor-weighted layer sum of frame f
- *   pass B: column transform + subharmonics + crop + scale into out
+ *   pass B: subharmonic tables (one block per frame), then column transform
+ *           + subharmonics + crop + scale into out
  * pt_spectral_rows then pt_spectral_cols on one stream == pt_spectral_frames.
  */
 int pt_spectral_rows(const void* spectra, int layers, int n,
                      double pixel_scale, const double* disp, int frames,
                      void* work, void* stream);
-int pt_spectral_cols(const void* work, int layers, int n, double pixel_scale,
+int pt_spectral_cols(void* work, int layers, int n, double pixel_scale,
                      const double* disp, int frames, const double* sh_coeffs,
                      int levels, int crop_offset, int n_out, float scale,
                      float* out, void* stream);

@@ -13,10 +13,10 @@
 // through LDS in place: read, barrier, write, barrier.
 //
 // Two data layouts are supported:
-//   ROWS  (COLS=false): sequence m occupies LDS[m*N + i]; butterfly b of a
+//   ROWS  (COLS=false): sequence m occupies LDS row m (padded); butterfly b of a
 //         stage maps to (m = b / (N/R), j = b % (N/R)), so consecutive lanes
 //         walk along a sequence -> coalesced row access in HBM.
-//   COLS  (COLS=true):  element i of sequence m sits at LDS[i*M + m];
+//   COLS  (COLS=true):  element i of sequence m sits at LDS[row(i)*M + m];
 //         butterfly b maps to (m = b % M, j = b / M), so consecutive lanes
 //         walk across M adjacent columns -> M*8-byte segments in HBM.
 //
@@ -118,10 +118,33 @@ __device__ __forceinline__ void init_twiddles(float2* tw, int tid, int nthreads)
   }
 }
 
+// Element i of a sequence is stored at padded row i + i/8: the radix-8
+// Stockham write pattern (index 8j + r across lanes) then spreads over all
+// LDS banks instead of hitting one bank group 4-16 times (64 banks x 4 B).
+template <int N>
+struct Pad {
+  static constexpr int ROWS = N + N / 8;
+  __device__ static __forceinline__ int at(int i) { return i + (i >> 3); }
+};
+
+// Column tiles (COLS): a half-wave (32 lanes) covers 32/M rows of M*8 bytes;
+// the radix-8 write pattern puts those rows 8 apart, i.e. on the same
+// banks.  Rows are stored permuted inside each aligned group of eight,
+// row ^ ((row / 8) mod (32/M)), which sends them to distinct bank slices
+// while consecutive rows (the read pattern) stay distinct too.  No padding.
+template <int N, int M>
+struct Swz {
+  static constexpr int MASK = M >= 32 ? 0 : 32 / M - 1;
+  __device__ static __forceinline__ int row(int i) { return i ^ ((i >> 3) & MASK); }
+};
+
 template <int N, int T, int M, bool COLS>
 struct Layout {
   static_assert(M * N == 8 * T, "eight values per thread");
-  __device__ static __forceinline__ int addr(int m, int i) { return COLS ? i * M + m : m * N + i; }
+  static constexpr int SIZE = COLS ? M * N : M * Pad<N>::ROWS;  // float2 elements of LDS
+  __device__ static __forceinline__ int addr(int m, int i) {
+    return COLS ? Swz<N, M>::row(i) * M + m : m * Pad<N>::ROWS + Pad<N>::at(i);
+  }
 };
 
 // One Stockham stage ST.  For ST == 0 inputs come from `load(m, i)`; for the

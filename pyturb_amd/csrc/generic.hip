@@ -119,102 +119,57 @@ __global__ void k_frame_pro(const float2* __restrict__ spectra, int L, int n, co
 }
 
 // ---------------------------------------------------------------- epilogues
-constexpr int EM = 32;  // columns per epilogue workgroup
-
+// One thread per output pixel; subharmonic field from the prep tables.
 struct GenEpiArgs {
   const float2* work;
   float* out;
-  const float* sh_noise;
-  const double* sh_amp;
   const float* screen_scale;
+  SubTables tab;
   int n, count, n_fft, levels, off, n_out, accumulate;
   float scale;
 };
 
 __global__ __launch_bounds__(256) void k_gen_epi(GenEpiArgs a) {
-  extern __shared__ __align__(16) unsigned char dyn[];
-  const int tid = threadIdx.x, n = a.n, P = a.levels;
-  const int b = blockIdx.y, c0 = blockIdx.x * EM;
-  if (c0 + EM <= a.off || c0 >= a.off + a.n_out) return;
+  const int n = a.n, P = a.levels;
+  const int b = blockIdx.z;
+  const int y = blockIdx.x * 32 + (threadIdx.x & 31);
+  const int x = blockIdx.y * 8 + (threadIdx.x >> 5);
+  if (x >= a.n_out || y >= a.n_out) return;
+  const int i = x + a.off, j = y + a.off;
   const int s0 = b, s1 = b + a.n_fft;
-  const bool has1 = s1 < a.count;
-  ShLds sh = sh_carve(dyn, n, P, 2, EM, false);
-  if (P > 0) {
-    auto valid = [&](int t) { return t == 0 || has1; };
-    auto block = [&](int t, int p, double2* C) {
-      const int s = t == 0 ? s0 : s1;
-      const float* re = a.sh_noise + ((size_t)(p * 2 + 0) * a.count + s) * 9;
-      const float* im = a.sh_noise + ((size_t)(p * 2 + 1) * a.count + s) * 9;
-      for (int k = 0; k < 9; ++k) {
-        const double am = a.sh_amp[p * 9 + k];
-        C[k] = make_double2((double)re[k] * am, (double)im[k] * am);
-      }
-    };
-    sh_setup(sh, n, P, 2, EM, c0, tid, 256, block, valid);
-  }
-  __syncthreads();
-  const int m = tid % EM;
-  const int y = c0 + m - a.off;
-  if (c0 + m >= n || (unsigned)y >= (unsigned)a.n_out) return;
-  const size_t oplane = (size_t)a.n_out * a.n_out;
+  const float2 v = a.work[(size_t)b * n * n + (size_t)i * n + j];
+  const size_t oplane = (size_t)a.n_out * a.n_out, o = (size_t)x * a.n_out + y;
+  const float l0 = P > 0 ? sub_eval(a.tab, s0, i, j, n, P) : 0.f;
   const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
-  const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
-  for (int i = tid / EM; i < n; i += 256 / EM) {
-    const int x = i - a.off;
-    if ((unsigned)x >= (unsigned)a.n_out) continue;
-    const float2 v = a.work[(size_t)b * n * n + (size_t)i * n + c0 + m];
-    const float l0 = P > 0 ? sh_eval(sh, 0, m, EM, i, n, P) : 0.f;
-    const size_t o = (size_t)x * a.n_out + y;
-    float* d0 = a.out + (size_t)s0 * oplane + o;
-    const float v0 = (v.x + l0) * f0;
-    *d0 = a.accumulate ? *d0 + v0 : v0;
-    if (has1) {
-      const float l1 = P > 0 ? sh_eval(sh, 1, m, EM, i, n, P) : 0.f;
-      float* d1 = a.out + (size_t)s1 * oplane + o;
-      const float v1 = (v.y + l1) * f1;
-      *d1 = a.accumulate ? *d1 + v1 : v1;
-    }
+  float* d0 = a.out + (size_t)s0 * oplane + o;
+  const float v0 = (v.x + l0) * f0;
+  *d0 = a.accumulate ? *d0 + v0 : v0;
+  if (s1 < a.count) {
+    const float l1 = P > 0 ? sub_eval(a.tab, s1, i, j, n, P) : 0.f;
+    const float f1 = a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f);
+    float* d1 = a.out + (size_t)s1 * oplane + o;
+    const float v1 = (v.y + l1) * f1;
+    *d1 = a.accumulate ? *d1 + v1 : v1;
   }
 }
 
 struct FrameEpiArgs {
   const float2* work;
   float* out;
-  const double* disp;
-  const double2* coeffs;
-  double inv_period;
-  int n, L, levels, off, n_out;
+  SubTables tab;
+  int n, levels, off, n_out;
   float scale;
 };
 
 __global__ __launch_bounds__(256) void k_frame_epi(FrameEpiArgs a) {
-  extern __shared__ __align__(16) unsigned char dyn[];
-  const int tid = threadIdx.x, n = a.n, P = a.levels, L = a.L;
-  const int f = blockIdx.y, c0 = blockIdx.x * EM;
-  if (c0 + EM <= a.off || c0 >= a.off + a.n_out) return;
-  ShLds sh = sh_carve(dyn, n, P, 1, EM, false);
-  double2* Q = (double2*)(dyn + sh_lds_bytes(n, P, 1, EM, false));
-  double2* SB = Q + 2 * (size_t)P * L;
-  if (P > 0) {
-    frame_sh_blocks(Q, SB, a.disp, a.coeffs, f, L, P, a.inv_period, tid, 256);
-    auto valid = [&](int t) { return true; };
-    auto block = [&](int t, int p, double2* C) {
-      for (int k = 0; k < 9; ++k) C[k] = SB[p * 9 + k];
-    };
-    sh_setup(sh, n, P, 1, EM, c0, tid, 256, block, valid);
-  }
-  __syncthreads();
-  const int m = tid % EM;
-  const int y = c0 + m - a.off;
-  if (c0 + m >= n || (unsigned)y >= (unsigned)a.n_out) return;
-  float* dst = a.out + (size_t)f * a.n_out * a.n_out;
-  for (int i = tid / EM; i < n; i += 256 / EM) {
-    const int x = i - a.off;
-    if ((unsigned)x >= (unsigned)a.n_out) continue;
-    const float2 v = a.work[(size_t)f * n * n + (size_t)i * n + c0 + m];
-    const float lo = P > 0 ? sh_eval(sh, 0, m, EM, i, n, P) : 0.f;
-    dst[(size_t)x * a.n_out + y] = (v.x + lo) * a.scale;
-  }
+  const int n = a.n, P = a.levels, f = blockIdx.z;
+  const int y = blockIdx.x * 32 + (threadIdx.x & 31);
+  const int x = blockIdx.y * 8 + (threadIdx.x >> 5);
+  if (x >= a.n_out || y >= a.n_out) return;
+  const int i = x + a.off, j = y + a.off;
+  const float2 v = a.work[(size_t)f * n * n + (size_t)i * n + j];
+  const float lo = P > 0 ? sub_eval(a.tab, f, i, j, n, P) : 0.f;
+  a.out[((size_t)f * a.n_out + x) * a.n_out + y] = (v.x + lo) * a.scale;
 }
 
 static int grid_stride_blocks(size_t total) {
@@ -248,9 +203,8 @@ int pt_generate_generic(const float* nre, const float* nim, const float* amp, co
   GenEpiArgs a;
   a.work = w0;
   a.out = out;
-  a.sh_noise = sh_noise;
-  a.sh_amp = sh_amp;
   a.screen_scale = screen_scale;
+  a.tab = sub_tables(w0 + 2 * total, levels, n, count);
   a.n = n;
   a.count = count;
   a.n_fft = n_fft;
@@ -259,8 +213,9 @@ int pt_generate_generic(const float* nre, const float* nim, const float* amp, co
   a.n_out = n_out;
   a.accumulate = accumulate;
   a.scale = scale;
-  const size_t dyn = sh_lds_bytes(n, levels, 2, EM, false);
-  k_gen_epi<<<dim3((n + EM - 1) / EM, n_fft), 256, dyn, st>>>(a);
+  rc = launch_gen_prep(sh_noise, sh_amp, count, levels, n, a.tab, st);
+  if (rc) return rc;
+  k_gen_epi<<<dim3((n_out + 31) / 32, (n_out + 7) / 8, n_fft), 256, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -281,18 +236,15 @@ int pt_frames_generic(const void* spectra, int L, int n, double pixel_scale, con
   FrameEpiArgs a;
   a.work = w0;
   a.out = out;
-  a.disp = disp;
-  a.coeffs = (const double2*)coeffs;
-  a.inv_period = inv_period;
+  a.tab = sub_tables(w0 + 2 * total, levels, n, frames);
   a.n = n;
-  a.L = L;
   a.levels = levels;
   a.off = off;
   a.n_out = n_out;
   a.scale = scale;
-  const size_t dyn = sh_lds_bytes(n, levels, 1, EM, false) + (levels ? (size_t)levels * L * 32 + levels * 9 * 16 : 0);
-  if (dyn > 64 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: subharmonic tables exceed LDS");
-  k_frame_epi<<<dim3((n + EM - 1) / EM, frames), 256, dyn, st>>>(a);
+  rc = launch_frame_prep(disp, (const double2*)coeffs, frames, L, levels, n, inv_period, a.tab, st);
+  if (rc) return rc;
+  k_frame_epi<<<dim3((n_out + 31) / 32, (n_out + 7) / 8, frames), 256, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }

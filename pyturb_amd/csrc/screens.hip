@@ -3,19 +3,17 @@
 //
 // A 2-D transform of an N x N grid is two passes over HBM:
 //   pass A (rows):    256-thread workgroups (N*N/8 when smaller), M = 8T/N
-//                     rows each; the
-//                     prologue builds the spectrum on the fly (noise x PSD
-//                     amplitude, or the phasor-weighted layer sum) and the
-//                     rows are transformed along axis 1 in LDS;
+//                     rows each; the prologue builds the spectrum on the fly
+//                     (noise x PSD amplitude, or the phasor-weighted layer
+//                     sum) and the rows are transformed along axis 1 in LDS;
 //   pass B (columns): up to 1024-thread workgroups, a tile of M adjacent
 //                     columns transformed along axis 0 in LDS; the epilogue
-//                     takes the real/imag parts, adds the subharmonic field,
-//                     scales, crops and writes float32 screens.
-// The intermediate (one c64 grid per transform) is the only extra traffic.
+//                     takes the real/imag parts, adds the subharmonic field
+//                     from the prep tables, scales, crops and writes float32.
+// A small prep kernel (one block per screen or frame) fills the subharmonic
+// tables between the passes.  The intermediate (one c64 grid per transform)
+// is the only extra HBM traffic.
 #include <hip/hip_runtime.h>
-
-#include <stdarg.h>
-#include <stdio.h>
 
 #include "common.hpp"
 #include "fft_core.hpp"
@@ -35,12 +33,63 @@ struct ColsCfg {
 };
 
 // ----------------------------------------------------------------------------
+// prep kernels (subharmonic tables)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_noise,
+                                                  const double* __restrict__ sh_amp, int count, int P, int n,
+                                                  SubTables t) {
+  __shared__ double2 C[PMAX * 9];
+  __shared__ double s_mean;
+  const int s = blockIdx.x, tid = threadIdx.x;
+  for (int q = tid; q < P * 9; q += 256) {
+    const int p = q / 9, k = q % 9;
+    const double am = sh_amp[q];
+    C[q] = make_double2((double)sh_noise[((size_t)(p * 2 + 0) * count + s) * 9 + k] * am,
+                        (double)sh_noise[((size_t)(p * 2 + 1) * count + s) * 9 + k] * am);
+  }
+  __syncthreads();
+  sub_fill_set(C, P, n, s, t, s == 0, &s_mean, tid, 256);
+}
+
+__global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ disp,
+                                                    const double2* __restrict__ coeffs, int L, int P, int n,
+                                                    double inv_period, SubTables t) {
+  extern __shared__ __align__(16) unsigned char dyn[];
+  __shared__ double s_mean;
+  double2* Q = (double2*)dyn;
+  double2* SB = Q + 2 * (size_t)P * L;
+  const int f = blockIdx.x, tid = threadIdx.x;
+  frame_sh_blocks(Q, SB, disp, coeffs, f, L, P, inv_period, tid, 256);
+  sub_fill_set(SB, P, n, f, t, f == 0, &s_mean, tid, 256);
+}
+
+int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
+                    hipStream_t st) {
+  if (P == 0) return PT_OK;
+  if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
+  k_gen_prep<<<count, 256, 0, st>>>(sh_noise, sh_amp, count, P, n, t);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
+                      SubTables t, hipStream_t st) {
+  if (P == 0) return PT_OK;
+  if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
+  const size_t dyn = (size_t)(2 * P * L + 9 * P) * sizeof(double2);
+  if (dyn > 60 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: %d layers x %d levels exceed LDS", L, P);
+  k_frame_prep<<<F, 256, dyn, st>>>(disp, coeffs, L, P, n, inv_period, t);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+// ----------------------------------------------------------------------------
 // plain 2-D transform passes (pt_fft2)
 // ----------------------------------------------------------------------------
 template <int N, int S>
 __global__ __launch_bounds__(256) void k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, int n1) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
-  __shared__ float2 lds[M * N];
+  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   init_twiddles<N, S>(tw, tid, T);
@@ -54,7 +103,7 @@ __global__ __launch_bounds__(256) void k_fft_rows(const float2* __restrict__ in,
 template <int N, int S>
 __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __restrict__ in, float2* __restrict__ out, int n1) {
   constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
-  __shared__ float2 lds[M * N];
+  __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   init_twiddles<N, S>(tw, tid, T);
@@ -65,6 +114,11 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __rest
   wg_fft<N, T, M, true, S>(lds, tw, tid, load, store);
 }
 
+// copy the basis table U (P x N float2) into LDS
+__device__ __forceinline__ void load_basis(float2* Us, const float2* U, int count, int tid, int nthreads) {
+  for (int i = tid; i < count; i += nthreads) Us[i] = U[i];
+}
+
 // ----------------------------------------------------------------------------
 // PhaseScreen.generate
 // ----------------------------------------------------------------------------
@@ -72,7 +126,7 @@ template <int N>
 __global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre, const float* __restrict__ nim,
                                                   const float* __restrict__ amp, float2* __restrict__ work) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
-  __shared__ float2 lds[M * N];
+  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   init_twiddles<N, 1>(tw, tid, T);
@@ -96,19 +150,18 @@ __global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre,
 struct GenArgs {
   const float2* work;
   float* out;
-  const float* sh_noise;
-  const double* sh_amp;
   const float* screen_scale;
-  int count, n_fft, levels, off, n_out, accumulate, u_in_lds;
+  SubTables tab;
+  int count, n_fft, levels, off, n_out, accumulate;
   float scale;
 };
 
 template <int N>
 __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
   constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
-  __shared__ float2 lds[M * N];
+  __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
   __shared__ float2 tw[N];
-  extern __shared__ __align__(16) unsigned char dyn[];
+  extern __shared__ __align__(16) float2 Us[];  // [P][N] basis
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
   const int c0 = blockIdx.x * M;
@@ -116,35 +169,28 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
   const int P = a.levels;
   const int s0 = b, s1 = b + a.n_fft;
   const bool has1 = s1 < a.count;
-  ShLds sh = sh_carve(dyn, N, P, 2, M, a.u_in_lds != 0);
   init_twiddles<N, 1>(tw, tid, T);
-  if (P > 0) {
-    auto valid = [&](int t) { return t == 0 || has1; };
-    auto block = [&](int t, int p, double2* C) {
-      const int s = t == 0 ? s0 : s1;
-      const float* re = a.sh_noise + ((size_t)(p * 2 + 0) * a.count + s) * 9;
-      const float* im = a.sh_noise + ((size_t)(p * 2 + 1) * a.count + s) * 9;
-      for (int k = 0; k < 9; ++k) {
-        const double am = a.sh_amp[p * 9 + k];
-        C[k] = make_double2((double)re[k] * am, (double)im[k] * am);
-      }
-    };
-    sh_setup(sh, N, P, 2, M, c0, tid, T, block, valid);
-  }
+  load_basis(Us, a.tab.U, P * N, tid, T);
   __syncthreads();
   const size_t plane = (size_t)N * N;
   const float2* src = a.work + (size_t)b * plane + c0;
   const size_t oplane = (size_t)a.n_out * a.n_out;
   const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
   const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
+  ShRegs r0, r1;
+  r0.y = -1;
   auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
   auto store = [&](int m, int i, float2 v) {
     const int x = i - a.off, y = c0 + m - a.off;
     if ((unsigned)x >= (unsigned)a.n_out || (unsigned)y >= (unsigned)a.n_out) return;
     float l0 = 0.0f, l1 = 0.0f;
     if (P > 0) {
-      l0 = sh_eval(sh, 0, m, M, i, N, P);
-      if (has1) l1 = sh_eval(sh, 1, m, M, i, N, P);
+      if (c0 + m != r0.y) {
+        r0.load(a.tab, s0, c0 + m, N, P);
+        if (has1) r1.load(a.tab, s1, c0 + m, N, P);
+      }
+      l0 = r0.eval(Us, i, N, P);
+      if (has1) l1 = r1.eval(Us, i, N, P);
     }
     const size_t o = (size_t)x * a.n_out + y;
     float* d0 = a.out + (size_t)s0 * oplane + o;
@@ -167,7 +213,7 @@ __global__ __launch_bounds__(256) void k_frame_rows(const float2* __restrict__ s
                                                     const double* __restrict__ disp, double inv_period,
                                                     float2* __restrict__ work) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
-  __shared__ float2 lds[M * N];
+  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
   extern __shared__ __align__(16) unsigned char dyn[];
   float2* py = (float2*)dyn;          // [L][N]
@@ -213,46 +259,39 @@ __global__ __launch_bounds__(256) void k_frame_rows(const float2* __restrict__ s
 struct FrameArgs {
   const float2* work;
   float* out;
-  const double* disp;     // (F, L, 2)
-  const double2* coeffs;  // (P, L, 9)
-  double inv_period;
-  int L, levels, off, n_out, u_in_lds;
+  SubTables tab;
+  int levels, off, n_out;
   float scale;
 };
 
 template <int N>
 __global__ __launch_bounds__(ColsCfg<N>::T) void k_frame_cols(FrameArgs a) {
   constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
-  __shared__ float2 lds[M * N];
+  __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
   __shared__ float2 tw[N];
-  extern __shared__ __align__(16) unsigned char dyn[];
+  extern __shared__ __align__(16) float2 Us[];
   const int tid = threadIdx.x;
   const int f = blockIdx.y;
   const int c0 = blockIdx.x * M;
   if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;
-  const int P = a.levels, L = a.L;
-  ShLds sh = sh_carve(dyn, N, P, 1, M, a.u_in_lds != 0);
-  // shifted 3x3 blocks of this frame: after the epilogue tables in LDS
-  double2* Q = (double2*)(dyn + sh_lds_bytes(N, P, 1, M, a.u_in_lds != 0));  // [P][L] (+df_p phasor, x then y)
-  double2* SB = Q + 2 * (size_t)P * L;                                         // [P][9]
+  const int P = a.levels;
   init_twiddles<N, 1>(tw, tid, T);
-  if (P > 0) {
-    frame_sh_blocks(Q, SB, a.disp, a.coeffs, f, L, P, a.inv_period, tid, T);
-    auto valid = [&](int t) { return true; };
-    auto block = [&](int t, int p, double2* C) {
-      for (int k = 0; k < 9; ++k) C[k] = SB[p * 9 + k];
-    };
-    sh_setup(sh, N, P, 1, M, c0, tid, T, block, valid);
-  }
+  load_basis(Us, a.tab.U, P * N, tid, T);
   __syncthreads();
   const size_t plane = (size_t)N * N;
   const float2* src = a.work + (size_t)f * plane + c0;
   float* dst = a.out + (size_t)f * a.n_out * a.n_out;
+  ShRegs rg;
+  rg.y = -1;
   auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
   auto store = [&](int m, int i, float2 v) {
     const int x = i - a.off, y = c0 + m - a.off;
     if ((unsigned)x >= (unsigned)a.n_out || (unsigned)y >= (unsigned)a.n_out) return;
-    const float lo = P > 0 ? sh_eval(sh, 0, m, M, i, N, P) : 0.0f;
+    float lo = 0.0f;
+    if (P > 0) {
+      if (c0 + m != rg.y) rg.load(a.tab, f, c0 + m, N, P);
+      lo = rg.eval(Us, i, N, P);
+    }
     dst[(size_t)x * a.n_out + y] = (v.x + lo) * a.scale;
   };
   wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
@@ -276,6 +315,17 @@ static int set_dyn_lds(K kernel, size_t bytes) {
 }
 
 template <int N>
+static size_t cols_static_lds() {
+  return (size_t)Layout<N, ColsCfg<N>::T, ColsCfg<N>::M, true>::SIZE * 8 + (size_t)N * 8;
+}
+
+// Fused column epilogues hold U (levels x N) in LDS and H in registers.
+template <int N>
+static bool cols_fused_ok(int levels) {
+  return levels <= PREG && cols_static_lds<N>() + (size_t)levels * N * 8 <= kLdsMax;
+}
+
+template <int N>
 static int fft2_pow2(const float2* in, float2* out, int batch, int sign, hipStream_t st) {
   dim3 gr(N / RowsCfg<N>::M, batch), gc(N / ColsCfg<N>::M, batch);
   if (sign > 0) {
@@ -289,18 +339,17 @@ static int fft2_pow2(const float2* in, float2* out, int batch, int sign, hipStre
   return PT_OK;
 }
 
-static size_t static_lds_cols(int n, int M) { return (size_t)M * n * 8 + (size_t)n * 8; }
-
 template <int N>
-static int generate_pow2(const float* nre, const float* nim, const float* amp, GenArgs a, hipStream_t st) {
+static int generate_pow2(const float* nre, const float* nim, const float* amp, const float* sh_noise,
+                         const double* sh_amp, GenArgs a, hipStream_t st) {
   constexpr int MC = ColsCfg<N>::M;
-  const size_t stat = static_lds_cols(N, MC);
-  a.u_in_lds = a.levels > 0 && stat + sh_lds_bytes(N, a.levels, 2, MC, true) <= kLdsMax;
-  const size_t dyn = sh_lds_bytes(N, a.levels, 2, MC, a.u_in_lds != 0);
-  if (stat + dyn > kLdsMax) return fail(PT_ERR_UNSUPPORTED, "generate: %d subharmonic levels exceed LDS at n=%d", a.levels, N);
+  const size_t dyn = (size_t)a.levels * N * 8;
   int rc = set_dyn_lds(k_gen_cols<N>, dyn);
   if (rc) return rc;
   k_gen_rows<N><<<dim3(N / RowsCfg<N>::M, a.n_fft), RowsCfg<N>::T, 0, st>>>(nre, nim, amp, (float2*)a.work);
+  PT_LAUNCH_CHECK();
+  rc = launch_gen_prep(sh_noise, sh_amp, a.count, a.levels, N, a.tab, st);
+  if (rc) return rc;
   k_gen_cols<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, dyn, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
@@ -316,7 +365,7 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
                             float2* work, hipStream_t st) {
   constexpr int MR = RowsCfg<N>::M;
   const size_t dyn_r = frames_rows_lds<N>(L);
-  if (dyn_r + (size_t)MR * N * 8 + (size_t)N * 8 > kLdsMax)
+  if (dyn_r + (size_t)Layout<N, RowsCfg<N>::T, MR, false>::SIZE * 8 + (size_t)N * 8 > kLdsMax)
     return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor table at n=%d", L, N);
   int rc = set_dyn_lds(k_frame_rows<N>, dyn_r);
   if (rc) return rc;
@@ -328,23 +377,12 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
 template <int N>
 static int frames_cols_pow2(int F, FrameArgs a, hipStream_t st) {
   constexpr int MC = ColsCfg<N>::M;
-  const size_t stat = static_lds_cols(N, MC);
-  const size_t extra = a.levels > 0 ? (size_t)a.levels * a.L * 16 * 2 + (size_t)a.levels * 9 * 16 : 0;
-  a.u_in_lds = a.levels > 0 && stat + sh_lds_bytes(N, a.levels, 1, MC, true) + extra <= kLdsMax;
-  const size_t dyn_c = sh_lds_bytes(N, a.levels, 1, MC, a.u_in_lds != 0) + extra;
-  if (stat + dyn_c > kLdsMax) return fail(PT_ERR_UNSUPPORTED, "frames: subharmonic tables exceed LDS at n=%d", N);
-  int rc = set_dyn_lds(k_frame_cols<N>, dyn_c);
+  const size_t dyn = (size_t)a.levels * N * 8;
+  int rc = set_dyn_lds(k_frame_cols<N>, dyn);
   if (rc) return rc;
-  k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, dyn_c, st>>>(a);
+  k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, dyn, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
-}
-
-template <int N>
-static int frames_pow2(const float2* spectra, int L, const double* disp, int F, FrameArgs a, hipStream_t st) {
-  int rc = frames_rows_pow2<N>(spectra, L, disp, F, a.inv_period, (float2*)a.work, st);
-  if (rc) return rc;
-  return frames_cols_pow2<N>(F, a, st);
 }
 
 #define PT_POW2_SWITCH(n, CALL) \
@@ -359,6 +397,13 @@ static int frames_pow2(const float2* spectra, int L, const double* disp, int F, 
     default: break;             \
   }
 
+static bool fused_ok(int n, int levels) {
+#define CALL(N) return cols_fused_ok<N>(levels);
+  PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  return false;
+}
+
 }  // namespace pt
 
 using namespace pt;
@@ -372,12 +417,21 @@ int pt_frames_generic(const void* spectra, int L, int n, double pixel_scale, con
                       const double* coeffs, int levels, int off, int n_out, float scale, float* out, void* work,
                       void* stream);
 
+extern "C" long long pt_work_elems(int kind, int n, int sets, int levels) {
+  // kind 0: generate (sets = screens), 1: spectral frames (sets = frames)
+  if (n < 1 || sets < 0 || levels < 0) return -1;
+  const bool fused = is_pow2_fast(n) && fused_ok(n, levels);
+  const size_t nfft = kind == 0 ? (size_t)(sets + 1) / 2 : (size_t)sets;
+  return (long long)((fused ? 1 : 2) * nfft * n * n + sub_table_c64(levels, n, sets));
+}
+
 extern "C" int pt_fft2(const void* in, void* out, void* work, int batch, int n0, int n1, int sign, void* stream) {
   if (!in || !out) return fail(PT_ERR_ARG, "pt_fft2: NULL array");
   if (batch < 0 || n0 < 1 || n1 < 1 || (sign != 1 && sign != -1)) return fail(PT_ERR_ARG, "pt_fft2: bad shape/sign");
   if (batch == 0) return PT_OK;
   hipStream_t st = (hipStream_t)stream;
   if (n0 == n1 && is_pow2_fast(n0)) {
+    if (batch > 65535) return fail(PT_ERR_ARG, "pt_fft2: batch > 65535");
 #define CALL(N) return fft2_pow2<N>((const float2*)in, (float2*)out, batch, sign, st);
     PT_POW2_SWITCH(n0, CALL)
 #undef CALL
@@ -393,25 +447,25 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
   if (count < 1) return fail(PT_ERR_ARG, "pt_generate: count must be >= 1");
   if (!noise_re || !noise_im || !amplitude || !out || !work) return fail(PT_ERR_ARG, "pt_generate: NULL array");
   if (levels < 0 || (levels > 0 && (!sh_noise || !sh_amp))) return fail(PT_ERR_ARG, "pt_generate: bad subharmonics");
+  if (levels > PMAX) return fail(PT_ERR_UNSUPPORTED, "pt_generate: more than %d subharmonic levels", PMAX);
   if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "pt_generate: crop outside grid");
-  GenArgs a;
-  a.work = (const float2*)work;
-  a.out = out;
-  a.sh_noise = sh_noise;
-  a.sh_amp = sh_amp;
-  a.screen_scale = screen_scale;
-  a.count = count;
-  a.n_fft = (count + 1) / 2;
-  a.levels = levels;
-  a.off = crop_offset;
-  a.n_out = n_out;
-  a.accumulate = accumulate ? 1 : 0;
-  a.u_in_lds = 0;
-  a.scale = scale;
-  if (a.n_fft > 65535) return fail(PT_ERR_ARG, "pt_generate: count > 131070 per call");
+  const int n_fft = (count + 1) / 2;
+  if (n_fft > 65535) return fail(PT_ERR_ARG, "pt_generate: count > 131070 per call");
   hipStream_t st = (hipStream_t)stream;
-  if (is_pow2_fast(n)) {
-#define CALL(N) return generate_pow2<N>(noise_re, noise_im, amplitude, a, st);
+  if (is_pow2_fast(n) && fused_ok(n, levels)) {
+    GenArgs a;
+    a.work = (const float2*)work;
+    a.out = out;
+    a.screen_scale = screen_scale;
+    a.tab = sub_tables((float2*)work + (size_t)n_fft * n * n, levels, n, count);
+    a.count = count;
+    a.n_fft = n_fft;
+    a.levels = levels;
+    a.off = crop_offset;
+    a.n_out = n_out;
+    a.accumulate = accumulate ? 1 : 0;
+    a.scale = scale;
+#define CALL(N) return generate_pow2<N>(noise_re, noise_im, amplitude, sh_noise, sh_amp, a, st);
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   }
@@ -419,28 +473,27 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
                              n_out, scale, screen_scale, accumulate, out, work, stream);
 }
 
-// The two passes of pt_spectral_frames as separate calls (power-of-two n):
-// pass A writes the row-transformed layer sum to `work`, pass B finishes the
-// column transform and the epilogue.  Same arguments as pt_spectral_frames.
-static int frames_check(const void* spectra, int layers, int n, double pixel_scale, const double* disp, int frames,
-                        const double* sh_coeffs, int levels, int crop_offset, int n_out, float* out, void* work) {
-  if (!disp || !work) return fail(PT_ERR_ARG, "spectral pass: NULL array");
+// The two passes of pt_spectral_frames as separate calls (power-of-two n,
+// fused epilogue): pass A writes the row-transformed layer sum to `work`;
+// pass B builds the subharmonic tables and finishes the column transform.
+static int frames_check(int layers, int n, double pixel_scale, const double* disp, int frames, const double* sh_coeffs,
+                        int levels, int crop_offset, int n_out, void* work) {
+  if (!disp || !work) return fail(PT_ERR_ARG, "spectral frames: NULL array");
   if (layers < 1 || frames < 0 || levels < 0 || (levels > 0 && !sh_coeffs))
-    return fail(PT_ERR_ARG, "spectral pass: bad layers/frames/levels");
-  if (!(pixel_scale > 0.0)) return fail(PT_ERR_ARG, "spectral pass: pixel_scale must be > 0");
-  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "spectral pass: crop outside grid");
-  if (frames > 65535) return fail(PT_ERR_ARG, "spectral pass: frames > 65535 per call");
-  if (!is_pow2_fast(n)) return fail(PT_ERR_UNSUPPORTED, "spectral pass: split passes need a power-of-two n");
-  (void)spectra;
-  (void)out;
+    return fail(PT_ERR_ARG, "spectral frames: bad layers/frames/levels");
+  if (!(pixel_scale > 0.0)) return fail(PT_ERR_ARG, "spectral frames: pixel_scale must be > 0");
+  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "spectral frames: crop outside grid");
+  if (frames > 65535) return fail(PT_ERR_ARG, "spectral frames: frames > 65535 per call");
+  if (levels > PMAX) return fail(PT_ERR_UNSUPPORTED, "spectral frames: more than %d subharmonic levels", PMAX);
   return PT_OK;
 }
 
 extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
                                 int frames, void* work, void* stream) {
   if (!spectra) return fail(PT_ERR_ARG, "pt_spectral_rows: NULL spectra");
-  int rc = frames_check(spectra, layers, n, pixel_scale, disp, frames, nullptr, 0, 0, n, nullptr, work);
+  int rc = frames_check(layers, n, pixel_scale, disp, frames, nullptr, 0, 0, n, work);
   if (rc || frames == 0) return rc;
+  if (!is_pow2_fast(n)) return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: power-of-two n 32..2048 only");
   const double inv_period = 1.0 / ((double)n * pixel_scale);
   hipStream_t st = (hipStream_t)stream;
 #define CALL(N) return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
@@ -449,26 +502,26 @@ extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double p
   return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: n=%d", n);
 }
 
-extern "C" int pt_spectral_cols(const void* work, int layers, int n, double pixel_scale, const double* disp,
-                                int frames, const double* sh_coeffs, int levels, int crop_offset, int n_out,
-                                float scale, float* out, void* stream) {
+extern "C" int pt_spectral_cols(void* work, int layers, int n, double pixel_scale, const double* disp, int frames,
+                                const double* sh_coeffs, int levels, int crop_offset, int n_out, float scale,
+                                float* out, void* stream) {
   if (!out) return fail(PT_ERR_ARG, "pt_spectral_cols: NULL out");
-  int rc = frames_check(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, out,
-                        (void*)work);
+  int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
   if (rc || frames == 0) return rc;
+  if (!is_pow2_fast(n) || !fused_ok(n, levels))
+    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: power-of-two n and <= %d levels only", PREG);
+  hipStream_t st = (hipStream_t)stream;
   FrameArgs a;
   a.work = (const float2*)work;
   a.out = out;
-  a.disp = disp;
-  a.coeffs = (const double2*)sh_coeffs;
-  a.inv_period = 1.0 / ((double)n * pixel_scale);
-  a.L = layers;
+  a.tab = sub_tables((float2*)work + (size_t)frames * n * n, levels, n, frames);
   a.levels = levels;
   a.off = crop_offset;
   a.n_out = n_out;
-  a.u_in_lds = 0;
   a.scale = scale;
-  hipStream_t st = (hipStream_t)stream;
+  rc = launch_frame_prep(disp, (const double2*)sh_coeffs, frames, layers, levels, n, 1.0 / ((double)n * pixel_scale),
+                         a.tab, st);
+  if (rc) return rc;
 #define CALL(N) return frames_cols_pow2<N>(frames, a, st);
   PT_POW2_SWITCH(n, CALL)
 #undef CALL
@@ -478,30 +531,14 @@ extern "C" int pt_spectral_cols(const void* work, int layers, int n, double pixe
 extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
                                   int frames, const double* sh_coeffs, int levels, int crop_offset, int n_out,
                                   float scale, float* out, void* work, void* stream) {
-  if (!spectra || !disp || !out || !work) return fail(PT_ERR_ARG, "pt_spectral_frames: NULL array");
-  if (layers < 1 || frames < 0 || levels < 0 || (levels > 0 && !sh_coeffs))
-    return fail(PT_ERR_ARG, "pt_spectral_frames: bad layers/frames/levels");
-  if (!(pixel_scale > 0.0)) return fail(PT_ERR_ARG, "pt_spectral_frames: pixel_scale must be > 0");
-  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "pt_spectral_frames: crop outside grid");
-  if (frames == 0) return PT_OK;
-  if (frames > 65535) return fail(PT_ERR_ARG, "pt_spectral_frames: frames > 65535 per call");
-  FrameArgs a;
-  a.work = (const float2*)work;
-  a.out = out;
-  a.disp = disp;
-  a.coeffs = (const double2*)sh_coeffs;
-  a.inv_period = 1.0 / ((double)n * pixel_scale);
-  a.L = layers;
-  a.levels = levels;
-  a.off = crop_offset;
-  a.n_out = n_out;
-  a.u_in_lds = 0;
-  a.scale = scale;
-  hipStream_t st = (hipStream_t)stream;
-  if (is_pow2_fast(n)) {
-#define CALL(N) return frames_pow2<N>((const float2*)spectra, layers, disp, frames, a, st);
-    PT_POW2_SWITCH(n, CALL)
-#undef CALL
+  if (!spectra || !out) return fail(PT_ERR_ARG, "pt_spectral_frames: NULL array");
+  int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
+  if (rc || frames == 0) return rc;
+  if (is_pow2_fast(n) && fused_ok(n, levels)) {
+    rc = pt_spectral_rows(spectra, layers, n, pixel_scale, disp, frames, work, stream);
+    if (rc) return rc;
+    return pt_spectral_cols(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, scale,
+                            out, stream);
   }
   return pt_frames_generic(spectra, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
                            scale, out, work, stream);

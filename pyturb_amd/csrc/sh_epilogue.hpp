@@ -1,5 +1,16 @@
-// Subharmonic epilogue tables shared by the fused column passes and the
-// general-size epilogue kernels (see common.hpp for the algebra).
+// Subharmonic (low-frequency) field tables and the shared epilogue pieces.
+//
+// A prep kernel computes, once per call, for every coefficient set s (a
+// Monte-Carlo screen or a frame) and every column y:
+//   H[s][p][y] = g_2 + conj(g_0)   and   K[s][y] = sum_p Re(g_1) - mean_s
+// (see common.hpp for g and the closed-form grid mean), plus the basis table
+// U[p][x] = e_p(x).  The column-pass epilogue then adds
+//   low(x, y) = K[s][y] + sum_p Re(U[p][x] * H[s][p][y])
+// with H/K in registers (one column per thread) and U in LDS: no per-tile
+// setup, no FP64 in the transform kernels.
+//
+// Tables live in the caller's workspace right after the FFT intermediate,
+// in c64 (float2) units:  U: P*n,  H: S*P*n,  K: S*n floats.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,97 +19,52 @@
 
 namespace pt {
 
-// Dynamic LDS of a column pass: U[P][N] basis values (when it fits), then
-// H[NT][M][P] float2, KP[NT][M][P] float, K[NT][M] float, MEAN[NT][P] double.
-struct ShLds {
+constexpr int PMAX = 32;  // subharmonic levels supported
+constexpr int PREG = 12;  // levels the fused column epilogues keep in registers
+
+struct SubTables {
   float2* U;
   float2* H;
-  float* KP;
   float* K;
-  double* MEAN;
-  bool u_in_lds;
 };
 
-__host__ __device__ inline size_t sh_lds_bytes(int n, int P, int NT, int M, bool u_in_lds) {
+__host__ __device__ inline size_t sub_table_c64(int P, int n, int S) {
   if (P == 0) return 0;
-  size_t b = u_in_lds ? (size_t)n * P * 8 : 0;
-  b += (size_t)NT * M * P * 8 + (size_t)NT * M * P * 4 + (size_t)NT * M * 4;
-  b = (b + 7) & ~(size_t)7;
-  b += (size_t)NT * P * 8;
-  return (b + 15) & ~(size_t)15;
+  return (size_t)P * n + (size_t)S * P * n + ((size_t)S * n + 1) / 2;
 }
 
-__device__ inline ShLds sh_carve(unsigned char* dyn, int n, int P, int NT, int M, bool u_in_lds) {
-  ShLds s;
-  size_t o = 0;
-  s.u_in_lds = u_in_lds;
-  s.U = (float2*)(dyn + o);
-  if (u_in_lds) o += (size_t)n * P * 8;
-  s.H = (float2*)(dyn + o);
-  o += (size_t)NT * M * P * 8;
-  s.KP = (float*)(dyn + o);
-  o += (size_t)NT * M * P * 4;
-  s.K = (float*)(dyn + o);
-  o += (size_t)NT * M * 4;
-  o = (o + 7) & ~(size_t)7;
-  s.MEAN = (double*)(dyn + o);
-  return s;
+__host__ __device__ inline SubTables sub_tables(float2* base, int P, int n, int S) {
+  SubTables t;
+  t.U = base;
+  t.H = base + (size_t)P * n;
+  t.K = (float*)(t.H + (size_t)S * P * n);
+  return t;
 }
 
-// Fill U (if in LDS), H, KP and MEAN for NT coefficient sets over columns
-// c0..c0+M-1.  `block(t, p, C)` writes the 3x3 block of set t, level p.
-// `valid(t)` says whether set t is used.  Ends with K filled; callers
-// __syncthreads() before reading.
-template <class Block, class Valid>
-__device__ inline void sh_setup(ShLds& s, int n, int P, int NT, int M, int c0, int tid, int nthreads,
-                                Block& block, Valid& valid) {
-  if (s.u_in_lds) {
-    for (int idx = tid; idx < P * n; idx += nthreads) s.U[idx] = sh_basis_e(idx / n, idx % n, n);
+// Fill the tables of one coefficient set (C: P blocks of 9, in LDS) for all
+// columns; `writeU` also writes the shared basis table.  256-thread blocks.
+__device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTables t, bool writeU,
+                                    double* s_mean, int tid, int nthreads) {
+  if (tid == 0) {
+    double m = 0.0;
+    for (int p = 0; p < P; ++p) m += sh_block_mean(C + p * 9, sh_mean_e(p, n));
+    *s_mean = m;
   }
-  const int ntask = NT * M * P;
-  for (int task = tid; task < ntask + NT * P; task += nthreads) {
-    if (task < ntask) {
-      const int p = task % P;
-      const int m = (task / P) % M;
-      const int t = task / (P * M);
-      if (!valid(t)) continue;
-      double2 C[9];
-      block(t, p, C);
-      int y = c0 + m;
-      if (y >= n) y = n - 1;  // tile overhang (never stored)
-      double2 H;
-      double g1;
-      sh_column(C, sh_basis_e(p, y, n), H, g1);
-      s.H[(t * M + m) * P + p] = make_float2((float)H.x, (float)H.y);
-      s.KP[(t * M + m) * P + p] = (float)g1;
-    } else {
-      const int q = task - ntask;
-      const int p = q % P, t = q / P;
-      if (!valid(t)) continue;
-      double2 C[9];
-      block(t, p, C);
-      s.MEAN[t * P + p] = sh_block_mean(C, sh_mean_e(p, n));
-    }
-  }
+  if (writeU)
+    for (int idx = tid; idx < P * n; idx += nthreads) t.U[idx] = sh_basis_e(idx / n, idx % n, n);
   __syncthreads();
-  for (int q = tid; q < NT * M; q += nthreads) {
-    const int t = q / M;
-    if (!valid(t)) continue;
-    double acc = 0.0;
-    for (int p = 0; p < P; ++p) acc += (double)s.KP[q * P + p] - s.MEAN[t * P + p];
-    s.K[q] = (float)acc;
+  const double mean = *s_mean;
+  for (int y = tid; y < n; y += nthreads) {
+    double kacc = -mean;
+    for (int p = 0; p < P; ++p) {
+      double2 Hd;
+      double g1;
+      sh_column(C + p * 9, sh_basis_e(p, y, n), Hd, g1);
+      t.H[((size_t)s * P + p) * n + y] = make_float2((float)Hd.x, (float)Hd.y);
+      kacc += g1;
+    }
+    t.K[(size_t)s * n + y] = (float)kacc;
   }
-}
-
-__device__ __forceinline__ float sh_eval(const ShLds& s, int t, int m, int M, int x, int n, int P) {
-  float acc = s.K[t * M + m];
-  const float2* H = s.H + (size_t)(t * M + m) * P;
-  for (int p = 0; p < P; ++p) {
-    float2 e = s.u_in_lds ? s.U[p * n + x] : sh_basis_e(p, x, n);
-    float2 h = H[p];
-    acc += e.x * h.x - e.y * h.y;
-  }
-  return acc;
 }
 
 // Shifted 3x3 subharmonic blocks of one frame: for level p,
@@ -130,5 +96,48 @@ __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* di
   }
   __syncthreads();
 }
+
+// One column's K and H in registers.
+struct ShRegs {
+  int y;
+  float k;
+  float2 h[PREG];
+  __device__ __forceinline__ void load(const SubTables& t, int s, int yy, int n, int P) {
+    y = yy;
+    k = t.K[(size_t)s * n + yy];
+#pragma unroll
+    for (int p = 0; p < PREG; ++p)
+      if (p < P) h[p] = t.H[((size_t)s * P + p) * n + yy];
+  }
+  // U: basis table with row stride n (LDS or global)
+  __device__ __forceinline__ float eval(const float2* U, int x, int n, int P) const {
+    float acc = k;
+#pragma unroll
+    for (int p = 0; p < PREG; ++p) {
+      if (p < P) {
+        const float2 e = U[p * n + x];
+        acc += e.x * h[p].x - e.y * h[p].y;
+      }
+    }
+    return acc;
+  }
+};
+
+// Unbounded-level evaluation straight from the tables (general-size path).
+__device__ __forceinline__ float sub_eval(const SubTables& t, int s, int x, int y, int n, int P) {
+  float acc = t.K[(size_t)s * n + y];
+  for (int p = 0; p < P; ++p) {
+    const float2 e = t.U[p * n + x];
+    const float2 h = t.H[((size_t)s * P + p) * n + y];
+    acc += e.x * h.x - e.y * h.y;
+  }
+  return acc;
+}
+
+// Prep launchers (kernels in screens.hip).  Tables must not alias inputs.
+int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
+                    hipStream_t st);
+int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
+                      SubTables t, hipStream_t st);
 
 }  // namespace pt

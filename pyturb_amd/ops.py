@@ -24,8 +24,12 @@ def fast_size(n: int) -> bool:
     return 32 <= n <= 2048 and (n & (n - 1)) == 0
 
 
-def _work_factor(n: int) -> int:
-    return 1 if fast_size(n) else 2
+def work_elems(kind: int, n: int, sets: int, levels: int) -> int:
+    """Workspace in complex64 elements (pt_work_elems): 0 generate, 1 frames."""
+    v = int(_hip.lib().pt_work_elems(int(kind), int(n), int(sets), int(levels)))
+    if v < 0:
+        raise ValueError("pt_work_elems: bad arguments")
+    return v
 
 
 def _require(t, dtype, shape, name):
@@ -83,7 +87,7 @@ def generate(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offset: int
     _require(out, t.float32, (count, n_out, n_out), "out")
     if screen_scale is not None:
         _require(screen_scale, t.float32, (count,), "screen_scale")
-    need = _work_factor(n) * n_fft * n * n
+    need = work_elems(0, n, count, P)
     if work is None:
         work = t.empty((need,), dtype=t.complex64, device=noise.device)
     elif work.numel() < need or work.dtype != t.complex64:
@@ -119,7 +123,7 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
     if out is None:
         out = t.empty((F, n_out, n_out), dtype=t.float32, device=spectra.device)
     _require(out, t.float32, (F, n_out, n_out), "out")
-    need = _work_factor(n) * F * n * n
+    need = work_elems(1, n, F, P)
     if work is None or work.numel() < need:
         work = t.empty((need,), dtype=t.complex64, device=spectra.device)
     rc = _hip.lib().pt_spectral_frames(_hip.ptr(spectra), L, n, float(pixel_scale),

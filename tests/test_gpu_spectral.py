@@ -209,3 +209,48 @@ def test_phase_screen_api_matches_reference_golden(hip):
     assert O.rel_rms(got[:2], g["screens01"]) < REL
     np.testing.assert_allclose(got.sum(axis=(1, 2)), g["sums"],
                                atol=REL * np.abs(got).sum() / 64)
+
+
+# ------------------------------------------------------------------ general-size path
+@pytest.mark.parametrize("n,count,P", [(96, 3, 2), (100, 4, 8), (64, 4, 14)])
+def test_generate_general_sizes_and_deep_levels(hip, n, count, P):
+    """Non-power-of-two grids (direct-DFT path) and > 12 levels."""
+    print(f"generate general n={n} P={P}", flush=True)
+    filt, main, sh = _gen_case(n, 8.0 / n, 0.12, 30.0, P, count, 11)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    ref = O.generate(filt, f32(main), f32(sh), count)
+    got = _np(_run_generate(filt, main, sh, count))
+    assert O.rel_rms(got, ref) < REL
+
+
+def test_fov_atmosphere_frames_and_directions_match_reference_golden(hip):
+    """30" field of view: 112^2 oversized screens, crop, three directions."""
+    import pyturb_amd as pa
+
+    g = load_golden("atmosphere_n64_seed2.npz")
+    layers = [pa.Layer(*ly) for ly in config2_layer_tuples()]
+    atm = pa.Atmosphere(layers, r0=0.15, diameter=8.0, n=64, L0=25.0, field_of_view=30.0,
+                        dtype="float64", seed=2)
+    assert atm.n_screen == int(g["n_screen"])
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for t, want in zip(g["times"], g["frames"]):
+            assert O.rel_rms(_np(atm.opd(float(t))), want) < REL
+    got = _np(atm.opd(0.01, directions=[tuple(d) for d in g["dirs"]]))
+    assert got.shape == (3, 64, 64)
+    for d in range(3):
+        assert O.rel_rms(got[d], g["dframes"][d]) < REL
+    batch = _np(atm.opd_batch([0.01, 0.01], directions=[tuple(d) for d in g["dirs"]]))
+    np.testing.assert_array_equal(batch[1], got)
+
+
+def test_fft2_general_size(hip):
+    import torch
+    from pyturb_amd import ops
+
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 72, 72)) + 1j * rng.standard_normal((2, 72, 72))
+    got = _np(ops.fft2(torch.from_numpy(x.astype(np.complex64)).cuda(), 1))
+    ref = np.fft.ifft2(x) * 72 * 72
+    assert np.sqrt(np.mean(np.abs(got - ref) ** 2) / np.mean(np.abs(ref) ** 2)) < 2e-6

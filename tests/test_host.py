@@ -48,7 +48,7 @@ def test_host_flow_draw_matches_oracle_seed_tree():
 
 def _header_symbols():
     text = open(os.path.join(ROOT, "include", "pyturb_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pt_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long long|const char\*)\s+(pt_\w+)\s*\(", text, re.M)))
 
 
 def test_library_loads_and_exports_every_header_symbol():
@@ -61,6 +61,11 @@ def test_library_loads_and_exports_every_header_symbol():
         assert hasattr(so, name), name
         assert name in _hip.SIGNATURES, name
     assert _hip.lib().pt_abi_version() == 1
+    assert "pt_work_elems" in syms
+    # host-only sizing: power-of-two fused path vs general path
+    assert _hip.lib().pt_work_elems(1, 512, 10, 0) == 10 * 512 * 512
+    assert _hip.lib().pt_work_elems(1, 100, 10, 0) == 2 * 10 * 100 * 100
+    assert _hip.lib().pt_work_elems(0, 64, 5, 2) == 3 * 64 * 64 + 2 * 64 + 5 * 2 * 64 + (5 * 64 + 1) // 2
 
 
 def test_c_abi_rejects_bad_arguments_without_launching():
