@@ -211,6 +211,60 @@ def _plain_gen_ctx(grp, n):
                                             device=dev)[:, None])
 
 
+EXT_N = 1024
+EXT_STEPS = 1000
+LGS_ALT = 90e3
+
+
+def build_extrude(dev):
+    """9-layer 1024^2 extruder, 30" field, innovations from seed 3 in HBM."""
+    import torch
+
+    from pyturb_amd.extrude import ExtrudedAtmosphere
+
+    layers = profile_layers()
+    dx = 8.0 / EXT_N
+    frac = np.array(FRACS) / np.sum(FRACS)
+    r0s = 0.15 * frac ** -0.6
+    winds = [ly.wind_vector for ly in layers]
+    arcsec = np.pi / (180 * 3600)
+    fov = [h * np.tan(30 * arcsec) / dx for h in HEIGHTS]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+    # rows each layer extrudes over the run: travel + lookahead, with margin
+    speeds = np.array([ly.wind_speed for ly in layers])
+    rows = (speeds * EXT_STEPS * 1e-3 / dx + 8).astype(int)
+    pool = {}
+    used = [0] * len(layers)
+    ext = None
+
+    def innov(l, k):
+        if l not in pool:
+            pool[l] = torch.randn((int(rows[l]) + 64, ext.width), generator=gen, device=dev)
+        out = pool[l][used[l]:used[l] + k]
+        used[l] += k
+        return out
+
+    ext = ExtrudedAtmosphere(n=EXT_N, pixel_scale=dx, layer_r0=list(r0s), layer_L0=[25.0] * 9,
+                             layer_wind=winds, layer_altitude_los=HEIGHTS,
+                             field_of_view_pix=fov, interp="linear", seeds=list(range(30, 39)),
+                             innovations=innov)
+    ang = np.radians(np.arange(6) * 60.0)
+    thx = np.concatenate([np.tan(30 * arcsec) * np.cos(ang), [0.0]])
+    thy = np.concatenate([np.tan(30 * arcsec) * np.sin(ang), [0.0]])
+    mags = np.ones((7, 9))
+    mags[6] = 1.0 - np.array(HEIGHTS) / LGS_ALT
+    out = torch.empty((7, EXT_N, EXT_N), dtype=torch.float32, device=dev)
+    return ext, thx, thy, mags, out
+
+
+def run_extrude(ext, thx, thy, mags, out, steps, t0=0):
+    scale = 500e-9 / (2 * np.pi)
+    for k in range(steps):
+        ext.set_time((t0 + k) * 1e-3)
+        ext.integrate_dirs(thx, thy, scale=scale, mags=mags, out=out)
+
+
 def timed(fn, steps, warmup):
     import torch
 
@@ -269,6 +323,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte-Carlo ensemble")
+    ap.add_argument("--no-extrude", action="store_true", help="skip the extrusion workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (counter scaling)")
     args = ap.parse_args(argv)
@@ -346,6 +401,22 @@ def main(argv=None):
             extra["mc_plain_screens_per_s"] = round(nscr / tpm, 1)
             extra["fused_vs_plain_mc"] = round(mc_rate / (nscr / tpm), 1)
         del classes
+        torch.cuda.empty_cache()
+    if not args.no_extrude:
+        print("bench: building 9-layer 1024^2 extruder (host FP64 A/B)", file=sys.stderr, flush=True)
+        tb = time.perf_counter()
+        ext, thx, thy, mags, eout = build_extrude(dev)
+        extra["extrude_setup_s"] = round(time.perf_counter() - tb, 1)
+        run_extrude(ext, thx, thy, mags, eout, 20)
+        torch.cuda.synchronize()
+        te0 = time.perf_counter()
+        run_extrude(ext, thx, thy, mags, eout, EXT_STEPS - 20, t0=20)
+        torch.cuda.synchronize()
+        te = time.perf_counter() - te0
+        extra["extrude_steps_per_s"] = round((EXT_STEPS - 20) / te, 1)
+        extra["extrude_frames_per_s_7dirs"] = round(7 * (EXT_STEPS - 20) / te, 1)
+        extra["extrude_width"] = ext.width
+        del ext, eout
         torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:

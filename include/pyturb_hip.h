@@ -124,30 +124,33 @@ int pt_spectral_cols(void* work, int layers, int n, double pixel_scale,
                      float* out, void* stream);
 
 /*
+ * Ring-buffer and matrix rows below have a pitch `ld` >= width, a multiple
+ * of 4 floats, with zero padding past `width`; arrays are 16-byte aligned.
+ *
  * Innovation terms of the Assemat-Wilson recurrence for a batch of new rows:
  *   out[k][w] = sum_j B_unit[w][j] * beta[k][j]      (out = beta * B^T)
- *   beta (rows, width) f32 white noise, B_unit (width, width) f32, out
- *   (rows, width) f32.  The per-layer r0^(-5/6) factor is applied by
- *   pt_extrude_rows.  Replaces the B @ beta half of
- *   src/pyturb/extrude.py:427-429 and src/pyturb/infinite.py:278-280,
- *   batched over every row of a step (as extrude.py:686-689 does for
- *   boiling).
+ *   beta (rows, ld) f32 white noise, B_unit (width, ld) f32, out (rows, ld).
+ *   The per-layer r0^(-5/6) factor is applied by pt_extrude_rows.
+ * Replaces the B @ beta half of src/pyturb/extrude.py:427-429 and
+ * src/pyturb/infinite.py:278-280, batched over every row of a step (as
+ * extrude.py:686-689 does for boiling).
  */
 int pt_innovation_gemm(const float* beta, const float* B_unit, int rows,
-                       int width, float* out, void* stream);
+                       int width, int ld, float* out, void* stream);
 
 /*
  * One round of the row recurrence for `nact` ring buffers:
  *   buf[layer[e]][trow[e]][:] = A @ concat(buf[layer[e]][zrows[e][0..m-1]])
  *                              + bscale[e] * bb[bbrow[e]][:]
- *   A (width, m*width) f32 (shared by the layers of one outer scale);
- *   buf (layers, cap, width) f32 ring buffers (physical row indices);
- *   bb from pt_innovation_gemm.  HOST arrays: layer, trow, bbrow (nact)
- *   int, zrows (nact, m) int, bscale (nact) f32.  m <= 4.
+ *   A (width, m*ld) f32, stencil row s in columns [s*ld, s*ld + width)
+ *   (shared by the layers of one outer scale); buf (layers, cap, ld) f32
+ *   ring buffers (physical row indices); bb from pt_innovation_gemm.
+ *   HOST arrays: layer, trow, bbrow (nact) int, zrows (nact, m) int,
+ *   bscale (nact) f32.  m <= 4.
  * Replaces src/pyturb/extrude.py:424-430 (and infinite.py:274-281) for all
  * layers that extrude a row in this round.
  */
-int pt_extrude_rows(const float* A, int width, int stencil_rows,
+int pt_extrude_rows(const float* A, int width, int ld, int stencil_rows,
                     const float* bb, float* buf, long long cap, int nact,
                     const int* layer, const int* zrows, const int* trow,
                     const int* bbrow, const float* bscale, void* stream);
@@ -165,9 +168,9 @@ int pt_extrude_rows(const float* A, int width, int stencil_rows,
  *   HOST arrays: start (layers) int, fill (layers) int64, wind_cs
  *   (layers, 2) f64 = (cos, sin), dir_geom (dirs, layers, 3) f64 =
  *   (sa, sp, mag).  interp: 0 linear, 1 Catmull-Rom cubic, 2 Lanczos-3.
- *   out (dirs, n, n) f32 device.
+ *   buf (layers, cap, ld), out (dirs, n, n) f32 device.
  */
-int pt_readout(const float* buf, int layers, long long cap, int width,
+int pt_readout(const float* buf, int layers, long long cap, int width, int ld,
                const int* start, const long long* fill, const double* wind_cs,
                const double* dir_geom, int dirs, int n, int interp, float scale,
                int accumulate, float* out, void* stream);

@@ -5,11 +5,13 @@
 // the reference's bookkeeping) lives at physical row (start + i) mod cap, so
 // "compaction" is an index shift -- no row is ever copied.
 //
-//  * k_gemm_nt    : innovation terms for every new row of a step at once,
-//                   BB = beta * B_unit^T  (K x W), one GEMM reading B once.
-//  * k_extrude_rows: one sequential round of the recurrence for up to 16
-//                   layers: row = A * z + s_l * BB[k]; one wave per output
-//                   element, A streamed once per round for all layers.
+//  * innovation terms for every new row of a step at once,
+//    BB = beta * B_unit^T (K x W), B streamed once per step;
+//  * one sequential round of the recurrence for up to 12 layers,
+//    row = A * z + s_l * BB[k], A streamed once per round for all layers.
+//    Both are k_multi_gemv (a W x m*ld matrix against <= 12 vectors).
+// Rows have a pitch `ld` (multiple of 4 floats, zero padding) so every
+// matrix and ring row is 16-byte aligned.
 //  * k_readout    : rotated, sub-pixel, per-layer-shifted interpolation of
 //                   all layers for up to 8 directions (each with its own
 //                   per-layer magnification, so NGS and LGS cones share one
@@ -22,69 +24,25 @@
 namespace pt {
 
 // ---------------------------------------------------------------------------
-// C (K x N) = X (K x J) * Y (N x J)^T, row-major, FP32
+// multi-vector GEMV: y_e[w] = sum_s sum_c M[w][s*ld + c] * v_{e,s}[c]
+//                             (+ scale_e * add_e[w])
+// Both the innovation terms (M = B_unit, one segment, v = beta rows) and a
+// recurrence round (M = A, m segments, v = the m stencil rows of each layer)
+// are this shape: a W x (m*ld) matrix streamed once against up to 12 short
+// vectors.  float4 loads (ld % 4 == 0, 16-byte aligned rows, zero padding
+// beyond the row width).
 // ---------------------------------------------------------------------------
-constexpr int GT = 64, GK = 16;
+constexpr int VMAX = 12;  // vectors per launch
+constexpr int MMAX = 4;   // stencil rows (segments)
 
-__global__ __launch_bounds__(256) void k_gemm_nt(const float* __restrict__ X, const float* __restrict__ Y,
-                                                 float* __restrict__ C, int K, int N, int J) {
-  __shared__ float xs[GK][GT + 1];
-  __shared__ float ys[GK][GT + 1];
-  const int tid = threadIdx.x;
-  const int k0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
-  const int tn = tid % 16, tk = tid / 16;
-  float acc[4][4] = {};
-  for (int j0 = 0; j0 < J; j0 += GK) {
-    for (int e = tid; e < GT * GK; e += 256) {
-      const int r = e / GK, jj = e % GK, j = j0 + jj;
-      xs[jj][r] = (k0 + r < K && j < J) ? X[(size_t)(k0 + r) * J + j] : 0.f;
-      ys[jj][r] = (n0 + r < N && j < J) ? Y[(size_t)(n0 + r) * J + j] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int jj = 0; jj < GK; ++jj) {
-      float xv[4], yv[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        xv[q] = xs[jj][tk + 16 * q];
-        yv[q] = ys[jj][tn + 16 * q];
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += xv[a] * yv[b];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int k = k0 + tk + 16 * a;
-    if (k >= K) continue;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int nn = n0 + tn + 16 * b;
-      if (nn < N) C[(size_t)k * N + nn] = acc[a][b];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// one round of the row recurrence
-// ---------------------------------------------------------------------------
-constexpr int XMAX = 16;  // layers per round
-constexpr int MMAX = 4;   // stencil rows
-
-struct RowsArgs {
-  const float* A;   // (W, m*W)
-  const float* bb;  // (K, W) innovation terms (unscaled)
-  float* buf;       // (layers, cap, W)
-  long long cap;
-  int W, m, nact;
-  int layer[XMAX];
-  int zrow[XMAX][MMAX];
-  int trow[XMAX];
-  int bbrow[XMAX];
-  float bscale[XMAX];
+struct GemvArgs {
+  const float* M;
+  long long ldm;  // matrix row pitch (m * ld)
+  int W, ld, m, nv;
+  const float* v[VMAX][MMAX];
+  float* out[VMAX];
+  const float* add[VMAX];
+  float scale[VMAX];
 };
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -93,68 +51,100 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_extrude_rows(RowsArgs a) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + wave;
-  if (row >= a.W) return;
-  const int W = a.W;
-  const float* Ar = a.A + (size_t)row * a.m * W;
-  float acc[XMAX];
+// A workgroup owns GR output rows: its 256 threads split the rows'
+// m*ld/4 float4 columns, so each vector float4 (from L2) is reused for GR
+// matrix rows and every thread has GR+1 independent loads per column in
+// flight; partial sums are reduced through LDS.
+constexpr int GR = 4;
+
+__global__ __launch_bounds__(256) void k_multi_gemv(GemvArgs a) {
+  __shared__ float part[4][GR][VMAX];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int row0 = blockIdx.x * GR;
+  const int nq = a.ld >> 2;  // float4 per segment
+  float acc[GR][VMAX];
 #pragma unroll
-  for (int e = 0; e < XMAX; ++e) acc[e] = 0.f;
+  for (int r = 0; r < GR; ++r)
+#pragma unroll
+    for (int e = 0; e < VMAX; ++e) acc[r][e] = 0.f;
 #pragma unroll
   for (int s = 0; s < MMAX; ++s) {
     if (s >= a.m) break;
-    const float* As = Ar + (size_t)s * W;
-    const float* zp[XMAX];
+    const float4* Ms[GR];
 #pragma unroll
-    for (int e = 0; e < XMAX; ++e)
-      zp[e] = a.buf + ((size_t)a.layer[e < a.nact ? e : 0] * a.cap + a.zrow[e < a.nact ? e : 0][s]) * W;
-    for (int c = lane; c < W; c += 64) {
-      const float av = As[c];
+    for (int r = 0; r < GR; ++r) {
+      const int row = min(row0 + r, a.W - 1);  // tail rows recompute the last row (not stored)
+      Ms[r] = (const float4*)(a.M + (size_t)row * a.ldm + (size_t)s * a.ld);
+    }
+    for (int q = tid; q < nq; q += 256) {
+      float4 x[GR];
 #pragma unroll
-      for (int e = 0; e < XMAX; ++e)
-        if (e < a.nact) acc[e] = fmaf(av, zp[e][c], acc[e]);
+      for (int r = 0; r < GR; ++r) x[r] = Ms[r][q];
+#pragma unroll
+      for (int e = 0; e < VMAX; ++e) {
+        if (e < a.nv) {
+          const float4 z = ((const float4*)a.v[e][s])[q];
+#pragma unroll
+          for (int r = 0; r < GR; ++r)
+            acc[r][e] = fmaf(x[r].x, z.x, fmaf(x[r].y, z.y, fmaf(x[r].z, z.z, fmaf(x[r].w, z.w, acc[r][e]))));
+        }
+      }
     }
   }
-  float mine = 0.f;
 #pragma unroll
-  for (int e = 0; e < XMAX; ++e) {
-    if (e < a.nact) {
-      const float t = wave_sum(acc[e]);
-      if (lane == e) mine = t;
+  for (int r = 0; r < GR; ++r)
+#pragma unroll
+    for (int e = 0; e < VMAX; ++e) {
+      if (e < a.nv) {
+        const float t = wave_sum(acc[r][e]);
+        if (lane == 0) part[wave][r][e] = t;
+      }
     }
-  }
-  if (lane < a.nact) {
-    int l = 0, tr = 0, br = 0;
+  __syncthreads();
+  const int r = tid / VMAX, e = tid % VMAX;
+  if (r < GR && e < a.nv && row0 + r < a.W) {
+    const float sum = (part[0][r][e] + part[1][r][e]) + (part[2][r][e] + part[3][r][e]);
+    float* out = nullptr;
+    const float* add = nullptr;
     float sc = 0.f;
 #pragma unroll
-    for (int e = 0; e < XMAX; ++e)
-      if (lane == e) {
-        l = a.layer[e];
-        tr = a.trow[e];
-        br = a.bbrow[e];
-        sc = a.bscale[e];
+    for (int k = 0; k < VMAX; ++k)
+      if (e == k) {
+        out = a.out[k];
+        add = a.add[k];
+        sc = a.scale[k];
       }
-    a.buf[((size_t)l * a.cap + tr) * W + row] = mine + sc * a.bb[(size_t)br * W + row];
+    out[row0 + r] = sum + (add ? sc * add[row0 + r] : 0.f);
   }
 }
+
+static int launch_gemv(GemvArgs& a, hipStream_t st) {
+  k_multi_gemv<<<(a.W + GR - 1) / GR, 256, 0, st>>>(a);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 // ---------------------------------------------------------------------------
 // fused readout
 // ---------------------------------------------------------------------------
+// Instruction-lean gather: the host splits each (direction, layer) shift
+// into an integer part and a fraction (float64 -> int + float), so the
+// per-pixel coordinate math is float32 on magnitudes <= ~n (abs. error
+// ~1e-4 px) and all indexing is 32-bit.
 constexpr int RL = 16, RD = 8;
 
 struct ReadoutArgs {
-  const float* buf;  // (L, cap, W)
+  const float* buf;  // (L, cap, ld)
   float* out;        // (D, n, n)
-  long long cap;
-  int W, n, L, D, accumulate;
+  int cap, W, ld, n, L, D, accumulate;
   float scale;
   int start[RL];
   int fillm1[RL];
-  double cosw[RL], sinw[RL];
-  double sa[RD][RL], sp[RD][RL], mag[RD][RL];
+  float cosw[RL], sinw[RL];
+  int r_int[RD][RL], c_int[RD][RL];
+  float r_frac[RD][RL], c_frac[RD][RL], mag[RD][RL];
 };
 
 template <int TAPS>
@@ -195,39 +185,37 @@ __global__ __launch_bounds__(256) void k_readout(ReadoutArgs a) {
   const int j = blockIdx.x * 32 + (threadIdx.x & 31);
   const int i = blockIdx.y * 8 + (threadIdx.x >> 5);
   if (i >= a.n || j >= a.n) return;
-  const double ci = (double)i - 0.5 * (a.n - 1), cj = (double)j - 0.5 * (a.n - 1);
-  const double halfw = 0.5 * a.W;
-  for (int d = 0; d < a.D; ++d) {
+  const float h = 0.5f * (float)(a.n - 1);
+  const float ci = (float)i - h, cj = (float)j - h;  // exact half-integers
+  const int d = blockIdx.z;  // one direction per block: short dependency chains
+  {
     float acc = 0.f;
+#pragma unroll 3
     for (int l = 0; l < a.L; ++l) {
-      const double mg = a.mag[d][l];
-      const double gi = ci * mg, gj = cj * mg;
-      const double c = a.cosw[l], s = a.sinw[l];
-      const double row = (gi * c + gj * s) + a.sa[d][l];
-      const double col = (-gi * s + gj * c + halfw) + a.sp[d][l];
-      const double rf = floor(row), cf = floor(col);
-      const long long r0 = (long long)rf, c0 = (long long)cf;
+      const float mg = a.mag[d][l];
+      const float gi = ci * mg, gj = cj * mg;
+      const float c = a.cosw[l], s = a.sinw[l];
+      const float row = fmaf(gi, c, fmaf(gj, s, a.r_frac[d][l]));
+      const float col = fmaf(-gi, s, fmaf(gj, c, a.c_frac[d][l]));
+      const float rf = floorf(row), cf = floorf(col);
+      const int r0 = (int)rf + a.r_int[d][l], c0 = (int)cf + a.c_int[d][l];
       float wr[TAPS], wc[TAPS];
-      taps_weights<TAPS>((float)(row - rf), wr);
-      taps_weights<TAPS>((float)(col - cf), wc);
-      const float* bl = a.buf + (size_t)l * a.cap * a.W;
-      const long long fm = a.fillm1[l];
-      const long long st = a.start[l];
+      taps_weights<TAPS>(row - rf, wr);
+      taps_weights<TAPS>(col - cf, wc);
+      const float* bl = a.buf + (size_t)l * a.cap * a.ld;
+      const int fm = a.fillm1[l], st = a.start[l];
+      int cc[TAPS];
+#pragma unroll
+      for (int v = 0; v < TAPS; ++v) cc[v] = min(max(c0 + OFF + v, 0), a.W - 1);
       float val = 0.f;
 #pragma unroll
       for (int u = 0; u < TAPS; ++u) {
-        long long rr = r0 + OFF + u;
-        rr = rr < 0 ? 0 : (rr > fm ? fm : rr);
-        long long ph = st + rr;
-        if (ph >= a.cap) ph -= a.cap;
-        const float* br = bl + ph * a.W;
+        int ph = st + min(max(r0 + OFF + u, 0), fm);
+        ph = ph >= a.cap ? ph - a.cap : ph;
+        const float* br = bl + ph * a.ld;
         float rs = 0.f;
 #pragma unroll
-        for (int v = 0; v < TAPS; ++v) {
-          long long cc = c0 + OFF + v;
-          cc = cc < 0 ? 0 : (cc > a.W - 1 ? a.W - 1 : cc);
-          rs = fmaf(wc[v], br[cc], rs);
-        }
+        for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], br[cc[v]], rs);
         val = fmaf(wr[u], rs, val);
       }
       acc += val;
@@ -238,79 +226,199 @@ __global__ __launch_bounds__(256) void k_readout(ReadoutArgs a) {
   }
 }
 
+// Box-staged readout for n >= 32: a workgroup owns a 32 x 32 pixel tile.
+// For each (layer, direction) the tile's footprint in the ring buffer is an
+// affine image of the tile; its bounding box plus the tap margins is loaded
+// into LDS with coalesced row reads (rows clamped to the valid ring rows and
+// columns to [0, W-1] -- exactly the per-tap clamping), then the taps are
+// read from LDS.  A footprint larger than the box (magnification > ~1.1)
+// falls back to the direct gather for that (layer, direction).
+constexpr int TB = 32;    // tile edge
+constexpr int BOXE = 60;  // box edge: 31 * sqrt(2) + taps + 2 spare
+
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_readout_box(ReadoutArgs a) {
+  constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
+  __shared__ float box[BOXE][BOXE + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int i0 = blockIdx.y * TB, j0 = blockIdx.x * TB;
+  const int jl = tid & 31, il = tid >> 5;  // pixel (i0 + il + 8k, j0 + jl), k < 4
+  const float h = 0.5f * (float)(a.n - 1);
+  const int ti1 = min(i0 + TB, a.n) - 1, tj1 = min(j0 + TB, a.n) - 1;
+  const int d = blockIdx.z;  // one direction per block: short dependency chains
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int l = 0; l < a.L; ++l) {
+    const float c = a.cosw[l], s = a.sinw[l];
+    const float* bl = a.buf + (size_t)l * a.cap * a.ld;
+    const int fm = a.fillm1[l], st = a.start[l];
+    {
+      const float mg = a.mag[d][l], rfr = a.r_frac[d][l], cfr = a.c_frac[d][l];
+      const int rint = a.r_int[d][l], cint = a.c_int[d][l];
+      // footprint of the tile (affine map: extremes at the corners)
+      float rmin = 3e38f, rmax = -3e38f, cmin = 3e38f, cmax = -3e38f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float gi = ((float)(q & 1 ? ti1 : i0) - h) * mg, gj = ((float)(q & 2 ? tj1 : j0) - h) * mg;
+        const float r = fmaf(gi, c, fmaf(gj, s, rfr)), cc = fmaf(-gi, s, fmaf(gj, c, cfr));
+        rmin = fminf(rmin, r);
+        rmax = fmaxf(rmax, r);
+        cmin = fminf(cmin, cc);
+        cmax = fmaxf(cmax, cc);
+      }
+      // one spare row/column each side absorbs last-bit rounding of interior pixels
+      const int rb0 = (int)floorf(rmin) + OFF - 1, cb0 = (int)floorf(cmin) + OFF - 1;
+      const int nr = (int)floorf(rmax) + OFF + TAPS + 1 - rb0;
+      const int nc = (int)floorf(cmax) + OFF + TAPS + 1 - cb0;
+      const bool fits = nr <= BOXE && nc <= BOXE;  // uniform across the block
+      if (fits) {
+        __syncthreads();  // previous box fully consumed
+        if (lane < nc) {
+          const int cc = min(max(cb0 + cint + lane, 0), a.W - 1);
+          for (int br = wave; br < nr; br += 4) {
+            int ph = st + min(max(rb0 + rint + br, 0), fm);
+            ph = ph >= a.cap ? ph - a.cap : ph;
+            box[br][lane] = bl[ph * a.ld + cc];
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int i = i0 + il + 8 * k, j = j0 + jl;
+        if (i >= a.n || j >= a.n) continue;
+        const float gi = ((float)i - h) * mg, gj = ((float)j - h) * mg;
+        const float row = fmaf(gi, c, fmaf(gj, s, rfr));
+        const float col = fmaf(-gi, s, fmaf(gj, c, cfr));
+        const float rf = floorf(row), cf = floorf(col);
+        float wr[TAPS], wc[TAPS];
+        taps_weights<TAPS>(row - rf, wr);
+        taps_weights<TAPS>(col - cf, wc);
+        float val = 0.f;
+        if (fits) {
+          const int br = min(max((int)rf + OFF - rb0, 0), nr - TAPS);
+          const int bc = min(max((int)cf + OFF - cb0, 0), nc - TAPS);
+#pragma unroll
+          for (int u = 0; u < TAPS; ++u) {
+            float rs = 0.f;
+#pragma unroll
+            for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], box[br + u][bc + v], rs);
+            val = fmaf(wr[u], rs, val);
+          }
+        } else {
+          const int r0 = (int)rf + rint, c0 = (int)cf + cint;
+#pragma unroll
+          for (int u = 0; u < TAPS; ++u) {
+            int ph = st + min(max(r0 + OFF + u, 0), fm);
+            ph = ph >= a.cap ? ph - a.cap : ph;
+            const float* brow = bl + ph * a.ld;
+            float rs = 0.f;
+#pragma unroll
+            for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], brow[min(max(c0 + OFF + v, 0), a.W - 1)], rs);
+            val = fmaf(wr[u], rs, val);
+          }
+        }
+        acc[k] += val;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = i0 + il + 8 * k, j = j0 + jl;
+    if (i >= a.n || j >= a.n) continue;
+    float* o = a.out + ((size_t)d * a.n + i) * a.n + j;
+    const float v = acc[k] * a.scale;
+    *o = a.accumulate ? *o + v : v;
+  }
+}
+
 }  // namespace pt
 
 using namespace pt;
 
-extern "C" int pt_innovation_gemm(const float* beta, const float* B, int rows, int width, float* out,
+extern "C" int pt_innovation_gemm(const float* beta, const float* B, int rows, int width, int ld, float* out,
                                   void* stream) {
   if (!beta || !B || !out) return fail(PT_ERR_ARG, "pt_innovation_gemm: NULL array");
-  if (rows < 0 || width < 1) return fail(PT_ERR_ARG, "pt_innovation_gemm: bad sizes");
-  if (rows == 0) return PT_OK;
-  if ((rows + GT - 1) / GT > 65535) return fail(PT_ERR_ARG, "pt_innovation_gemm: too many rows");
-  dim3 g((width + GT - 1) / GT, (rows + GT - 1) / GT);
-  k_gemm_nt<<<g, 256, 0, (hipStream_t)stream>>>(beta, B, out, rows, width, width);
-  PT_LAUNCH_CHECK();
-  return PT_OK;
-}
-
-extern "C" int pt_extrude_rows(const float* A, int width, int stencil_rows, const float* bb, float* buf,
-                               long long cap, int nact, const int* layer, const int* zrows, const int* trow,
-                               const int* bbrow, const float* bscale, void* stream) {
-  if (!A || !buf || !layer || !zrows || !trow || !bbrow || !bscale || !bb)
-    return fail(PT_ERR_ARG, "pt_extrude_rows: NULL array");
-  if (width < 1 || stencil_rows < 1 || cap < stencil_rows + 1 || nact < 0)
-    return fail(PT_ERR_ARG, "pt_extrude_rows: bad sizes");
-  if (stencil_rows > MMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_rows: stencil_rows > %d", MMAX);
-  if (nact == 0) return PT_OK;
+  if (rows < 0 || width < 1 || ld < width || (ld & 3)) return fail(PT_ERR_ARG, "pt_innovation_gemm: bad sizes (ld %% 4)");
+  if (!aligned16(beta) || !aligned16(B) || !aligned16(out)) return fail(PT_ERR_ARG, "pt_innovation_gemm: arrays must be 16-byte aligned");
   hipStream_t st = (hipStream_t)stream;
-  for (int e0 = 0; e0 < nact; e0 += XMAX) {
-    RowsArgs a;
-    a.A = A;
-    a.bb = bb;
-    a.buf = buf;
-    a.cap = cap;
+  for (int k0 = 0; k0 < rows; k0 += VMAX) {
+    GemvArgs a = {};
+    a.M = B;
+    a.ldm = ld;
     a.W = width;
-    a.m = stencil_rows;
-    a.nact = nact - e0 < XMAX ? nact - e0 : XMAX;
-    for (int e = 0; e < XMAX; ++e) {
-      const int src = e0 + (e < a.nact ? e : 0);
-      a.layer[e] = layer[src];
-      for (int s = 0; s < MMAX; ++s) {
-        const int r = s < stencil_rows ? zrows[src * stencil_rows + s] : 0;
-        if (r < 0 || r >= cap) return fail(PT_ERR_ARG, "pt_extrude_rows: stencil row %d outside ring", r);
-        a.zrow[e][s] = r;
-      }
-      if (trow[src] < 0 || trow[src] >= cap) return fail(PT_ERR_ARG, "pt_extrude_rows: target row outside ring");
-      a.trow[e] = trow[src];
-      a.bbrow[e] = bbrow[src];
-      a.bscale[e] = bscale[src];
+    a.ld = ld;
+    a.m = 1;
+    a.nv = rows - k0 < VMAX ? rows - k0 : VMAX;
+    for (int e = 0; e < VMAX; ++e) {
+      const int k = k0 + (e < a.nv ? e : 0);
+      a.v[e][0] = beta + (size_t)k * ld;
+      a.out[e] = out + (size_t)k * ld;
+      a.add[e] = nullptr;
+      a.scale[e] = 0.f;
     }
-    k_extrude_rows<<<(width + 3) / 4, 256, 0, st>>>(a);
-    PT_LAUNCH_CHECK();
+    int rc = launch_gemv(a, st);
+    if (rc) return rc;
   }
   return PT_OK;
 }
 
-extern "C" int pt_readout(const float* buf, int layers, long long cap, int width, const int* start,
+extern "C" int pt_extrude_rows(const float* A, int width, int ld, int stencil_rows, const float* bb, float* buf,
+                               long long cap, int nact, const int* layer, const int* zrows, const int* trow,
+                               const int* bbrow, const float* bscale, void* stream) {
+  if (!A || !buf || !layer || !zrows || !trow || !bbrow || !bscale || !bb)
+    return fail(PT_ERR_ARG, "pt_extrude_rows: NULL array");
+  if (width < 1 || ld < width || (ld & 3) || stencil_rows < 1 || cap < stencil_rows + 1 || nact < 0)
+    return fail(PT_ERR_ARG, "pt_extrude_rows: bad sizes (ld %% 4 == 0, ld >= width)");
+  if (stencil_rows > MMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_rows: stencil_rows > %d", MMAX);
+  if (!aligned16(A) || !aligned16(buf) || !aligned16(bb)) return fail(PT_ERR_ARG, "pt_extrude_rows: arrays must be 16-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
+  for (int e0 = 0; e0 < nact; e0 += VMAX) {
+    GemvArgs a = {};
+    a.M = A;
+    a.ldm = (long long)stencil_rows * ld;
+    a.W = width;
+    a.ld = ld;
+    a.m = stencil_rows;
+    a.nv = nact - e0 < VMAX ? nact - e0 : VMAX;
+    for (int e = 0; e < VMAX; ++e) {
+      const int src = e0 + (e < a.nv ? e : 0);
+      const size_t slab = (size_t)layer[src] * cap;
+      for (int s = 0; s < MMAX; ++s) {
+        const int r = zrows[src * stencil_rows + (s < stencil_rows ? s : 0)];
+        if (r < 0 || r >= cap) return fail(PT_ERR_ARG, "pt_extrude_rows: stencil row %d outside ring", r);
+        a.v[e][s] = buf + (slab + r) * ld;
+      }
+      if (trow[src] < 0 || trow[src] >= cap) return fail(PT_ERR_ARG, "pt_extrude_rows: target row outside ring");
+      a.out[e] = buf + (slab + trow[src]) * ld;
+      a.add[e] = bb + (size_t)bbrow[src] * ld;
+      a.scale[e] = bscale[src];
+    }
+    int rc = launch_gemv(a, st);
+    if (rc) return rc;
+  }
+  return PT_OK;
+}
+
+extern "C" int pt_readout(const float* buf, int layers, long long cap, int width, int ld, const int* start,
                           const long long* fill, const double* wind_cs, const double* dir_geom, int dirs, int n,
                           int interp, float scale, int accumulate, float* out, void* stream) {
   if (!buf || !start || !fill || !wind_cs || !dir_geom || !out) return fail(PT_ERR_ARG, "pt_readout: NULL array");
-  if (layers < 1 || dirs < 1 || n < 1 || width < 1 || cap < 1) return fail(PT_ERR_ARG, "pt_readout: bad sizes");
+  if (layers < 1 || dirs < 1 || n < 1 || width < 1 || cap < 1 || ld < width) return fail(PT_ERR_ARG, "pt_readout: bad sizes");
+  if (cap * (long long)ld > (1LL << 30)) return fail(PT_ERR_UNSUPPORTED, "pt_readout: ring slab over 2^30 floats");
   if (interp < 0 || interp > 2) return fail(PT_ERR_ARG, "pt_readout: interp must be 0 linear, 1 cubic, 2 lanczos");
   for (int l = 0; l < layers; ++l) {
     if (fill[l] < 1 || fill[l] > cap || start[l] < 0 || start[l] >= cap)
       return fail(PT_ERR_ARG, "pt_readout: layer %d ring state out of range", l);
   }
   hipStream_t st = (hipStream_t)stream;
-  dim3 g((n + 31) / 32, (n + 7) / 8);
   for (int d0 = 0; d0 < dirs; d0 += RD) {
     for (int l0 = 0; l0 < layers; l0 += RL) {
       ReadoutArgs a;
-      a.buf = buf + (size_t)l0 * cap * width;
+      a.buf = buf + (size_t)l0 * cap * ld;
       a.out = out + (size_t)d0 * n * n;
-      a.cap = cap;
+      a.cap = (int)cap;
       a.W = width;
+      a.ld = ld;
       a.n = n;
       a.L = layers - l0 < RL ? layers - l0 : RL;
       a.D = dirs - d0 < RD ? dirs - d0 : RD;
@@ -320,23 +428,38 @@ extern "C" int pt_readout(const float* buf, int layers, long long cap, int width
         const int src = l0 + (l < a.L ? l : 0);
         a.start[l] = start[src];
         a.fillm1[l] = (int)(fill[src] - 1);
-        a.cosw[l] = wind_cs[2 * src];
-        a.sinw[l] = wind_cs[2 * src + 1];
+        a.cosw[l] = (float)wind_cs[2 * src];
+        a.sinw[l] = (float)wind_cs[2 * src + 1];
       }
       for (int d = 0; d < RD; ++d)
         for (int l = 0; l < RL; ++l) {
           const int sd = d0 + (d < a.D ? d : 0), sl = l0 + (l < a.L ? l : 0);
           const double* gp = dir_geom + ((size_t)sd * layers + sl) * 3;
-          a.sa[d][l] = gp[0];
-          a.sp[d][l] = gp[1];
-          a.mag[d][l] = gp[2];
+          const double rs = gp[0], cs = 0.5 * width + gp[1];  // row / column shifts
+          const double rfl = floor(rs), cfl = floor(cs);
+          if (fabs(rfl) > 1e9 || fabs(cfl) > 1e9) return fail(PT_ERR_ARG, "pt_readout: shift out of range");
+          a.r_int[d][l] = (int)rfl;
+          a.c_int[d][l] = (int)cfl;
+          a.r_frac[d][l] = (float)(rs - rfl);
+          a.c_frac[d][l] = (float)(cs - cfl);
+          a.mag[d][l] = (float)gp[2];
         }
-      if (interp == 0)
+      const dim3 g((n + 31) / 32, (n + 7) / 8, a.D);
+      if (n >= TB) {
+        dim3 gb((n + TB - 1) / TB, (n + TB - 1) / TB, a.D);
+        if (interp == 0)
+          k_readout_box<2><<<gb, 256, 0, st>>>(a);
+        else if (interp == 1)
+          k_readout_box<4><<<gb, 256, 0, st>>>(a);
+        else
+          k_readout_box<6><<<gb, 256, 0, st>>>(a);
+      } else if (interp == 0) {
         k_readout<2><<<g, 256, 0, st>>>(a);
-      else if (interp == 1)
+      } else if (interp == 1) {
         k_readout<4><<<g, 256, 0, st>>>(a);
-      else
+      } else {
         k_readout<6><<<g, 256, 0, st>>>(a);
+      }
       PT_LAUNCH_CHECK();
     }
   }

@@ -29,7 +29,7 @@ from . import _hip
 from .backend import resolve_device
 from .config import ExtrusionConfig
 from .fourier import PhaseScreen
-from .infinite import device_extrusion, extrude_rounds, readout
+from .infinite import device_extrusion, extrude_rounds, readout, row_pitch
 from .ring import Ring
 
 __all__ = ["ExtrudedAtmosphere", "along_extent_and_span"]
@@ -85,7 +85,7 @@ class ExtrudedAtmosphere:
         self.capacity = max(s for _, s in ext_span) + self.n + self.m + 32
         W, cap = self.width, self.capacity
         t = _hip.torch()
-        self._buf = t.empty((L, cap, W), dtype=t.float32, device=self._dev)
+        self._buf = t.zeros((L, cap, row_pitch(W)), dtype=t.float32, device=self._dev)
         self._innovations = innovations
         self.layers: List[_Layer] = []
         g = (np.arange(self.n, dtype=np.float64) - (self.n - 1) / 2.0)
@@ -115,7 +115,7 @@ class ExtrudedAtmosphere:
                 gen = PhaseScreen(W, self.dx, float(r0), float(L0), seed=ly.rng,
                                   device=self.device, dtype=self.dtype.name)
                 init = gen.generate()
-            self._buf[i, :span] = init[:span].float()
+            self._buf[i, :span, :W] = init[:span].float()
             ly.ring = Ring(cap, self.m, int(np.floor(ly.along_min - ly.fov)) - 2, span)
             ly.travel = 0.0
             self.layers.append(ly)

@@ -82,19 +82,35 @@ def build_extrusion(width: int, stencil_rows: int, pixel_scale: float, L0: float
 _MATRIX_CACHE: Dict[tuple, Tuple[Any, Any]] = {}
 
 
+def row_pitch(width: int) -> int:
+    """Device row pitch in floats: a multiple of 64 (256-byte rows)."""
+    return (int(width) + 63) // 64 * 64
+
+
 def device_extrusion(width, stencil_rows, pixel_scale, L0, r0, dev):
-    """(A, B) as float32 device tensors, cached per geometry and device."""
+    """(A, B) as float32 device tensors with pitched, zero-padded rows.
+
+    A is (W, m*ld) with stencil row s in columns [s*ld, s*ld + W); B is
+    (W, ld).  Cached per geometry and device.
+    """
     key = (int(width), int(stencil_rows), float(pixel_scale), round(float(L0), 9),
            float(r0), str(dev))
     hit = _MATRIX_CACHE.get(key)
     if hit is None:
-        a, b = build_extrusion(width, stencil_rows, pixel_scale, L0, r0)
-        hit = (_hip.to_device(a, np.float32, dev), _hip.to_device(b, np.float32, dev))
+        W, m = int(width), int(stencil_rows)
+        ld = row_pitch(W)
+        a, b = build_extrusion(W, m, pixel_scale, L0, r0)
+        ap = np.zeros((W, m * ld), dtype=np.float32)
+        for s_ in range(m):
+            ap[:, s_ * ld:s_ * ld + W] = a[:, s_ * W:(s_ + 1) * W]
+        bp = np.zeros((W, ld), dtype=np.float32)
+        bp[:, :W] = b
+        hit = (_hip.to_device(ap, None, dev), _hip.to_device(bp, None, dev))
         _MATRIX_CACHE[key] = hit
     return hit
 
 
-def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, groups=None):
+def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales):
     """Run the planned rows of several layers on the device.
 
     ``plans[l]``: list of (target_phys, z_phys) for layer l, in order;
@@ -104,6 +120,7 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, groups=None)
     """
     t = _hip.torch()
     L = len(plans)
+    ld = int(buf.shape[-1])
     lib = _hip.lib()
     st = _hip.stream_ptr()
     As = A if isinstance(A, (list, tuple)) else [A] * L
@@ -115,10 +132,16 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, groups=None)
         members = [l for l in range(L) if id(Bs[l]) == key and len(plans[l])]
         if not members:
             continue
-        beta = t.cat([betas[l] for l in members]).contiguous()
+        total = sum(len(plans[l]) for l in members)
+        beta = t.zeros((total, ld), dtype=t.float32, device=buf.device)
+        off = 0
+        for l in members:
+            k = len(plans[l])
+            beta[off:off + k, :width] = betas[l]
+            off += k
         bb = t.empty_like(beta)
-        rc = lib.pt_innovation_gemm(_hip.ptr(beta), _hip.ptr(Bs[members[0]]), beta.shape[0],
-                                    width, _hip.ptr(bb), st)
+        rc = lib.pt_innovation_gemm(_hip.ptr(beta), _hip.ptr(Bs[members[0]]), total,
+                                    width, ld, _hip.ptr(bb), st)
         _hip.check(rc, "pt_innovation_gemm")
         off = 0
         for l in members:
@@ -136,7 +159,7 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, groups=None)
             tr = (ctypes_int * n)(*[plans[l][r][0] for l in grp])
             br = (ctypes_int * n)(*[bbrow[l] + r for l in grp])
             sc = (ctypes_float * n)(*[float(bscales[l]) for l in grp])
-            rc = lib.pt_extrude_rows(_hip.ptr(As[grp[0]]), width, m, _hip.ptr(bb_of[grp[0]]),
+            rc = lib.pt_extrude_rows(_hip.ptr(As[grp[0]]), width, ld, m, _hip.ptr(bb_of[grp[0]]),
                                      _hip.ptr(buf), cap, n, layer, zr, tr, br, sc, st)
             _hip.check(rc, "pt_extrude_rows")
 
@@ -161,7 +184,8 @@ def readout(buf, cap, width, rings, wind_cs, dir_geom, n, interp, scale=1.0, out
     fill = (ctypes_ll * L)(*[r.fill for r in rings])
     cs = (ctypes_double * (2 * L))(*[v for c in wind_cs for v in c])
     geom = (ctypes_double * (3 * D * L))(*np.asarray(dir_geom, dtype=np.float64).ravel())
-    rc = _hip.lib().pt_readout(_hip.ptr(buf), L, cap, width, start, fill, cs, geom, D, n,
+    rc = _hip.lib().pt_readout(_hip.ptr(buf), L, cap, width, int(buf.shape[-1]), start, fill,
+                               cs, geom, D, n,
                                _INTERP_CODE[interp], float(scale), int(bool(accumulate)),
                                _hip.ptr(out), _hip.stream_ptr())
     _hip.check(rc, "pt_readout")
@@ -205,8 +229,9 @@ class InfinitePhaseScreen:
         self._innovations = innovations
         self._capacity = self.n + self.stencil_rows + 6
         t = _hip.torch()
-        self._buf = t.empty((1, self._capacity, self.n), dtype=t.float32, device=self._dev)
-        self._buf[0, :self.n] = gen.generate().float()
+        self._buf = t.zeros((1, self._capacity, row_pitch(self.n)), dtype=t.float32,
+                            device=self._dev)
+        self._buf[0, :self.n, :self.n] = gen.generate().float()
         self._ring = Ring(self._capacity, self.stencil_rows, 0, self.n)
         self._travel = 0.0
         self._count = 0

@@ -22,35 +22,40 @@ def _np(t):
 
 
 @pytest.mark.parametrize("interp", ["linear", "cubic", "lanczos"])
-def test_readout_matches_oracle_on_random_ring(hip, interp):
-    """Pure-function check of pt_readout incl. ring wrap, clamping, mags."""
+@pytest.mark.parametrize("n,W,cap", [(16, 30, 40), (80, 150, 160)])
+def test_readout_matches_oracle_on_random_ring(hip, interp, n, W, cap):
+    """Pure-function check of pt_readout: ring wrap, row/column clamping,
+    per-direction magnification; n=16 runs the direct-gather kernel, n=80
+    the LDS box kernel with partial tiles."""
     import torch
     from pyturb_amd.infinite import readout
     from pyturb_amd.ring import Ring
 
-    print(f"readout {interp}", flush=True)
+    print(f"readout {interp} n={n}", flush=True)
     rng = np.random.default_rng(3)
-    L, cap, W, n = 3, 40, 30, 16
+    L = 3
     buf = rng.standard_normal((L, cap, W))
     rings = []
-    for l, (start, fill) in enumerate([(0, 40), (13, 35), (37, 30)]):
+    for start, fill in [(0, cap), (cap // 3, cap - 5), (cap - 3, cap // 2)]:
         r = Ring(cap, 2, 0, fill)
         r.start = start
         rings.append(r)
     cs = [(np.cos(a), np.sin(a)) for a in (0.3, 2.0, -1.1)]
-    geom = np.array([[[10.3, 0.7, 1.0], [12.9, -1.3, 1.0], [9.5, 0.2, 1.0]],
-                     [[11.1, 2.2, 0.8], [14.4, 0.0, 0.6], [8.0, -3.0, 0.9]]])
+    c = 0.4 * n  # centre the footprints in the buffer; some taps still clamp
+    geom = np.array([[[c + 0.3, 0.7, 1.0], [c + 2.9, -1.3, 1.0], [c - 0.5, 0.2, 1.0]],
+                     [[c + 1.1, 2.2, 0.8], [c + 4.4, 0.0, 0.6], [c - 2.0, -3.0, 0.9]],
+                     [[2.5, -0.4, 1.0], [float(cap), 1.5, 1.0], [1.0, float(W), 0.7]]])
     dev = torch.device("cuda")
     got = _np(readout(torch.from_numpy(buf.astype(np.float32)).to(dev), cap, W, rings, cs,
                       geom, n, interp, scale=0.5))
-    # oracle: unroll each ring into local-row order
     local = np.stack([np.roll(buf[l], -rings[l].start, axis=0) for l in range(L)])
     fill = np.array([r.fill for r in rings])
-    for d in range(2):
-        along, perp = zip(*[O.pupil_grids(n, c, s, W, geom[d, l, 2]) for l, (c, s) in enumerate(cs)])
+    for d in range(len(geom)):
+        along, perp = zip(*[O.pupil_grids(n, cc, ss, W, geom[d, l, 2])
+                            for l, (cc, ss) in enumerate(cs)])
         ref = O.readout(local.astype(np.float32).astype(np.float64), fill, np.stack(along),
                         np.stack(perp), geom[d, :, 0], geom[d, :, 1], interp) * 0.5
-        assert O.rel_rms(got[d], ref) < REL
+        assert O.rel_rms(got[d], ref) < REL, (d, O.rel_rms(got[d], ref))
 
 
 @pytest.mark.parametrize("interp", ["linear", "cubic", "lanczos"])
