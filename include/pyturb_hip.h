@@ -175,6 +175,50 @@ int pt_readout(const float* buf, int layers, long long cap, int width, int ld,
                const double* dir_geom, int dirs, int n, int interp, float scale,
                int accumulate, float* out, void* stream);
 
+/*
+ * Differentiable frame model (profile and wind fitting).  The reference has
+ * no derivatives; these entry points serve a fit of per-layer Cn2 weights,
+ * r0, L0 and winds to OPD frames.  The layer spectra are written as
+ *   S_l(k) = gain_l * amplitude[group_l](k) * (w_l,re(k) + i w_l,im(k)),
+ * gain_l = r0_l^(-5/6) for unit-r0 amplitudes (src/pyturb/atmosphere.py:
+ * 401-402 and 532-579, src/pyturb/fourier.py:143,174-191), so the forward
+ * model is pt_layer_spectra followed by pt_spectral_frames.
+ *
+ * pt_layer_spectra: noise (layers, 2, n, n) f32 white-noise planes,
+ *   amplitude (groups, n, n) f32; HOST arrays group (layers) int and gain
+ *   (layers) f64; spectra (layers, n, n) c64 out.  layers <= 16.
+ */
+int pt_layer_spectra(const float* noise, const float* amplitude,
+                     const int* group, const double* gain, int layers, int n,
+                     void* spectra, void* stream);
+
+/* Workspace (c64 elements) of pt_spectral_frames_adjoint; -1 on bad input. */
+long long pt_adjoint_work_elems(int n, int frames, int levels);
+
+/*
+ * Adjoint of pt_layer_spectra + pt_spectral_frames for a real cotangent
+ * grad = dLoss/d(out) (frames, n_out, n_out) f32 (the output crop of the
+ * forward call).  Layer spectra are recomputed from the noise, never read.
+ *   dlnamp  : (groups, n, n) f32, d ln(amplitude) / d L0
+ *   sh_unit : (levels, layers, 3, 3) c128 unit-gain subharmonic coefficients
+ *             (the forward's sh_coeffs are gain_l * sh_unit)
+ *   sh_rho  : (levels, layers, 3, 3) f64, d ln|sh_unit| / d L0
+ *   disp    : (frames, layers, 2) f64 device, as in pt_spectral_frames
+ *   out     : (frames, layers, 4) f64 device:
+ *             dLoss/dgain_l, dLoss/ds_x, dLoss/ds_y, dLoss/dL0_l per frame
+ *             (sum over frames for the per-layer parameters)
+ *   group, gain: HOST arrays as for pt_layer_spectra.
+ * Power-of-two n from 32 to 2048, layers <= 16, levels <= 12.
+ */
+int pt_spectral_frames_adjoint(const float* noise, const float* amplitude,
+                               const float* dlnamp, const int* group,
+                               const double* gain, int layers, int n,
+                               double pixel_scale, const double* disp,
+                               int frames, const float* grad, int crop_offset,
+                               int n_out, const double* sh_unit,
+                               const double* sh_rho, int levels, double scale,
+                               double* out, void* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

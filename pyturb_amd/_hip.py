@@ -45,6 +45,10 @@ SIGNATURES = {
                              _vp, _vp, _vp]),
     "pt_readout": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _f,
                         _i, _vp, _vp]),
+    "pt_layer_spectra": (_i, [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp]),
+    "pt_adjoint_work_elems": (ctypes.c_longlong, [_i, _i, _i]),
+    "pt_spectral_frames_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i, _vp, _i, _i,
+                                        _vp, _vp, _i, _d, _vp, _vp, _vp]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

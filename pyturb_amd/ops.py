@@ -132,3 +132,70 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
                                        _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
     _hip.check(rc, "pt_spectral_frames")
     return out
+
+
+def layer_spectra(noise, amplitude, group, gain, out=None):
+    """Layer spectra ``gain_l * amplitude[group_l] * (w_re + i w_im)`` (pt_layer_spectra).
+
+    ``noise`` (L, 2, n, n) float32, ``amplitude`` (G, n, n) float32; ``group``
+    and ``gain`` are host sequences of length L.  Returns (L, n, n) complex64.
+    """
+    t = _t()
+    L, n = int(noise.shape[0]), int(noise.shape[-1])
+    G = int(amplitude.shape[0])
+    _require(noise, t.float32, (L, 2, n, n), "noise")
+    _require(amplitude, t.float32, (G, n, n), "amplitude")
+    grp = np.ascontiguousarray(np.asarray(group, dtype=np.int32))
+    gn = np.ascontiguousarray(np.asarray(gain, dtype=np.float64))
+    if grp.shape != (L,) or gn.shape != (L,) or grp.min() < 0 or grp.max() >= G:
+        raise ValueError("layer_spectra: group/gain must have one entry per layer, groups < G")
+    if out is None:
+        out = t.empty((L, n, n), dtype=t.complex64, device=noise.device)
+    _require(out, t.complex64, (L, n, n), "out")
+    rc = _hip.lib().pt_layer_spectra(_hip.ptr(noise), _hip.ptr(amplitude), grp.ctypes.data,
+                                     gn.ctypes.data, L, n, _hip.ptr(out), _hip.stream_ptr())
+    _hip.check(rc, "pt_layer_spectra")
+    return out
+
+
+def spectral_frames_adjoint(noise, amplitude, dlnamp, group, gain, disp, grad, sh_unit, sh_rho,
+                            pixel_scale: float, *, crop_offset: int = 0, scale: float = 1.0,
+                            out=None, work=None):
+    """Adjoint of layer_spectra + spectral_frames (pt_spectral_frames_adjoint).
+
+    ``grad`` (F, n_out, n_out) float32 is dLoss/dframes.  Returns (F, L, 4)
+    float64: dLoss/dgain_l, dLoss/ds_x, dLoss/ds_y, dLoss/dL0_l per frame.
+    """
+    t = _t()
+    L, n = int(noise.shape[0]), int(noise.shape[-1])
+    G = int(amplitude.shape[0])
+    _require(noise, t.float32, (L, 2, n, n), "noise")
+    _require(amplitude, t.float32, (G, n, n), "amplitude")
+    _require(dlnamp, t.float32, (G, n, n), "dlnamp")
+    F = int(disp.shape[0])
+    _require(disp, t.float64, (F, L, 2), "disp")
+    n_out = int(grad.shape[-1])
+    _require(grad, t.float32, (F, n_out, n_out), "grad")
+    P = 0 if sh_unit is None else int(sh_unit.shape[0])
+    if P:
+        _require(sh_unit, t.complex128, (P, L, 3, 3), "sh_unit")
+        _require(sh_rho, t.float64, (P, L, 3, 3), "sh_rho")
+    grp = np.ascontiguousarray(np.asarray(group, dtype=np.int32))
+    gn = np.ascontiguousarray(np.asarray(gain, dtype=np.float64))
+    if grp.shape != (L,) or gn.shape != (L,) or grp.min() < 0 or grp.max() >= G:
+        raise ValueError("spectral_frames_adjoint: group/gain must have one entry per layer")
+    if out is None:
+        out = t.empty((F, L, 4), dtype=t.float64, device=noise.device)
+    _require(out, t.float64, (F, L, 4), "out")
+    need = int(_hip.lib().pt_adjoint_work_elems(n, F, P))
+    if need < 0:
+        raise ValueError("pt_adjoint_work_elems: bad arguments")
+    if work is None or work.numel() < need:
+        work = t.empty((need,), dtype=t.complex64, device=noise.device)
+    rc = _hip.lib().pt_spectral_frames_adjoint(
+        _hip.ptr(noise), _hip.ptr(amplitude), _hip.ptr(dlnamp), grp.ctypes.data, gn.ctypes.data,
+        L, n, float(pixel_scale), _hip.ptr(disp), F, _hip.ptr(grad), int(crop_offset), n_out,
+        _hip.ptr(sh_unit) if P else None, _hip.ptr(sh_rho) if P else None, P, float(scale),
+        _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "pt_spectral_frames_adjoint")
+    return out

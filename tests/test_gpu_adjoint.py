@@ -1,0 +1,182 @@
+"""GPU checks of the differentiable frame model and its HIP adjoint.
+
+Forward against the oracle (<= 1e-5 relative RMS); gradients with respect to
+layer gain, outer scale and displacement against FP64 central finite
+differences of the oracle (h = 1e-6) and against the plain-torch autograd
+reference (both within 1e-4 relative, norm-wise per parameter group); the
+full chain (softmax Cn2 fractions, r0, winds) against the torch reference;
+a short Adam fit of a hidden profile.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_adjoint_host import oracle_frames
+
+pytestmark = pytest.mark.gpu
+
+GRAD_REL = 1e-4
+
+
+def _case(n, L, T, P, seed):
+    rng = np.random.default_rng(seed)
+    return dict(
+        n=n, L=L, T=T, P=P, dx=8.0 / n,
+        noise=rng.standard_normal((L, 2, n, n)).astype(np.float32).astype(np.float64),
+        sh_noise=rng.standard_normal((P, L, 2, 3, 3)),
+        gain=rng.uniform(0.5, 2.0, L), L0=rng.uniform(15.0, 40.0, L),
+        wind=rng.uniform(-25.0, 25.0, (L, 2)),
+        times=np.concatenate([[0.0], np.sort(rng.uniform(0.0, 0.6, T - 1))]),
+        G=rng.standard_normal((T, n, n)).astype(np.float32).astype(np.float64))
+
+
+def _model(c):
+    from pyturb_amd.fit import ProfileFrames
+
+    return ProfileFrames(c["n"], c["dx"], c["L"], subharmonics=c["P"], noise=c["noise"],
+                         sh_noise=c["sh_noise"])
+
+
+def _device_grads(model, c):
+    import torch
+    from pyturb_amd.fit import _SpectralFramesFn
+
+    dev = torch.device("cuda")
+    gain = torch.tensor(c["gain"], device=dev, requires_grad=True)
+    L0 = torch.tensor(c["L0"], device=dev, requires_grad=True)
+    disp = torch.tensor(c["times"][:, None, None] * c["wind"][None], device=dev,
+                        requires_grad=True)
+    out = _SpectralFramesFn.apply(gain, L0, disp, model)
+    (out.double() * torch.tensor(c["G"], device=dev)).sum().backward()
+    return out.detach().cpu().numpy(), gain.grad.cpu().numpy(), L0.grad.cpu().numpy(), \
+        disp.grad.cpu().numpy()
+
+
+def _relnorm(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("T", [5, 6])
+def test_forward_and_adjoint_match_fp64_finite_differences(hip, T):
+    print(f"adjoint vs FD n=64 L=9 T={T}", flush=True)
+    c = _case(64, 9, T, 3, seed=11 + T)
+    model = _model(c)
+    scale = model.scale
+    disp = c["times"][:, None, None] * c["wind"][None]
+    out, g_gain, g_L0, g_disp = _device_grads(model, c)
+    ref = oracle_frames(c["noise"], c["sh_noise"], c["gain"], c["L0"], disp, c["dx"], c["P"],
+                        scale)
+    assert O.rel_rms(out, ref) < 1e-5
+
+    def loss(gn, l0, dp):
+        fr = oracle_frames(c["noise"], c["sh_noise"], gn, l0, dp, c["dx"], c["P"], scale)
+        return float(np.sum(fr * c["G"]))
+
+    h = 1e-6
+    eye = np.eye(c["L"])
+    fd_gain = np.array([(loss(c["gain"] + h * e, c["L0"], disp) -
+                         loss(c["gain"] - h * e, c["L0"], disp)) / (2 * h) for e in eye])
+    fd_L0 = np.array([(loss(c["gain"], c["L0"] + h * e, disp) -
+                       loss(c["gain"], c["L0"] - h * e, disp)) / (2 * h) for e in eye])
+    fd_disp = np.zeros_like(disp)
+    for idx in np.ndindex(*disp.shape):
+        e = np.zeros_like(disp)
+        e[idx] = h
+        fd_disp[idx] = (loss(c["gain"], c["L0"], disp + e) -
+                        loss(c["gain"], c["L0"], disp - e)) / (2 * h)
+    errs = (_relnorm(g_gain, fd_gain), _relnorm(g_L0, fd_L0), _relnorm(g_disp, fd_disp))
+    print("rel err gain/L0/disp", errs)
+    assert max(errs) < GRAD_REL, errs
+
+
+def test_adjoint_matches_torch_autograd_at_512(hip):
+    """Config-5 scale: 9 layers, 512^2, 8 subharmonic levels."""
+    import torch
+
+    import torch_reference as R
+
+    print("adjoint vs torch autograd n=512 L=9 T=4 P=8", flush=True)
+    c = _case(512, 9, 4, 8, seed=5)
+    model = _model(c)
+    out, g_gain, g_L0, g_disp = _device_grads(model, c)
+    gain = torch.tensor(c["gain"], requires_grad=True)
+    L0 = torch.tensor(c["L0"], requires_grad=True)
+    disp = torch.tensor(c["times"][:, None, None] * c["wind"][None], requires_grad=True)
+    ref = R.frames(torch.tensor(c["noise"]), torch.tensor(c["sh_noise"]), gain, L0, disp,
+                   c["dx"], c["P"], model.scale)
+    (ref * torch.tensor(c["G"])).sum().backward()
+    assert O.rel_rms(out, ref.detach().numpy()) < 1e-5
+    errs = (_relnorm(g_gain, gain.grad.numpy()), _relnorm(g_L0, L0.grad.numpy()),
+            _relnorm(g_disp, disp.grad.numpy()))
+    print("rel err gain/L0/disp", errs)
+    assert max(errs) < GRAD_REL, errs
+
+
+def test_parameter_chain_matches_torch_autograd(hip):
+    """Softmax fractions, r0, shared L0 and winds through ProfileFrames.frames."""
+    import torch
+
+    import torch_reference as R
+
+    c = _case(64, 9, 5, 3, seed=21)
+    model = _model(c)
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(3)
+    logits0 = rng.standard_normal(9)
+
+    def params(device):
+        z = torch.tensor(logits0, device=device, requires_grad=True)
+        r0 = torch.tensor(0.15, dtype=torch.float64, device=device, requires_grad=True)
+        L0 = torch.tensor(25.0, dtype=torch.float64, device=device, requires_grad=True)
+        w = torch.tensor(c["wind"], device=device, requires_grad=True)
+        return z, r0, L0, w
+
+    z, r0, L0, w = params(dev)
+    out = model.frames(torch.softmax(z, 0), r0, L0, w, c["times"])
+    (out.double() * torch.tensor(c["G"], device=dev)).sum().backward()
+    zr, r0r, L0r, wr = params("cpu")
+    gain = r0r.pow(-5.0 / 6.0) * torch.softmax(zr, 0).sqrt()
+    disp = torch.tensor(c["times"])[:, None, None] * wr[None]
+    ref = R.frames(torch.tensor(c["noise"]), torch.tensor(c["sh_noise"]), gain,
+                   L0r.expand(9), disp, c["dx"], c["P"], model.scale)
+    (ref * torch.tensor(c["G"])).sum().backward()
+    for a, b in ((z, zr), (r0, r0r), (L0, L0r), (w, wr)):
+        err = _relnorm(np.atleast_1d(a.grad.cpu().numpy()), np.atleast_1d(b.grad.numpy()))
+        assert err < GRAD_REL, (err, a.grad, b.grad)
+
+
+def test_adam_fit_recovers_hidden_profile_direction(hip):
+    """A short Adam fit of fractions and winds to frames of a hidden profile
+    (seed 5) under fixed noise lowers the loss by an order of magnitude."""
+    import torch
+
+    from pyturb_amd.fit import ProfileFrames
+
+    print("adam fit n=64", flush=True)
+    L, n = 4, 64
+    rng = np.random.default_rng(5)
+    hidden_frac = rng.dirichlet(np.ones(L))
+    hidden_wind = rng.uniform(-15.0, 15.0, (L, 2))
+    model = ProfileFrames(n, 8.0 / n, L, subharmonics=3, seed=5)
+    times = np.arange(20) * 2e-3
+    with torch.no_grad():
+        target = model.frames(torch.tensor(hidden_frac), 0.15, 25.0, torch.tensor(hidden_wind),
+                              times)
+    dev = torch.device("cuda")
+    z = torch.tensor(np.log(hidden_frac) + 0.3 * rng.standard_normal(L), device=dev,
+                     requires_grad=True)
+    w = torch.tensor(hidden_wind + rng.normal(0.0, 0.5, (L, 2)), device=dev, requires_grad=True)
+    opt = torch.optim.Adam([z, w], lr=0.05)
+    losses = []
+    for _ in range(60):
+        opt.zero_grad()
+        out = model.frames(torch.softmax(z, 0), 0.15, 25.0, w, times)
+        loss = ((out - target) ** 2).mean() / (target ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+    print("loss first/last", losses[0], losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
