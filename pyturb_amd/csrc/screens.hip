@@ -86,6 +86,18 @@ int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, i
 // ----------------------------------------------------------------------------
 // plain 2-D transform passes (pt_fft2)
 // ----------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Column tiles of one grid share 128-byte lines (M * 4 or M * 8 bytes
+// per row per tile), so the flat tile index is remapped to give every XCD a
+// contiguous run of tiles: the partial lines of neighbouring tiles then meet
+// in one L2 instead of being fetched and written back by several.  A
+// bijection on [0, nx) for any nx.
+__device__ __forceinline__ int xcd_tile(int bx, int nx) {
+  const int xcd = bx & 7, k = bx >> 3;
+  const int q = nx >> 3, r = nx & 7;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
+}
+
 template <int N, int S>
 __global__ __launch_bounds__(256) void k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, int n1) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
@@ -108,16 +120,12 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __rest
   const int tid = threadIdx.x;
   init_twiddles<N, S>(tw, tid, T);
   __syncthreads();
-  const size_t base = (size_t)blockIdx.y * N * n1 + (size_t)blockIdx.x * M;
+  const size_t base = (size_t)blockIdx.y * N * n1 + (size_t)xcd_tile(blockIdx.x, gridDim.x) * M;
   auto load = [&](int m, int i) { return in[base + (size_t)i * n1 + m]; };
   auto store = [&](int m, int i, float2 v) { out[base + (size_t)i * n1 + m] = v; };
   wg_fft<N, T, M, true, S>(lds, tw, tid, load, store);
 }
 
-// copy the basis table U (P x N float2) into LDS
-__device__ __forceinline__ void load_basis(float2* Us, const float2* U, int count, int tid, int nthreads) {
-  for (int i = tid; i < count; i += nthreads) Us[i] = U[i];
-}
 
 // ----------------------------------------------------------------------------
 // PhaseScreen.generate
@@ -161,48 +169,66 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
   constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
   __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
   __shared__ float2 tw[N];
-  extern __shared__ __align__(16) float2 Us[];  // [P][N] basis
+  using Sub = ColSub<N, M, 2, T>;
+  __shared__ Sub sub;
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
-  const int c0 = blockIdx.x * M;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * M;
   if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;  // tile entirely outside the crop
   const int P = a.levels;
   const int s0 = b, s1 = b + a.n_fft;
   const bool has1 = s1 < a.count;
+  const int sets[2] = {s0, has1 ? s1 : -1};
   init_twiddles<N, 1>(tw, tid, T);
-  load_basis(Us, a.tab.U, P * N, tid, T);
   __syncthreads();
+  // issued after the barrier: the loads complete under the tile load
+  typename Sub::Regs pre;
+  if (P > 0) pre = Sub::prefetch(a.tab, sets, c0, P, tid);
   const size_t plane = (size_t)N * N;
   const float2* src = a.work + (size_t)b * plane + c0;
-  const size_t oplane = (size_t)a.n_out * a.n_out;
-  const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
-  const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
-  ShRegs r0, r1;
-  r0.y = -1;
+  // The transform leaves each thread column c0 + m and the rows
+  // x0 + row_step(c) * N/8, c < 8, in call order.
+  float v0[8], v1[8];
+  int cnt = 0;
   auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
   auto store = [&](int m, int i, float2 v) {
-    const int x = i - a.off, y = c0 + m - a.off;
-    if ((unsigned)x >= (unsigned)a.n_out || (unsigned)y >= (unsigned)a.n_out) return;
-    float l0 = 0.0f, l1 = 0.0f;
-    if (P > 0) {
-      if (c0 + m != r0.y) {
-        r0.load(a.tab, s0, c0 + m, N, P);
-        if (has1) r1.load(a.tab, s1, c0 + m, N, P);
-      }
-      l0 = r0.eval(Us, i, N, P);
-      if (has1) l1 = r1.eval(Us, i, N, P);
-    }
-    const size_t o = (size_t)x * a.n_out + y;
-    float* d0 = a.out + (size_t)s0 * oplane + o;
-    const float v0 = (v.x + l0) * f0;
-    *d0 = a.accumulate ? *d0 + v0 : v0;
-    if (has1) {
-      float* d1 = a.out + (size_t)s1 * oplane + o;
-      const float v1 = (v.y + l1) * f1;
-      *d1 = a.accumulate ? *d1 + v1 : v1;
-    }
+    v0[cnt] = v.x;
+    v1[cnt] = v.y;
+    ++cnt;
   };
   wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
+  if (P > 0) sub.commit(pre, P, tid);
+  __syncthreads();
+  constexpr int RL = Plan<N>::radix(Plan<N>::NST - 1);
+  const int m = tid % M, x0 = tid / M;
+  const int y = c0 + m - a.off;
+  if ((unsigned)y >= (unsigned)a.n_out) return;  // no barrier follows
+  const size_t oplane = (size_t)a.n_out * a.n_out;
+  float* o0 = a.out + (size_t)s0 * oplane + y;
+  float* o1 = a.out + (size_t)s1 * oplane + y;
+  float prev0[8], prev1[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    prev0[c] = prev1[c] = 0.0f;
+    const int x = x0 + Sub::template row_step<RL>(c) * (N / 8) - a.off;
+    if (a.accumulate && (unsigned)x < (unsigned)a.n_out) {
+      prev0[c] = o0[(size_t)x * a.n_out];
+      if (has1) prev1[c] = o1[(size_t)x * a.n_out];
+    }
+  }
+  if (P > 0) {
+    sub.template add_low<RL>(0, m, x0, P, v0);
+    if (has1) sub.template add_low<RL>(1, m, x0, P, v1);
+  }
+  const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
+  const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = x0 + Sub::template row_step<RL>(c) * (N / 8) - a.off;
+    if ((unsigned)x >= (unsigned)a.n_out) continue;
+    o0[(size_t)x * a.n_out] = prev0[c] + v0[c] * f0;
+    if (has1) o1[(size_t)x * a.n_out] = prev1[c] + v1[c] * f1;
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -269,32 +295,43 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_frame_cols(FrameArgs a) {
   constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
   __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
   __shared__ float2 tw[N];
-  extern __shared__ __align__(16) float2 Us[];
+  using Sub = ColSub<N, M, 1, T>;
+  __shared__ Sub sub;
   const int tid = threadIdx.x;
   const int f = blockIdx.y;
-  const int c0 = blockIdx.x * M;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * M;
   if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;
   const int P = a.levels;
+  const int sets[1] = {f};
   init_twiddles<N, 1>(tw, tid, T);
-  load_basis(Us, a.tab.U, P * N, tid, T);
   __syncthreads();
+  // issued after the barrier: the loads complete under the tile load
+  typename Sub::Regs pre;
+  if (P > 0) pre = Sub::prefetch(a.tab, sets, c0, P, tid);
   const size_t plane = (size_t)N * N;
   const float2* src = a.work + (size_t)f * plane + c0;
-  float* dst = a.out + (size_t)f * a.n_out * a.n_out;
-  ShRegs rg;
-  rg.y = -1;
+  float v0[8];
+  int cnt = 0;
   auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
   auto store = [&](int m, int i, float2 v) {
-    const int x = i - a.off, y = c0 + m - a.off;
-    if ((unsigned)x >= (unsigned)a.n_out || (unsigned)y >= (unsigned)a.n_out) return;
-    float lo = 0.0f;
-    if (P > 0) {
-      if (c0 + m != rg.y) rg.load(a.tab, f, c0 + m, N, P);
-      lo = rg.eval(Us, i, N, P);
-    }
-    dst[(size_t)x * a.n_out + y] = (v.x + lo) * a.scale;
+    v0[cnt] = v.x;
+    ++cnt;
   };
   wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
+  if (P > 0) sub.commit(pre, P, tid);
+  __syncthreads();
+  constexpr int RL = Plan<N>::radix(Plan<N>::NST - 1);
+  const int m = tid % M, x0 = tid / M;
+  const int y = c0 + m - a.off;
+  if ((unsigned)y >= (unsigned)a.n_out) return;  // no barrier follows
+  float* dst = a.out + (size_t)f * a.n_out * a.n_out + y;
+  if (P > 0) sub.template add_low<RL>(0, m, x0, P, v0);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = x0 + Sub::template row_step<RL>(c) * (N / 8) - a.off;
+    if ((unsigned)x >= (unsigned)a.n_out) continue;
+    dst[(size_t)x * a.n_out] = v0[c] * a.scale;
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -319,10 +356,11 @@ static size_t cols_static_lds() {
   return (size_t)Layout<N, ColsCfg<N>::T, ColsCfg<N>::M, true>::SIZE * 8 + (size_t)N * 8;
 }
 
-// Fused column epilogues hold U (levels x N) in LDS and H in registers.
+// Fused column epilogues keep the per-column subharmonic terms in registers
+// (up to PREG levels), staged through a small LDS block (ColSub).
 template <int N>
 static bool cols_fused_ok(int levels) {
-  return levels <= PREG && cols_static_lds<N>() + (size_t)levels * N * 8 <= kLdsMax;
+  return levels <= PREG && cols_static_lds<N>() + sizeof(ColSub<N, ColsCfg<N>::M, 2, ColsCfg<N>::T>) <= kLdsMax;
 }
 
 template <int N>
@@ -343,14 +381,11 @@ template <int N>
 static int generate_pow2(const float* nre, const float* nim, const float* amp, const float* sh_noise,
                          const double* sh_amp, GenArgs a, hipStream_t st) {
   constexpr int MC = ColsCfg<N>::M;
-  const size_t dyn = (size_t)a.levels * N * 8;
-  int rc = set_dyn_lds(k_gen_cols<N>, dyn);
-  if (rc) return rc;
   k_gen_rows<N><<<dim3(N / RowsCfg<N>::M, a.n_fft), RowsCfg<N>::T, 0, st>>>(nre, nim, amp, (float2*)a.work);
   PT_LAUNCH_CHECK();
-  rc = launch_gen_prep(sh_noise, sh_amp, a.count, a.levels, N, a.tab, st);
+  int rc = launch_gen_prep(sh_noise, sh_amp, a.count, a.levels, N, a.tab, st);
   if (rc) return rc;
-  k_gen_cols<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, dyn, st>>>(a);
+  k_gen_cols<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -377,10 +412,7 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
 template <int N>
 static int frames_cols_pow2(int F, FrameArgs a, hipStream_t st) {
   constexpr int MC = ColsCfg<N>::M;
-  const size_t dyn = (size_t)a.levels * N * 8;
-  int rc = set_dyn_lds(k_frame_cols<N>, dyn);
-  if (rc) return rc;
-  k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, dyn, st>>>(a);
+  k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }

@@ -6,8 +6,12 @@
 // (see common.hpp for g and the closed-form grid mean), plus the basis table
 // U[p][x] = e_p(x).  The column-pass epilogue then adds
 //   low(x, y) = K[s][y] + sum_p Re(U[p][x] * H[s][p][y])
-// with H/K in registers (one column per thread) and U in LDS: no per-tile
-// setup, no FP64 in the transform kernels.
+// through ColSub: a thread of a column tile holds one column y and the rows
+// x0 + d*N/8 (d < 8, x0 < N/8), and since the basis is exponential in x,
+//   e_p(x0 + d*N/8) = e_p(x0) * exp(i*pi*d / (4 * 3^(p+1))),
+// so with A_p = e_p(x0) * H_p(y) in registers and the 8-entry step table
+// F[p][d] in LDS each pixel costs two FMAs per level.  No per-tile setup, no
+// FP64 in the transform kernels.
 //
 // Tables live in the caller's workspace right after the FFT intermediate,
 // in c64 (float2) units:  U: P*n,  H: S*P*n,  K: S*n floats.
@@ -16,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "fft_core.hpp"
 
 namespace pt {
 
@@ -97,30 +102,97 @@ __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* di
   __syncthreads();
 }
 
-// One column's K and H in registers.
-struct ShRegs {
-  int y;
-  float k;
-  float2 h[PREG];
-  __device__ __forceinline__ void load(const SubTables& t, int s, int yy, int n, int P) {
-    y = yy;
-    k = t.K[(size_t)s * n + yy];
-#pragma unroll
-    for (int p = 0; p < PREG; ++p)
-      if (p < P) h[p] = t.H[((size_t)s * P + p) * n + yy];
+// Subharmonic terms of one column tile, staged through LDS: the row-step
+// table F, the basis at the N/8 base rows, and H, K of the tile's M columns
+// for up to S coefficient sets (screens / frames).  prefetch() issues the
+// global loads into registers before the transform (they complete under
+// the tile load and the FFT stages); commit() writes them to LDS after the
+// transform, followed by one barrier, so the epilogue never waits on HBM.
+template <int N, int M, int S, int T>
+struct ColSub {
+  static constexpr int EMAX = PREG * (N / 8) + S * PREG * M + S * M;
+  static constexpr int R = (EMAX + T - 1) / T;  // prefetched values per thread
+  float2 F[PREG][8];
+  float2 Ux[PREG][N / 8];
+  float2 H[S][PREG][M];
+  float K[S][M];
+
+  struct Regs {
+    float2 v[R];
+  };
+
+  // element q of the packed list [U: P*N/8 | H: S*P*M | K: S*M]
+  __device__ __forceinline__ static float2 fetch(const SubTables& t, const int (&set)[S], int c0, int P, int q) {
+    const int eu = P * (N / 8), eh = S * P * M;
+    if (q < eu) return t.U[(size_t)(q / (N / 8)) * N + q % (N / 8)];
+    q -= eu;
+    if (q < eh) {
+      const int s = q / (P * M), r = q % (P * M), p = r / M, m = r % M;
+      return set[s] >= 0 ? t.H[((size_t)set[s] * P + p) * N + c0 + m] : make_float2(0.f, 0.f);
+    }
+    q -= eh;
+    if (q < S * M) {
+      const int s = q / M, m = q % M;
+      return make_float2(set[s] >= 0 ? t.K[(size_t)set[s] * N + c0 + m] : 0.f, 0.f);
+    }
+    return make_float2(0.f, 0.f);
   }
-  // U: basis table with row stride n (LDS or global)
-  __device__ __forceinline__ float eval(const float2* U, int x, int n, int P) const {
-    float acc = k;
+
+  __device__ __forceinline__ static Regs prefetch(const SubTables& t, const int (&set)[S], int c0, int P, int tid) {
+    Regs g;
 #pragma unroll
-    for (int p = 0; p < PREG; ++p) {
-      if (p < P) {
-        const float2 e = U[p * n + x];
-        acc += e.x * h[p].x - e.y * h[p].y;
+    for (int r = 0; r < R; ++r) g.v[r] = fetch(t, set, c0, P, tid + r * T);
+    return g;
+  }
+
+  __device__ __forceinline__ void commit(const Regs& g, int P, int tid) {
+    const int eu = P * (N / 8), eh = S * P * M;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      int q = tid + r * T;
+      if (q < eu) {
+        Ux[q / (N / 8)][q % (N / 8)] = g.v[r];
+        continue;
+      }
+      q -= eu;
+      if (q < eh) {
+        const int s = q / (P * M), rr = q % (P * M);
+        H[s][rr / M][rr % M] = g.v[r];
+        continue;
+      }
+      q -= eh;
+      if (q < S * M) K[q / M][q % M] = g.v[r].x;
+    }
+    for (int q = tid; q < P * 8; q += T) {
+      const int p = q >> 3, d = q & 7;
+      float sn, cs;
+      sincospif((float)((double)d / (4.0 * pow3(p + 1))), &sn, &cs);
+      F[p][d] = make_float2(cs, sn);
+    }
+  }
+
+  // Adds low(x0 + d_c * N/8, y) to v[c] for the eight rows of a thread
+  // (column m of set s); d_c is the row step of the c-th value the
+  // transform hands out (see row_step).  Levels loop at run time, so only
+  // the eight sums stay live.
+  template <int RL>
+  __device__ __forceinline__ void add_low(int s, int m, int x0, int P, float (&v)[8]) const {
+    const float k = K[s][m];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] += k;
+    for (int p = 0; p < P; ++p) {
+      const float2 a = cmul(Ux[p][x0], H[s][p][m]);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float2 f = F[p][row_step<RL>(c)];
+        v[c] += f.x * a.x - f.y * a.y;
       }
     }
-    return acc;
   }
+  // c-th output of a thread in the last Stockham stage (radix RL): stage
+  // block q = c / RL, butterfly output r = c % RL, row x0 + (q + r * 8/RL) * N/8.
+  template <int RL>
+  __device__ static constexpr int row_step(int c) { return c / RL + (c % RL) * (8 / RL); }
 };
 
 // Unbounded-level evaluation straight from the tables (general-size path).
