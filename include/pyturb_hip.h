@@ -39,12 +39,19 @@ int pt_abi_version(void);
 
 /*
  * Workspace (in c64 elements) that pt_generate (kind 0, sets = count) or
- * pt_spectral_frames / pt_spectral_rows+cols (kind 1, sets = frames) needs
- * for an n x n grid with `levels` subharmonic levels: the FFT intermediate
- * (doubled on the general-size path) followed by the subharmonic tables.
- * -1 on bad arguments.  Host-only, no device work.
+ * pt_spectral_frames / pt_spectral_rows+cols (kind 1, sets = frames, any
+ * layer count) needs for an n x n grid with `levels` subharmonic levels:
+ * the FFT intermediate (doubled on the general-size path) followed by the
+ * subharmonic tables.  -1 on bad arguments.  Host-only, no device work.
  */
 long long pt_work_elems(int kind, int n, int sets, int levels);
+
+/*
+ * Exact workspace (c64 elements) of pt_spectral_frames, or of
+ * pt_spectral_rows + pt_spectral_cols, for `layers` layers.  pt_work_elems
+ * kind 1 is an upper bound of this for any layer count.  -1 on bad input.
+ */
+long long pt_frames_work_elems(int n, int frames, int layers, int levels);
 
 /*
  * Unnormalised 2-D DFT over the last two axes of `in` (batch, n0, n1) c64,
@@ -97,7 +104,8 @@ int pt_generate(const float* noise_re, const float* noise_im,
  *   out       : (frames, n_out, n_out) f32,
  *               out = scale * Re(ifft2(sum_l S_l P_x P_y) * n^2 + low)
  *               cropped at crop_offset
- *   work      : pt_work_elems(1, n, frames, levels) c64
+ *   work      : pt_frames_work_elems(n, frames, layers, levels) c64 (or the
+ *               layer-independent bound pt_work_elems(1, n, frames, levels))
  * Shift-theorem phases are reduced exactly (32-bit fixed-point cycles), so
  * large travel keeps full FP32 phase accuracy.
  */
@@ -108,12 +116,19 @@ int pt_spectral_frames(const void* spectra, int layers, int n,
                        void* stream);
 
 /*
- * pt_spectral_frames split into its two passes (power-of-two n, <= 12
- * subharmonic levels), for callers that schedule or time them separately:
- *   pass A: work[f] = row-transformed phasor-weighted layer sum of frame f
- *   pass B: subharmonic tables (one block per frame), then column transform
- *           + subharmonics + crop + scale into out
- * pt_spectral_rows then pt_spectral_cols on one stream == pt_spectral_frames.
+ * pt_spectral_frames split into its two passes (power-of-two n, <= 16
+ * layers, <= 12 subharmonic levels), for callers that schedule or time them
+ * separately.  Frames are transformed in pairs: Re(ifft2(X)) equals the
+ * transform of the Hermitian part of X, so frames 2p and 2p+1 share one
+ * complex transform of Herm(X_2p) + i Herm(X_2p+1).
+ *   pass A: fold the layer stack into its Hermitian part (plus an exact
+ *           Nyquist-line correction table), then work[p] = row-transformed
+ *           packed sum of frame pair p
+ *   pass B: subharmonic tables (one block per frame), then the column
+ *           transform of each pair; real part -> frame 2p, imaginary part ->
+ *           frame 2p+1, + subharmonics + crop + scale into out
+ * pt_spectral_rows then pt_spectral_cols on one stream == pt_spectral_frames;
+ * the workspace is pt_frames_work_elems(n, frames, layers, levels).
  */
 int pt_spectral_rows(const void* spectra, int layers, int n,
                      double pixel_scale, const double* disp, int frames,

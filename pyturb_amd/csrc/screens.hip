@@ -155,17 +155,21 @@ __global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre,
   wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
 }
 
+// Column pass with a two-output epilogue.  Transform b yields the sets
+// s0 = b * s_mul (real part) and s1 = s0 + s_add (imaginary part, when
+// s1 < count): generate pairs screen b with screen b + n_fft (s_mul 1,
+// s_add n_fft); the frame engine pairs frames 2b and 2b + 1 (2, 1).
 struct GenArgs {
   const float2* work;
   float* out;
   const float* screen_scale;
   SubTables tab;
-  int count, n_fft, levels, off, n_out, accumulate;
+  int count, n_fft, levels, off, n_out, accumulate, s_mul, s_add;
   float scale;
 };
 
-template <int N>
-__global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
+template <int N, bool ACC>
+__device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
   constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
   __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
   __shared__ float2 tw[N];
@@ -176,7 +180,7 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
   const int c0 = xcd_tile(blockIdx.x, gridDim.x) * M;
   if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;  // tile entirely outside the crop
   const int P = a.levels;
-  const int s0 = b, s1 = b + a.n_fft;
+  const int s0 = b * a.s_mul, s1 = s0 + a.s_add;
   const bool has1 = s1 < a.count;
   const int sets[2] = {s0, has1 ? s1 : -1};
   init_twiddles<N, 1>(tw, tid, T);
@@ -211,7 +215,7 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
   for (int c = 0; c < 8; ++c) {
     prev0[c] = prev1[c] = 0.0f;
     const int x = x0 + Sub::template row_step<RL>(c) * (N / 8) - a.off;
-    if (a.accumulate && (unsigned)x < (unsigned)a.n_out) {
+    if (ACC && (unsigned)x < (unsigned)a.n_out) {
       prev0[c] = o0[(size_t)x * a.n_out];
       if (has1) prev1[c] = o1[(size_t)x * a.n_out];
     }
@@ -234,104 +238,165 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols(GenArgs a) {
 // ----------------------------------------------------------------------------
 // spectral frozen-flow frames
 // ----------------------------------------------------------------------------
+// Writing column pass: the epilogue fits 64 VGPRs, so two 1024-thread
+// workgroups share a CU where LDS allows (N <= 512).
 template <int N>
-__global__ __launch_bounds__(256) void k_frame_rows(const float2* __restrict__ spectra, int L,
-                                                    const double* __restrict__ disp, double inv_period,
-                                                    float2* __restrict__ work) {
-  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
+__global__ __launch_bounds__(ColsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(8))) void k_gen_cols_set(GenArgs a) {
+  gen_cols_body<N, false>(a);
+}
+// Accumulating column pass (sums L0 groups into out).
+template <int N>
+__global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols_acc(GenArgs a) {
+  gen_cols_body<N, true>(a);
+}
+
+// The frame engine transforms two frames at once.  A frame is
+// Re(IFFT(X)) with X = sum_l S_l P_l, and Re(IFFT(X)) = IFFT(Herm X) with
+// Herm X(k) = (X(k) + conj X(-k)) / 2, which is Hermitian, so
+//   IFFT(Herm X_t1 + i Herm X_t2) = frame_t1 + i frame_t2.
+// For a real displacement P_l(-k) = conj P_l(k) except on the Nyquist row
+// and column, where the signed frequency -N/2 maps to itself; there
+//   conj P(-k) = P(k) * E(k),  E = exp(2 pi i ([k0 = N/2] s_0 + [k1 = N/2] s_1) / dx).
+// So with A_l = S_l / 2 and B_l(k) = conj(S_l(-k)) / 2 (folded once per
+// call by k_frame_fold):
+//   Herm X(k) = sum_l P_l(k) (A_l + B_l)            off the Nyquist lines,
+//             = sum_l P_l(k) (A_l + B_l E_l(k))       on them.
+// fold[l] holds A_l + B_l (A_l on the Nyquist lines); side[l][j] holds B_l
+// on the Nyquist row (j = k1) and column (j = N + k0).
+__global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ spectra, int n,
+                                                    float2* __restrict__ fold, float2* __restrict__ side) {
+  const size_t plane = (size_t)n * n;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= plane) return;
+  const int l = blockIdx.y;
+  const int k0 = (int)(idx / n), k1 = (int)(idx % n);
+  const int m0 = (n - k0) & (n - 1), m1 = (n - k1) & (n - 1);
+  const float2 sv = spectra[(size_t)l * plane + idx];
+  const float2 sm = spectra[(size_t)l * plane + (size_t)m0 * n + m1];
+  const float2 A = make_float2(0.5f * sv.x, 0.5f * sv.y), B = make_float2(0.5f * sm.x, -0.5f * sm.y);
+  const int h = n / 2;
+  if (k0 == h || k1 == h) {
+    fold[(size_t)l * plane + idx] = A;
+    side[(size_t)l * 2 * n + (k0 == h ? k1 : n + k0)] = B;
+  } else {
+    fold[(size_t)l * plane + idx] = make_float2(A.x + B.x, A.y + B.y);
+  }
+}
+
+// Pass A of a frame pair: the Hermitian layer sums of frames 2p and 2p + 1
+// packed as Z = Herm X_t1 + i Herm X_t2, then the row transform.  Shift
+// phasors use the exact fixed-point phases; the column factor is split as
+// exp(2 pi i (8a + b - N/2) u) = t8[a] * t1[b], (k1 + N/2) mod N = 8a + b,
+// so the per-layer LDS tables are N/8 + 8 entries instead of N.
+template <int N>
+__host__ __device__ constexpr size_t pair_rows_lds(int L) {
+  return (size_t)2 * L * (N / 8 + 8 + RowsCfg<N>::M + 2) * 8 + (size_t)2 * L * 2 * 4;
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restrict__ fold,
+                                                         const float2* __restrict__ side, int L,
+                                                         const double* __restrict__ disp, int F,
+                                                         double inv_period, float2* __restrict__ work) {
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8;
+  static_assert(T == M * NB, "one row position per thread in the first stage");
   __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
   extern __shared__ __align__(16) unsigned char dyn[];
-  float2* py = (float2*)dyn;          // [L][N]
-  float2* px = py + (size_t)L * N;    // [L][M]
-  uint32_t* UX = (uint32_t*)(px + (size_t)L * M);
-  uint32_t* UY = UX + L;
+  float2* t8 = (float2*)dyn;              // [2][L][NB]
+  float2* t1 = t8 + 2 * L * NB;           // [2][L][8]
+  float2* px = t1 + 2 * L * 8;            // [2][L][M]
+  float2* ex = px + 2 * L * M;            // [2][L]
+  float2* ey = ex + 2 * L;                // [2][L]
+  uint32_t* U = (uint32_t*)(ey + 2 * L);  // [2][L][2]
   const int tid = threadIdx.x;
-  const int f = blockIdx.y;
+  const int pr = blockIdx.y;
   const int row0 = blockIdx.x * M;
-  for (int l = tid; l < L; l += T) {
-    UX[l] = cycles_fixed(disp[((size_t)f * L + l) * 2 + 0] * inv_period);
-    UY[l] = cycles_fixed(disp[((size_t)f * L + l) * 2 + 1] * inv_period);
+  const bool two = 2 * pr + 1 < F;
+  for (int q = tid; q < 4 * L; q += T) {
+    const int t = 2 * pr + q / (2 * L);
+    const int l = (q % (2 * L)) >> 1, ax = q & 1;
+    U[q] = t < F ? cycles_fixed(disp[((size_t)t * L + l) * 2 + ax] * inv_period) : 0u;
   }
   init_twiddles<N, 1>(tw, tid, T);
   __syncthreads();
-  for (int idx = tid; idx < L * N; idx += T) {
-    const int l = idx / N, j = idx % N;
-    py[idx] = phasor_fixed((uint32_t)signed_freq(j, N) * UY[l]);
+  for (int q = tid; q < 2 * L * NB; q += T) t8[q] = phasor_fixed((uint32_t)(8 * (q % NB) - N / 2) * U[(q / NB) * 2 + 1]);
+  for (int q = tid; q < 2 * L * 8; q += T) t1[q] = phasor_fixed((uint32_t)(q & 7) * U[(q >> 3) * 2 + 1]);
+  for (int q = tid; q < 2 * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[(q / M) * 2]);
+  for (int q = tid; q < 2 * L; q += T) {
+    ex[q] = phasor_fixed((uint32_t)N * U[q * 2]);
+    ey[q] = phasor_fixed((uint32_t)N * U[q * 2 + 1]);
   }
-  for (int idx = tid; idx < L * M; idx += T) {
-    const int l = idx / M, m = idx % M;
-    px[idx] = phasor_fixed((uint32_t)signed_freq(row0 + m, N) * UX[l]);
-  }
   __syncthreads();
+  // first-stage positions of this thread: row m, columns jq + r * NB (r < 8)
+  const int m = tid / NB, jq = tid % NB;
+  const int k0 = row0 + m;
   const size_t plane = (size_t)N * N;
-  const float2* sp = spectra + (size_t)row0 * N;
-  auto load = [&](int m, int i) {
-    float2 acc = make_float2(0.0f, 0.0f);
-    for (int l = 0; l < L; ++l) {
-      const float2 s = sp[(size_t)l * plane + (size_t)m * N + i];
-      const float2 ph = cmul(px[l * M + m], py[l * N + i]);
-      const float2 t = cmul(s, ph);
-      acc.x += t.x;
-      acc.y += t.y;
-    }
-    return acc;
-  };
-  float2* out = work + (size_t)f * plane + (size_t)row0 * N;
-  auto store = [&](int m, int i, float2 v) { out[(size_t)m * N + i] = v; };
-  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
-}
-
-struct FrameArgs {
-  const float2* work;
-  float* out;
-  SubTables tab;
-  int levels, off, n_out;
-  float scale;
-};
-
-template <int N>
-__global__ __launch_bounds__(ColsCfg<N>::T) void k_frame_cols(FrameArgs a) {
-  constexpr int T = ColsCfg<N>::T, M = ColsCfg<N>::M;
-  __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
-  __shared__ float2 tw[N];
-  using Sub = ColSub<N, M, 1, T>;
-  __shared__ Sub sub;
-  const int tid = threadIdx.x;
-  const int f = blockIdx.y;
-  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * M;
-  if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;
-  const int P = a.levels;
-  const int sets[1] = {f};
-  init_twiddles<N, 1>(tw, tid, T);
-  __syncthreads();
-  // issued after the barrier: the loads complete under the tile load
-  typename Sub::Regs pre;
-  if (P > 0) pre = Sub::prefetch(a.tab, sets, c0, P, tid);
-  const size_t plane = (size_t)N * N;
-  const float2* src = a.work + (size_t)f * plane + c0;
-  float v0[8];
-  int cnt = 0;
-  auto load = [&](int m, int i) { return src[(size_t)i * N + m]; };
-  auto store = [&](int m, int i, float2 v) {
-    v0[cnt] = v.x;
-    ++cnt;
-  };
-  wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
-  if (P > 0) sub.commit(pre, P, tid);
-  __syncthreads();
-  constexpr int RL = Plan<N>::radix(Plan<N>::NST - 1);
-  const int m = tid % M, x0 = tid / M;
-  const int y = c0 + m - a.off;
-  if ((unsigned)y >= (unsigned)a.n_out) return;  // no barrier follows
-  float* dst = a.out + (size_t)f * a.n_out * a.n_out + y;
-  if (P > 0) sub.template add_low<RL>(0, m, x0, P, v0);
+  const float2* fp = fold + (size_t)k0 * N + jq;
+  float2 z0[8], z1[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int x = x0 + Sub::template row_step<RL>(c) * (N / 8) - a.off;
-    if ((unsigned)x >= (unsigned)a.n_out) continue;
-    dst[(size_t)x * a.n_out] = v0[c] * a.scale;
+  for (int r = 0; r < 8; ++r) z0[r] = z1[r] = make_float2(0.f, 0.f);
+  // For N >= 64 the eight columns jq + r*N/8 share (k1 + N/2) mod 8, so the
+  // row phasor times t1 is one product per layer and frame.
+  constexpr bool BSAME = NB % 8 == 0;
+  for (int l = 0; l < L; ++l) {
+    const float2 pa = px[l * M + m], pb = px[(L + l) * M + m];
+    const int b0 = (jq + N / 2) & 7;
+    const float2 ca0 = cmul(pa, t1[l * 8 + b0]), cb0 = cmul(pb, t1[(L + l) * 8 + b0]);
+    float2 g[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) g[r] = fp[(size_t)l * plane + r * NB];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int qq = (jq + r * NB + N / 2) & (N - 1);
+      const float2 ca = BSAME ? ca0 : cmul(pa, t1[l * 8 + (qq & 7)]);
+      const float2 cb = BSAME ? cb0 : cmul(pb, t1[(L + l) * 8 + (qq & 7)]);
+      const float2 p0 = cmul(ca, t8[l * NB + (qq >> 3)]);
+      const float2 p1 = cmul(cb, t8[(L + l) * NB + (qq >> 3)]);
+      const float2 u0 = cmul(g[r], p0), u1 = cmul(g[r], p1);
+      z0[r].x += u0.x;
+      z0[r].y += u0.y;
+      z1[r].x += u1.x;
+      z1[r].y += u1.y;
+    }
   }
+  if (k0 == N / 2 || (N / 2) % NB == jq) {  // Nyquist row, or the thread holding column N/2
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int k1 = jq + r * NB;
+      if (k0 != N / 2 && k1 != N / 2) continue;
+      const int j = k0 == N / 2 ? k1 : N + k0;
+      const int qq = (k1 + N / 2) & (N - 1);
+      for (int l = 0; l < L; ++l) {
+        float2 e0 = make_float2(1.f, 0.f), e1 = e0;
+        if (k0 == N / 2) {
+          e0 = ex[l];
+          e1 = ex[L + l];
+        }
+        if (k1 == N / 2) {
+          e0 = cmul(e0, ey[l]);
+          e1 = cmul(e1, ey[L + l]);
+        }
+        const float2 p0 = cmul(cmul(px[l * M + m], t1[l * 8 + (qq & 7)]), t8[l * NB + (qq >> 3)]);
+        const float2 p1 = cmul(cmul(px[(L + l) * M + m], t1[(L + l) * 8 + (qq & 7)]), t8[(L + l) * NB + (qq >> 3)]);
+        const float2 bb = side[(size_t)l * 2 * N + j];
+        const float2 u0 = cmul(bb, cmul(p0, e0)), u1 = cmul(bb, cmul(p1, e1));
+        z0[r].x += u0.x;
+        z0[r].y += u0.y;
+        z1[r].x += u1.x;
+        z1[r].y += u1.y;
+      }
+    }
+  }
+  int cnt = 0;
+  auto load = [&](int, int) {
+    const float2 a0 = z0[cnt], a1 = two ? z1[cnt] : make_float2(0.f, 0.f);
+    ++cnt;
+    return make_float2(a0.x - a1.y, a0.y + a1.x);
+  };
+  float2* out = work + (size_t)pr * plane + (size_t)row0 * N;
+  auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
+  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
 }
 
 // ----------------------------------------------------------------------------
@@ -385,34 +450,59 @@ static int generate_pow2(const float* nre, const float* nim, const float* amp, c
   PT_LAUNCH_CHECK();
   int rc = launch_gen_prep(sh_noise, sh_amp, a.count, a.levels, N, a.tab, st);
   if (rc) return rc;
-  k_gen_cols<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
+  if (a.accumulate)
+    k_gen_cols_acc<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
+  else
+    k_gen_cols_set<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
 
-template <int N>
-static size_t frames_rows_lds(int L) {
-  return (size_t)L * N * 8 + (size_t)L * RowsCfg<N>::M * 8 + (size_t)2 * L * 4;
+constexpr int FOLD_LMAX = 16;  // layers of the fused (paired) frame path
+
+// Workspace of the fused frame path (c64 elements): the paired intermediate,
+// the folded stack, its Nyquist side table, then the subharmonic tables.
+struct FramesLayout {
+  size_t fold, side, tab, total;
+};
+static inline size_t r32(size_t v) { return (v + 31) & ~(size_t)31; }
+static FramesLayout frames_layout(int n, int F, int L, int P) {
+  FramesLayout w;
+  const size_t plane = (size_t)n * n;
+  w.fold = r32((size_t)((F + 1) / 2) * plane);
+  w.side = w.fold + r32((size_t)L * plane);
+  w.tab = w.side + r32((size_t)L * 2 * n);
+  w.total = w.tab + sub_table_c64(P, n, F);
+  return w;
 }
 
 template <int N>
 static int frames_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
                             float2* work, hipStream_t st) {
   constexpr int MR = RowsCfg<N>::M;
-  const size_t dyn_r = frames_rows_lds<N>(L);
+  const size_t dyn_r = pair_rows_lds<N>(L);
   if (dyn_r + (size_t)Layout<N, RowsCfg<N>::T, MR, false>::SIZE * 8 + (size_t)N * 8 > kLdsMax)
-    return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor table at n=%d", L, N);
-  int rc = set_dyn_lds(k_frame_rows<N>, dyn_r);
+    return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor tables at n=%d", L, N);
+  int rc = set_dyn_lds(k_frame_pair_rows<N>, dyn_r);
   if (rc) return rc;
-  k_frame_rows<N><<<dim3(N / MR, F), RowsCfg<N>::T, dyn_r, st>>>(spectra, L, disp, inv_period, work);
+  const FramesLayout w = frames_layout(N, F, L, 0);
+  float2* fold = work + w.fold;
+  float2* side = work + w.side;
+  k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
+  PT_LAUNCH_CHECK();
+  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), RowsCfg<N>::T, dyn_r, st>>>(fold, side, L, disp, F,
+                                                                                inv_period, work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
 
 template <int N>
-static int frames_cols_pow2(int F, FrameArgs a, hipStream_t st) {
+static int frames_cols_pow2(GenArgs a, hipStream_t st) {
   constexpr int MC = ColsCfg<N>::M;
-  k_frame_cols<N><<<dim3(N / MC, F), ColsCfg<N>::T, 0, st>>>(a);
+  if (a.accumulate)
+    k_gen_cols_acc<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
+  else
+    k_gen_cols_set<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -449,11 +539,29 @@ int pt_frames_generic(const void* spectra, int L, int n, double pixel_scale, con
                       const double* coeffs, int levels, int off, int n_out, float scale, float* out, void* work,
                       void* stream);
 
+static bool frames_fused(int n, int layers, int levels) {
+  return is_pow2_fast(n) && fused_ok(n, levels) && layers <= FOLD_LMAX;
+}
+
+static size_t frames_generic_elems(int n, int F, int P) { return 2 * (size_t)F * n * n + sub_table_c64(P, n, F); }
+
+extern "C" long long pt_frames_work_elems(int n, int frames, int layers, int levels) {
+  if (n < 1 || frames < 0 || layers < 1 || levels < 0) return -1;
+  return (long long)(frames_fused(n, layers, levels) ? frames_layout(n, frames, layers, levels).total
+                                                     : frames_generic_elems(n, frames, levels));
+}
+
 extern "C" long long pt_work_elems(int kind, int n, int sets, int levels) {
-  // kind 0: generate (sets = screens), 1: spectral frames (sets = frames)
+  // kind 0: generate (sets = screens), 1: spectral frames (sets = frames, any layer count)
   if (n < 1 || sets < 0 || levels < 0) return -1;
   const bool fused = is_pow2_fast(n) && fused_ok(n, levels);
-  const size_t nfft = kind == 0 ? (size_t)(sets + 1) / 2 : (size_t)sets;
+  if (kind == 1) {
+    const size_t g = frames_generic_elems(n, sets, levels);
+    if (!fused) return (long long)g;
+    const size_t f = frames_layout(n, sets, FOLD_LMAX, levels).total;
+    return (long long)(f > g ? f : g);
+  }
+  const size_t nfft = (size_t)(sets + 1) / 2;
   return (long long)((fused ? 1 : 2) * nfft * n * n + sub_table_c64(levels, n, sets));
 }
 
@@ -496,6 +604,8 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
     a.off = crop_offset;
     a.n_out = n_out;
     a.accumulate = accumulate ? 1 : 0;
+    a.s_mul = 1;
+    a.s_add = n_fft;
     a.scale = scale;
 #define CALL(N) return generate_pow2<N>(noise_re, noise_im, amplitude, sh_noise, sh_amp, a, st);
     PT_POW2_SWITCH(n, CALL)
@@ -506,8 +616,9 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
 }
 
 // The two passes of pt_spectral_frames as separate calls (power-of-two n,
-// fused epilogue): pass A writes the row-transformed layer sum to `work`;
-// pass B builds the subharmonic tables and finishes the column transform.
+// fused epilogue, <= 16 layers): pass A folds the layer stack and writes the
+// row-transformed Hermitian sums of frame pairs to `work`; pass B builds the
+// subharmonic tables and finishes the column transforms of the pairs.
 static int frames_check(int layers, int n, double pixel_scale, const double* disp, int frames, const double* sh_coeffs,
                         int levels, int crop_offset, int n_out, void* work) {
   if (!disp || !work) return fail(PT_ERR_ARG, "spectral frames: NULL array");
@@ -525,7 +636,8 @@ extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double p
   if (!spectra) return fail(PT_ERR_ARG, "pt_spectral_rows: NULL spectra");
   int rc = frames_check(layers, n, pixel_scale, disp, frames, nullptr, 0, 0, n, work);
   if (rc || frames == 0) return rc;
-  if (!is_pow2_fast(n)) return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: power-of-two n 32..2048 only");
+  if (!is_pow2_fast(n) || layers > FOLD_LMAX)
+    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: power-of-two n 32..2048 and <= %d layers only", FOLD_LMAX);
   const double inv_period = 1.0 / ((double)n * pixel_scale);
   hipStream_t st = (hipStream_t)stream;
 #define CALL(N) return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
@@ -540,21 +652,28 @@ extern "C" int pt_spectral_cols(void* work, int layers, int n, double pixel_scal
   if (!out) return fail(PT_ERR_ARG, "pt_spectral_cols: NULL out");
   int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
   if (rc || frames == 0) return rc;
-  if (!is_pow2_fast(n) || !fused_ok(n, levels))
-    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: power-of-two n and <= %d levels only", PREG);
+  if (!frames_fused(n, layers, levels))
+    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: power-of-two n, <= %d layers and <= %d levels only",
+                FOLD_LMAX, PREG);
   hipStream_t st = (hipStream_t)stream;
-  FrameArgs a;
+  GenArgs a;
   a.work = (const float2*)work;
   a.out = out;
-  a.tab = sub_tables((float2*)work + (size_t)frames * n * n, levels, n, frames);
+  a.screen_scale = nullptr;
+  a.tab = sub_tables((float2*)work + frames_layout(n, frames, layers, levels).tab, levels, n, frames);
+  a.count = frames;
+  a.n_fft = (frames + 1) / 2;
   a.levels = levels;
   a.off = crop_offset;
   a.n_out = n_out;
+  a.accumulate = 0;
+  a.s_mul = 2;
+  a.s_add = 1;
   a.scale = scale;
   rc = launch_frame_prep(disp, (const double2*)sh_coeffs, frames, layers, levels, n, 1.0 / ((double)n * pixel_scale),
                          a.tab, st);
   if (rc) return rc;
-#define CALL(N) return frames_cols_pow2<N>(frames, a, st);
+#define CALL(N) return frames_cols_pow2<N>(a, st);
   PT_POW2_SWITCH(n, CALL)
 #undef CALL
   return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: n=%d", n);
@@ -566,7 +685,7 @@ extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double
   if (!spectra || !out) return fail(PT_ERR_ARG, "pt_spectral_frames: NULL array");
   int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
   if (rc || frames == 0) return rc;
-  if (is_pow2_fast(n) && fused_ok(n, levels)) {
+  if (frames_fused(n, layers, levels)) {
     rc = pt_spectral_rows(spectra, layers, n, pixel_scale, disp, frames, work, stream);
     if (rc) return rc;
     return pt_spectral_cols(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, scale,

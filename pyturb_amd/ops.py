@@ -32,6 +32,14 @@ def work_elems(kind: int, n: int, sets: int, levels: int) -> int:
     return v
 
 
+def frames_work_elems(n: int, frames: int, layers: int, levels: int) -> int:
+    """Exact workspace of spectral_frames in complex64 elements."""
+    v = int(_hip.lib().pt_frames_work_elems(int(n), int(frames), int(layers), int(levels)))
+    if v < 0:
+        raise ValueError("pt_frames_work_elems: bad arguments")
+    return v
+
+
 def _require(t, dtype, shape, name):
     if tuple(t.shape) != tuple(shape):
         raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
@@ -123,7 +131,7 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
     if out is None:
         out = t.empty((F, n_out, n_out), dtype=t.float32, device=spectra.device)
     _require(out, t.float32, (F, n_out, n_out), "out")
-    need = work_elems(1, n, F, P)
+    need = frames_work_elems(n, F, L, P)
     if work is None or work.numel() < need:
         work = t.empty((need,), dtype=t.complex64, device=spectra.device)
     rc = _hip.lib().pt_spectral_frames(_hip.ptr(spectra), L, n, float(pixel_scale),

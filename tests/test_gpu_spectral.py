@@ -144,6 +144,31 @@ def test_spectral_frames_config2_match_oracle(hip, n):
         assert O.rel_rms(got[k], ref) < REL, (t, O.rel_rms(got[k], ref))
 
 
+@pytest.mark.parametrize("n,frames", [(32, 3), (64, 4), (256, 5)])
+def test_spectral_frames_white_spectra_all_modes(hip, n, frames):
+    """Flat random layer spectra put full weight on every mode, including the
+    Nyquist row and column, where the paired (Hermitian) transform needs its
+    exact correction; odd frame counts leave a single frame in the last pair."""
+    import torch
+    from pyturb_amd import ops
+
+    print(f"white frames n={n} frames={frames}", flush=True)
+    rng = np.random.default_rng(n + frames)
+    L, dx = 3, 0.05
+    spectra = (rng.standard_normal((L, n, n)) + 1j * rng.standard_normal((L, n, n))).astype(np.complex64)
+    disp = rng.uniform(-3.0, 3.0, (frames, L, 2))
+    disp[0] = 0.0
+    f = np.fft.fftfreq(n, d=dx)
+    sp64 = spectra.astype(np.complex128)
+    dev = torch.device("cuda")
+    got = _np(ops.spectral_frames(torch.from_numpy(spectra).to(dev),
+                                  torch.from_numpy(disp).to(dev), None, dx))
+    for t in range(frames):
+        ref = O.integrate(sp64, np.zeros((0, L, 3, 3), complex), f, None, None,
+                          disp[t, :, 0], disp[t, :, 1])
+        assert O.rel_rms(got[t], ref) < REL, (t, O.rel_rms(got[t], ref))
+
+
 def test_spectral_frame_matches_reference_golden(hip):
     g = load_golden("atmosphere_n64_seed2.npz")
     atm = _frames_case(64, None)

@@ -63,7 +63,13 @@ def test_library_loads_and_exports_every_header_symbol():
     assert _hip.lib().pt_abi_version() == 1
     assert "pt_work_elems" in syms
     # host-only sizing: power-of-two fused path vs general path
-    assert _hip.lib().pt_work_elems(1, 512, 10, 0) == 10 * 512 * 512
+    # frames: pairs of frames share one transform, plus the folded stack and its
+    # Nyquist side table (each section rounded to 32 elements)
+    assert _hip.lib().pt_frames_work_elems(512, 10, 9, 0) == 5 * 512 * 512 + 9 * 512 * 512 + 9 * 1024
+    assert _hip.lib().pt_frames_work_elems(100, 10, 9, 0) == 2 * 10 * 100 * 100
+    assert _hip.lib().pt_frames_work_elems(512, 10, 17, 0) == 2 * 10 * 512 * 512
+    for L in (1, 9, 16, 40):
+        assert _hip.lib().pt_work_elems(1, 512, 10, 3) >= _hip.lib().pt_frames_work_elems(512, 10, L, 3)
     assert _hip.lib().pt_work_elems(1, 100, 10, 0) == 2 * 10 * 100 * 100
     assert _hip.lib().pt_work_elems(0, 64, 5, 2) == 3 * 64 * 64 + 2 * 64 + 5 * 2 * 64 + (5 * 64 + 1) // 2
 
