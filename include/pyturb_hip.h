@@ -93,6 +93,27 @@ int pt_generate(const float* noise_re, const float* noise_im,
                 void* work, void* stream);
 
 /*
+ * Atmosphere.sample for several outer-scale groups in one pass
+ * (src/pyturb/atmosphere.py:439-470,1191-1219: one generator per distinct
+ * L0, their screens summed).  The transform and the subharmonic field are
+ * linear, so the groups' filtered noise and subharmonic coefficients are
+ * summed before a single transform per screen pair:
+ *   out = (sum_g screen_g + low_g) * scale * screen_scale[s]
+ *   noise     : (groups, 2, n_fft, n, n) f32, each group's real then
+ *               imaginary planes (as pt_generate's noise_re / noise_im)
+ *   amplitude : (groups, n, n) f32
+ *   sh_noise  : (groups, levels, 2, count, 3, 3) f32, NULL when levels == 0
+ *   sh_amp    : (groups, levels, 3, 3) f64
+ *   work      : pt_work_elems(0, n, count, levels) c64
+ * groups == 1 is pt_generate.  1 <= groups <= 64.
+ */
+int pt_generate_groups(const float* noise, const float* amplitude,
+                       const float* sh_noise, const double* sh_amp, int groups,
+                       int levels, int count, int n, int crop_offset,
+                       int n_out, float scale, const float* screen_scale,
+                       int accumulate, float* out, void* work, void* stream);
+
+/*
  * Spectral frozen-flow frames (src/pyturb/atmosphere.py:882-938 on-axis,
  * :940-998 for directions; src/pyturb/flow.py:96-129 for one layer).
  *

@@ -38,6 +38,8 @@ SIGNATURES = {
     "pt_frames_work_elems": (ctypes.c_longlong, [_i, _i, _i, _i]),
     "pt_fft2": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp]),
     "pt_generate": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp]),
+    "pt_generate_groups": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp,
+                                _vp]),
     "pt_spectral_frames": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp]),
     "pt_spectral_rows": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _vp]),
     "pt_spectral_cols": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp]),

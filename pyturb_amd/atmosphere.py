@@ -175,6 +175,7 @@ class Atmosphere:
                 self._sample_specs.append(
                     ("group", self.layers[idx[0]].L0, r0_agg, int(child.generate_state(1)[0])))
         self._sample_generators: Optional[List[PhaseScreen]] = None
+        self._group_filter_cache = None
         self._flow_seeds = flow_seeds
 
         self._t = 0.0
@@ -425,18 +426,50 @@ class Atmosphere:
         if cnt < 1:
             raise ValueError("count must be >= 1")
         scale = self._scale(wavelength)
-        out = None
-        for g, gen in enumerate(self._generators()):
+        gens = self._generators()
+        if len(gens) == 1:
+            gen = gens[0]
             if noises is None:
                 main, sh = gen.draw_noise(cnt)
                 noise, shn = gen._upload_noise(main, sh)
             else:
-                noise, shn = noises[g]
+                noise, shn = noises[0]
             out = ops.generate(noise, shn, gen._amplitude, gen._sh_amp, cnt,
-                               crop_offset=self.margin_pix, n_out=self.n, scale=scale,
-                               out=out, accumulate=out is not None)
+                               crop_offset=self.margin_pix, n_out=self.n, scale=scale)
+        else:
+            # all L0 groups in one pass: their filtered noise is summed before a
+            # single transform (the groups' draws keep their own streams)
+            t = _hip.torch()
+            dev = _hip.device(self._device_index)
+            G, n, P = len(gens), self.n_screen, self.subharmonics
+            n_fft = (cnt + 1) // 2
+            noise = t.empty((G, 2, n_fft, n, n), dtype=t.float32, device=dev)
+            shn = t.empty((G, P, 2, cnt, 3, 3), dtype=t.float32, device=dev) if P else None
+            for g, gen in enumerate(gens):
+                if noises is None:
+                    main, sh = gen.draw_noise(cnt)
+                    noise[g].copy_(t.from_numpy(np.ascontiguousarray(main, dtype=np.float32)))
+                    if P:
+                        shn[g].copy_(t.from_numpy(np.ascontiguousarray(sh, dtype=np.float32)))
+                else:
+                    noise[g].copy_(noises[g][0])
+                    if P:
+                        shn[g].copy_(noises[g][1])
+            amp, sh_amp = self._group_filters()
+            out = ops.generate_groups(noise, shn, amp, sh_amp, cnt, crop_offset=self.margin_pix,
+                                      n_out=self.n, scale=scale)
         out = self._cast(out)
         return out[0] if squeeze else out
+
+    def _group_filters(self):
+        """Stacked device filters of the Monte-Carlo groups (built once)."""
+        if self._group_filter_cache is None:
+            t = _hip.torch()
+            gens = self._generators()
+            amp = t.stack([g._amplitude for g in gens]).contiguous()
+            sh = t.stack([g._sh_amp for g in gens]).contiguous() if self.subharmonics else None
+            self._group_filter_cache = (amp, sh)
+        return self._group_filter_cache
 
     def reset(self) -> "Atmosphere":
         self._t = 0.0

@@ -35,17 +35,26 @@ struct ColsCfg {
 // ----------------------------------------------------------------------------
 // prep kernels (subharmonic tables)
 // ----------------------------------------------------------------------------
+// Subharmonic tables of each screen; with G groups the coefficient blocks
+// of the groups are summed (the low-frequency field is linear in them).
+// Group g's noise is sh_noise + g * P * 2 * count * 9, its amplitudes
+// sh_amp + g * P * 9.
 __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_noise,
                                                   const double* __restrict__ sh_amp, int count, int P, int n,
-                                                  SubTables t) {
+                                                  int G, SubTables t) {
   __shared__ double2 C[PMAX * 9];
   __shared__ double s_mean;
   const int s = blockIdx.x, tid = threadIdx.x;
   for (int q = tid; q < P * 9; q += 256) {
     const int p = q / 9, k = q % 9;
-    const double am = sh_amp[q];
-    C[q] = make_double2((double)sh_noise[((size_t)(p * 2 + 0) * count + s) * 9 + k] * am,
-                        (double)sh_noise[((size_t)(p * 2 + 1) * count + s) * 9 + k] * am);
+    double2 c = make_double2(0.0, 0.0);
+    for (int g = 0; g < G; ++g) {
+      const float* sn = sh_noise + (size_t)g * P * 2 * count * 9;
+      const double am = sh_amp[(size_t)g * P * 9 + q];
+      c.x += (double)sn[((size_t)(p * 2 + 0) * count + s) * 9 + k] * am;
+      c.y += (double)sn[((size_t)(p * 2 + 1) * count + s) * 9 + k] * am;
+    }
+    C[q] = c;
   }
   __syncthreads();
   sub_fill_set(C, P, n, s, t, s == 0, &s_mean, tid, 256);
@@ -64,10 +73,10 @@ __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ d
 }
 
 int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
-                    hipStream_t st) {
+                    hipStream_t st, int groups) {
   if (P == 0) return PT_OK;
   if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
-  k_gen_prep<<<count, 256, 0, st>>>(sh_noise, sh_amp, count, P, n, t);
+  k_gen_prep<<<count, 256, 0, st>>>(sh_noise, sh_amp, count, P, n, groups, t);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -130,9 +139,14 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __rest
 // ----------------------------------------------------------------------------
 // PhaseScreen.generate
 // ----------------------------------------------------------------------------
+// Pass A of generate for G outer-scale groups at once: the transform is
+// linear, so the groups' filtered noise is summed before one FFT,
+//   spectrum = sum_g (re_g + i im_g) * amp_g,
+// where group g's planes start at nre + g * gstride (real) and
+// nre + g * gstride + im_off (imaginary), and amp_g = amp + g * N * N.
 template <int N>
-__global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre, const float* __restrict__ nim,
-                                                  const float* __restrict__ amp, float2* __restrict__ work) {
+__global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre, ptrdiff_t im_off, size_t gstride,
+                                                  int G, const float* __restrict__ amp, float2* __restrict__ work) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
   __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
@@ -143,13 +157,19 @@ __global__ __launch_bounds__(256) void k_gen_rows(const float* __restrict__ nre,
   const size_t b = blockIdx.y;
   const size_t r0 = (size_t)blockIdx.x * M * N;
   const float* re = nre + b * plane + r0;
-  const float* im = nim + b * plane + r0;
   const float* am = amp + r0;
   float2* out = work + b * plane + r0;
   auto load = [&](int m, int i) {
     const size_t o = (size_t)m * N + i;
-    const float a = am[o];
-    return make_float2(re[o] * a, im[o] * a);
+    float2 acc = make_float2(0.f, 0.f);
+    for (int g = 0; g < G; ++g) {
+      const float* rg = re + (size_t)g * gstride;
+      const float* ig = rg + im_off;
+      const float a = am[(size_t)g * plane + o];
+      acc.x = fmaf(rg[o], a, acc.x);
+      acc.y = fmaf(ig[o], a, acc.y);
+    }
+    return acc;
   };
   auto store = [&](int m, int i, float2 v) { out[(size_t)m * N + i] = v; };
   wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
@@ -238,13 +258,14 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
 // ----------------------------------------------------------------------------
 // spectral frozen-flow frames
 // ----------------------------------------------------------------------------
-// Writing column pass: the epilogue fits 64 VGPRs, so two 1024-thread
-// workgroups share a CU where LDS allows (N <= 512).
+// Writing column pass: the epilogue fits 64 VGPRs and the LDS stays under
+// 80 KiB for N <= 1024, so two 1024-thread workgroups share a CU.
 template <int N>
 __global__ __launch_bounds__(ColsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(8))) void k_gen_cols_set(GenArgs a) {
   gen_cols_body<N, false>(a);
 }
-// Accumulating column pass (sums L0 groups into out).
+// Accumulating variant (adds into out); its read-modify-write needs a few
+// more registers, so no occupancy target (it would spill).
 template <int N>
 __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols_acc(GenArgs a) {
   gen_cols_body<N, true>(a);
@@ -283,14 +304,18 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
   }
 }
 
-// Pass A of a frame pair: the Hermitian layer sums of frames 2p and 2p + 1
-// packed as Z = Herm X_t1 + i Herm X_t2, then the row transform.  Shift
-// phasors use the exact fixed-point phases; the column factor is split as
+// Pass A of the frame engine: FPB frame pairs per workgroup.  For each pair
+// p the Hermitian layer sums of frames 2p and 2p + 1 are packed as
+// Z = Herm X_2p + i Herm X_2p+1 and row-transformed.  The folded stack is
+// read once per workgroup for all 2 * FPB frames.  Shift phasors use the
+// exact fixed-point phases; the column factor is split as
 // exp(2 pi i (8a + b - N/2) u) = t8[a] * t1[b], (k1 + N/2) mod N = 8a + b,
 // so the per-layer LDS tables are N/8 + 8 entries instead of N.
+constexpr int FPB = 1;  // frame pairs per workgroup
+
 template <int N>
 __host__ __device__ constexpr size_t pair_rows_lds(int L) {
-  return (size_t)2 * L * (N / 8 + 8 + RowsCfg<N>::M + 2) * 8 + (size_t)2 * L * 2 * 4;
+  return (size_t)2 * FPB * L * (N / 8 + 8 + RowsCfg<N>::M + 2) * 8 + (size_t)2 * FPB * L * 2 * 4;
 }
 
 template <int N>
@@ -298,32 +323,31 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
                                                          const float2* __restrict__ side, int L,
                                                          const double* __restrict__ disp, int F,
                                                          double inv_period, float2* __restrict__ work) {
-  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8;
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8, NF = 2 * FPB;
   static_assert(T == M * NB, "one row position per thread in the first stage");
   __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
   extern __shared__ __align__(16) unsigned char dyn[];
-  float2* t8 = (float2*)dyn;              // [2][L][NB]
-  float2* t1 = t8 + 2 * L * NB;           // [2][L][8]
-  float2* px = t1 + 2 * L * 8;            // [2][L][M]
-  float2* ex = px + 2 * L * M;            // [2][L]
-  float2* ey = ex + 2 * L;                // [2][L]
-  uint32_t* U = (uint32_t*)(ey + 2 * L);  // [2][L][2]
+  float2* t8 = (float2*)dyn;              // [NF][L][NB]
+  float2* t1 = t8 + NF * L * NB;          // [NF][L][8]
+  float2* px = t1 + NF * L * 8;           // [NF][L][M]
+  float2* ex = px + NF * L * M;           // [NF][L]
+  float2* ey = ex + NF * L;               // [NF][L]
+  uint32_t* U = (uint32_t*)(ey + NF * L); // [NF][L][2]
   const int tid = threadIdx.x;
-  const int pr = blockIdx.y;
+  const int t0 = blockIdx.y * NF;         // first frame of this workgroup
   const int row0 = blockIdx.x * M;
-  const bool two = 2 * pr + 1 < F;
-  for (int q = tid; q < 4 * L; q += T) {
-    const int t = 2 * pr + q / (2 * L);
+  for (int q = tid; q < NF * 2 * L; q += T) {
+    const int t = t0 + q / (2 * L);
     const int l = (q % (2 * L)) >> 1, ax = q & 1;
     U[q] = t < F ? cycles_fixed(disp[((size_t)t * L + l) * 2 + ax] * inv_period) : 0u;
   }
   init_twiddles<N, 1>(tw, tid, T);
   __syncthreads();
-  for (int q = tid; q < 2 * L * NB; q += T) t8[q] = phasor_fixed((uint32_t)(8 * (q % NB) - N / 2) * U[(q / NB) * 2 + 1]);
-  for (int q = tid; q < 2 * L * 8; q += T) t1[q] = phasor_fixed((uint32_t)(q & 7) * U[(q >> 3) * 2 + 1]);
-  for (int q = tid; q < 2 * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[(q / M) * 2]);
-  for (int q = tid; q < 2 * L; q += T) {
+  for (int q = tid; q < NF * L * NB; q += T) t8[q] = phasor_fixed((uint32_t)(8 * (q % NB) - N / 2) * U[(q / NB) * 2 + 1]);
+  for (int q = tid; q < NF * L * 8; q += T) t1[q] = phasor_fixed((uint32_t)(q & 7) * U[(q >> 3) * 2 + 1]);
+  for (int q = tid; q < NF * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[(q / M) * 2]);
+  for (int q = tid; q < NF * L; q += T) {
     ex[q] = phasor_fixed((uint32_t)N * U[q * 2]);
     ey[q] = phasor_fixed((uint32_t)N * U[q * 2 + 1]);
   }
@@ -333,31 +357,32 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
   const int k0 = row0 + m;
   const size_t plane = (size_t)N * N;
   const float2* fp = fold + (size_t)k0 * N + jq;
-  float2 z0[8], z1[8];
+  float2 z[NF][8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) z0[r] = z1[r] = make_float2(0.f, 0.f);
-  // For N >= 64 the eight columns jq + r*N/8 share (k1 + N/2) mod 8, so the
-  // row phasor times t1 is one product per layer and frame.
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) z[f][r] = make_float2(0.f, 0.f);
+  // For N >= 64 the eight columns share (k1 + N/2) mod 8: one t1 factor per
+  // layer and frame.
   constexpr bool BSAME = NB % 8 == 0;
+  const int b0 = (jq + N / 2) & 7;
   for (int l = 0; l < L; ++l) {
-    const float2 pa = px[l * M + m], pb = px[(L + l) * M + m];
-    const int b0 = (jq + N / 2) & 7;
-    const float2 ca0 = cmul(pa, t1[l * 8 + b0]), cb0 = cmul(pb, t1[(L + l) * 8 + b0]);
+    float2 c[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) c[f] = cmul(px[(f * L + l) * M + m], t1[(f * L + l) * 8 + b0]);
     float2 g[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) g[r] = fp[(size_t)l * plane + r * NB];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int qq = (jq + r * NB + N / 2) & (N - 1);
-      const float2 ca = BSAME ? ca0 : cmul(pa, t1[l * 8 + (qq & 7)]);
-      const float2 cb = BSAME ? cb0 : cmul(pb, t1[(L + l) * 8 + (qq & 7)]);
-      const float2 p0 = cmul(ca, t8[l * NB + (qq >> 3)]);
-      const float2 p1 = cmul(cb, t8[(L + l) * NB + (qq >> 3)]);
-      const float2 u0 = cmul(g[r], p0), u1 = cmul(g[r], p1);
-      z0[r].x += u0.x;
-      z0[r].y += u0.y;
-      z1[r].x += u1.x;
-      z1[r].y += u1.y;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const float2 cf = BSAME ? c[f] : cmul(px[(f * L + l) * M + m], t1[(f * L + l) * 8 + (qq & 7)]);
+        const float2 u = cmul(g[r], cmul(cf, t8[(f * L + l) * NB + (qq >> 3)]));
+        z[f][r].x += u.x;
+        z[f][r].y += u.y;
+      }
     }
   }
   if (k0 == N / 2 || (N / 2) % NB == jq) {  // Nyquist row, or the thread holding column N/2
@@ -368,35 +393,37 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
       const int j = k0 == N / 2 ? k1 : N + k0;
       const int qq = (k1 + N / 2) & (N - 1);
       for (int l = 0; l < L; ++l) {
-        float2 e0 = make_float2(1.f, 0.f), e1 = e0;
-        if (k0 == N / 2) {
-          e0 = ex[l];
-          e1 = ex[L + l];
-        }
-        if (k1 == N / 2) {
-          e0 = cmul(e0, ey[l]);
-          e1 = cmul(e1, ey[L + l]);
-        }
-        const float2 p0 = cmul(cmul(px[l * M + m], t1[l * 8 + (qq & 7)]), t8[l * NB + (qq >> 3)]);
-        const float2 p1 = cmul(cmul(px[(L + l) * M + m], t1[(L + l) * 8 + (qq & 7)]), t8[(L + l) * NB + (qq >> 3)]);
         const float2 bb = side[(size_t)l * 2 * N + j];
-        const float2 u0 = cmul(bb, cmul(p0, e0)), u1 = cmul(bb, cmul(p1, e1));
-        z0[r].x += u0.x;
-        z0[r].y += u0.y;
-        z1[r].x += u1.x;
-        z1[r].y += u1.y;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const int fl = f * L + l;
+          float2 e = make_float2(1.f, 0.f);
+          if (k0 == N / 2) e = ex[fl];
+          if (k1 == N / 2) e = cmul(e, ey[fl]);
+          const float2 pf = cmul(cmul(px[fl * M + m], t1[fl * 8 + (qq & 7)]), t8[fl * NB + (qq >> 3)]);
+          const float2 u = cmul(bb, cmul(pf, e));
+          z[f][r].x += u.x;
+          z[f][r].y += u.y;
+        }
       }
     }
   }
-  int cnt = 0;
-  auto load = [&](int, int) {
-    const float2 a0 = z0[cnt], a1 = two ? z1[cnt] : make_float2(0.f, 0.f);
-    ++cnt;
-    return make_float2(a0.x - a1.y, a0.y + a1.x);
-  };
-  float2* out = work + (size_t)pr * plane + (size_t)row0 * N;
-  auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
-  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
+#pragma unroll
+  for (int j = 0; j < FPB; ++j) {
+    const int t = t0 + 2 * j;
+    if (t >= F) break;  // workgroup-uniform
+    if (j > 0) __syncthreads();  // the previous transform's last LDS reads are done
+    const bool two = t + 1 < F;
+    int cnt = 0;
+    auto load = [&](int, int) {
+      const float2 a0 = z[2 * j][cnt], a1 = two ? z[2 * j + 1][cnt] : make_float2(0.f, 0.f);
+      ++cnt;
+      return make_float2(a0.x - a1.y, a0.y + a1.x);
+    };
+    float2* out = work + (size_t)(t / 2) * plane + (size_t)row0 * N;
+    auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
+    wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -443,12 +470,13 @@ static int fft2_pow2(const float2* in, float2* out, int batch, int sign, hipStre
 }
 
 template <int N>
-static int generate_pow2(const float* nre, const float* nim, const float* amp, const float* sh_noise,
-                         const double* sh_amp, GenArgs a, hipStream_t st) {
+static int generate_pow2(const float* nre, ptrdiff_t im_off, size_t gstride, int G, const float* amp,
+                         const float* sh_noise, const double* sh_amp, GenArgs a, hipStream_t st) {
   constexpr int MC = ColsCfg<N>::M;
-  k_gen_rows<N><<<dim3(N / RowsCfg<N>::M, a.n_fft), RowsCfg<N>::T, 0, st>>>(nre, nim, amp, (float2*)a.work);
+  k_gen_rows<N><<<dim3(N / RowsCfg<N>::M, a.n_fft), RowsCfg<N>::T, 0, st>>>(nre, im_off, gstride, G, amp,
+                                                                            (float2*)a.work);
   PT_LAUNCH_CHECK();
-  int rc = launch_gen_prep(sh_noise, sh_amp, a.count, a.levels, N, a.tab, st);
+  int rc = launch_gen_prep(sh_noise, sh_amp, a.count, a.levels, N, a.tab, st, G);
   if (rc) return rc;
   if (a.accumulate)
     k_gen_cols_acc<N><<<dim3(N / MC, a.n_fft), ColsCfg<N>::T, 0, st>>>(a);
@@ -490,8 +518,9 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   float2* side = work + w.side;
   k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
   PT_LAUNCH_CHECK();
-  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), RowsCfg<N>::T, dyn_r, st>>>(fold, side, L, disp, F,
-                                                                                inv_period, work);
+  const int groups = (F + 2 * FPB - 1) / (2 * FPB);
+  k_frame_pair_rows<N><<<dim3(N / MR, groups), RowsCfg<N>::T, dyn_r, st>>>(fold, side, L, disp, F, inv_period,
+                                                                          work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -580,17 +609,20 @@ extern "C" int pt_fft2(const void* in, void* out, void* work, int batch, int n0,
   return pt_fft2_generic(in, out, work, batch, n0, sign, stream);
 }
 
-extern "C" int pt_generate(const float* noise_re, const float* noise_im, const float* amplitude,
-                           const float* sh_noise, const double* sh_amp, int levels, int count, int n,
-                           int crop_offset, int n_out, float scale, const float* screen_scale, int accumulate,
-                           float* out, void* work, void* stream) {
-  if (count < 1) return fail(PT_ERR_ARG, "pt_generate: count must be >= 1");
-  if (!noise_re || !noise_im || !amplitude || !out || !work) return fail(PT_ERR_ARG, "pt_generate: NULL array");
-  if (levels < 0 || (levels > 0 && (!sh_noise || !sh_amp))) return fail(PT_ERR_ARG, "pt_generate: bad subharmonics");
-  if (levels > PMAX) return fail(PT_ERR_UNSUPPORTED, "pt_generate: more than %d subharmonic levels", PMAX);
-  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "pt_generate: crop outside grid");
+// Shared body of pt_generate (one group, separate real/imaginary planes) and
+// pt_generate_groups (G groups, planes at nre + g * gstride [+ im_off]).
+static int generate_impl(const char* who, const float* noise_re, ptrdiff_t im_off, size_t gstride, int G,
+                         const float* amplitude, const float* sh_noise, const double* sh_amp, int levels,
+                         int count, int n, int crop_offset, int n_out, float scale, const float* screen_scale,
+                         int accumulate, float* out, void* work, void* stream) {
+  if (count < 1) return fail(PT_ERR_ARG, "%s: count must be >= 1", who);
+  if (G < 1 || G > 64) return fail(PT_ERR_ARG, "%s: 1..64 groups", who);
+  if (!noise_re || !amplitude || !out || !work) return fail(PT_ERR_ARG, "%s: NULL array", who);
+  if (levels < 0 || (levels > 0 && (!sh_noise || !sh_amp))) return fail(PT_ERR_ARG, "%s: bad subharmonics", who);
+  if (levels > PMAX) return fail(PT_ERR_UNSUPPORTED, "%s: more than %d subharmonic levels", who, PMAX);
+  if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "%s: crop outside grid", who);
   const int n_fft = (count + 1) / 2;
-  if (n_fft > 65535) return fail(PT_ERR_ARG, "pt_generate: count > 131070 per call");
+  if (n_fft > 65535) return fail(PT_ERR_ARG, "%s: count > 131070 per call", who);
   hipStream_t st = (hipStream_t)stream;
   if (is_pow2_fast(n) && fused_ok(n, levels)) {
     GenArgs a;
@@ -607,12 +639,41 @@ extern "C" int pt_generate(const float* noise_re, const float* noise_im, const f
     a.s_mul = 1;
     a.s_add = n_fft;
     a.scale = scale;
-#define CALL(N) return generate_pow2<N>(noise_re, noise_im, amplitude, sh_noise, sh_amp, a, st);
+#define CALL(N) return generate_pow2<N>(noise_re, im_off, gstride, G, amplitude, sh_noise, sh_amp, a, st);
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   }
-  return pt_generate_generic(noise_re, noise_im, amplitude, sh_noise, sh_amp, levels, count, n, crop_offset,
-                             n_out, scale, screen_scale, accumulate, out, work, stream);
+  // general sizes: one group at a time, accumulating
+  const size_t plane = (size_t)n * n;
+  for (int g = 0; g < G; ++g) {
+    const float* re = noise_re + (size_t)g * gstride;
+    int rc = pt_generate_generic(re, re + im_off, amplitude + (size_t)g * plane,
+                                 levels ? sh_noise + (size_t)g * levels * 2 * count * 9 : nullptr,
+                                 levels ? sh_amp + (size_t)g * levels * 9 : nullptr, levels, count, n, crop_offset,
+                                 n_out, scale, screen_scale, (accumulate || g > 0) ? 1 : 0, out, work, stream);
+    if (rc) return rc;
+  }
+  return PT_OK;
+}
+
+extern "C" int pt_generate(const float* noise_re, const float* noise_im, const float* amplitude,
+                           const float* sh_noise, const double* sh_amp, int levels, int count, int n,
+                           int crop_offset, int n_out, float scale, const float* screen_scale, int accumulate,
+                           float* out, void* work, void* stream) {
+  if (!noise_re || !noise_im) return fail(PT_ERR_ARG, "pt_generate: NULL array");
+  return generate_impl("pt_generate", noise_re, noise_im - noise_re, 0, 1, amplitude, sh_noise, sh_amp, levels,
+                       count, n, crop_offset, n_out, scale, screen_scale, accumulate, out, work, stream);
+}
+
+extern "C" int pt_generate_groups(const float* noise, const float* amplitude, const float* sh_noise,
+                                  const double* sh_amp, int groups, int levels, int count, int n, int crop_offset,
+                                  int n_out, float scale, const float* screen_scale, int accumulate, float* out,
+                                  void* work, void* stream) {
+  const size_t n_fft = (size_t)(count + 1) / 2;
+  const size_t plane = (size_t)n * n;
+  return generate_impl("pt_generate_groups", noise, (ptrdiff_t)(n_fft * plane), 2 * n_fft * plane, groups,
+                       amplitude, sh_noise, sh_amp, levels, count, n, crop_offset, n_out, scale, screen_scale,
+                       accumulate, out, work, stream);
 }
 
 // The two passes of pt_spectral_frames as separate calls (power-of-two n,

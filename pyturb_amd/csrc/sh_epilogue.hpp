@@ -103,17 +103,22 @@ __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* di
 }
 
 // Subharmonic terms of one column tile, staged through LDS: the row-step
-// table F, the basis at the N/8 base rows, and H, K of the tile's M columns
-// for up to S coefficient sets (screens / frames).  prefetch() issues the
+// table F, the basis at the N/8 base rows (N <= 512), and H, K of the
+// tile's M columns for up to S coefficient sets (screens / frames).  prefetch() issues the
 // global loads into registers before the transform (they complete under
 // the tile load and the FFT stages); commit() writes them to LDS after the
 // transform, followed by one barrier, so the epilogue never waits on HBM.
 template <int N, int M, int S, int T>
 struct ColSub {
-  static constexpr int EMAX = PREG * (N / 8) + S * PREG * M + S * M;
+  // The basis at the N/8 base rows is staged in LDS up to N = 512; larger
+  // tiles would then exceed the 80 KiB that two workgroups per CU allow, so
+  // there e_p(x0) is evaluated per thread instead.
+  static constexpr bool UXL = N <= 512;
+  static constexpr int UXP = UXL ? PREG : 1, UXN = UXL ? N / 8 : 1;
+  static constexpr int EMAX = UXP * UXN * (UXL ? 1 : 0) + S * PREG * M + S * M;
   static constexpr int R = (EMAX + T - 1) / T;  // prefetched values per thread
   float2 F[PREG][8];
-  float2 Ux[PREG][N / 8];
+  float2 Ux[UXP][UXN];
   float2 H[S][PREG][M];
   float K[S][M];
 
@@ -121,9 +126,9 @@ struct ColSub {
     float2 v[R];
   };
 
-  // element q of the packed list [U: P*N/8 | H: S*P*M | K: S*M]
+  // element q of the packed list [U: P*N/8 (UXL only) | H: S*P*M | K: S*M]
   __device__ __forceinline__ static float2 fetch(const SubTables& t, const int (&set)[S], int c0, int P, int q) {
-    const int eu = P * (N / 8), eh = S * P * M;
+    const int eu = UXL ? P * (N / 8) : 0, eh = S * P * M;
     if (q < eu) return t.U[(size_t)(q / (N / 8)) * N + q % (N / 8)];
     q -= eu;
     if (q < eh) {
@@ -146,12 +151,12 @@ struct ColSub {
   }
 
   __device__ __forceinline__ void commit(const Regs& g, int P, int tid) {
-    const int eu = P * (N / 8), eh = S * P * M;
+    const int eu = UXL ? P * (N / 8) : 0, eh = S * P * M;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       int q = tid + r * T;
       if (q < eu) {
-        Ux[q / (N / 8)][q % (N / 8)] = g.v[r];
+        Ux[q / UXN][q % UXN] = g.v[r];
         continue;
       }
       q -= eu;
@@ -181,7 +186,8 @@ struct ColSub {
 #pragma unroll
     for (int c = 0; c < 8; ++c) v[c] += k;
     for (int p = 0; p < P; ++p) {
-      const float2 a = cmul(Ux[p][x0], H[s][p][m]);
+      const float2 e = UXL ? Ux[p][UXL ? x0 : 0] : sh_basis_e(p, x0, N);  // e_p(x0)
+      const float2 a = cmul(e, H[s][p][m]);
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const float2 f = F[p][row_step<RL>(c)];
@@ -208,7 +214,7 @@ __device__ __forceinline__ float sub_eval(const SubTables& t, int s, int x, int 
 
 // Prep launchers (kernels in screens.hip).  Tables must not alias inputs.
 int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
-                    hipStream_t st);
+                    hipStream_t st, int groups = 1);
 int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
                       SubTables t, hipStream_t st);
 

@@ -112,6 +112,46 @@ def generate(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offset: int
     return out
 
 
+def generate_groups(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offset: int = 0,
+                    n_out: Optional[int] = None, scale: float = 1.0, screen_scale=None,
+                    out=None, accumulate: bool = False, work=None):
+    """Summed screens of G outer-scale groups in one pass (pt_generate_groups).
+
+    ``noise`` (G, 2, ceil(count/2), n, n) float32; ``sh_noise``
+    (G, P, 2, count, 3, 3) float32 or None; ``amplitude`` (G, n, n) float32;
+    ``sh_amp`` (G, P, 3, 3) float64.  Returns ``out`` (count, n_out, n_out).
+    """
+    t = _t()
+    G, n = int(amplitude.shape[0]), int(amplitude.shape[-1])
+    n_fft = (count + 1) // 2
+    _require(noise, t.float32, (G, 2, n_fft, n, n), "noise")
+    _require(amplitude, t.float32, (G, n, n), "amplitude")
+    P = 0 if sh_amp is None else int(sh_amp.shape[1])
+    if P:
+        _require(sh_amp, t.float64, (G, P, 3, 3), "sh_amp")
+        _require(sh_noise, t.float32, (G, P, 2, count, 3, 3), "sh_noise")
+    n_out = n if n_out is None else int(n_out)
+    if out is None:
+        out = t.zeros((count, n_out, n_out), dtype=t.float32, device=noise.device) if accumulate \
+            else t.empty((count, n_out, n_out), dtype=t.float32, device=noise.device)
+    _require(out, t.float32, (count, n_out, n_out), "out")
+    if screen_scale is not None:
+        _require(screen_scale, t.float32, (count,), "screen_scale")
+    need = work_elems(0, n, count, P)
+    if work is None:
+        work = t.empty((need,), dtype=t.complex64, device=noise.device)
+    elif work.numel() < need or work.dtype != t.complex64:
+        raise ValueError(f"work: need {need} complex64 elements")
+    rc = _hip.lib().pt_generate_groups(_hip.ptr(noise), _hip.ptr(amplitude),
+                                       _hip.ptr(sh_noise) if P else None,
+                                       _hip.ptr(sh_amp) if P else None, G, P, int(count), n,
+                                       int(crop_offset), n_out, float(scale),
+                                       _hip.ptr(screen_scale), int(bool(accumulate)),
+                                       _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "pt_generate_groups")
+    return out
+
+
 def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset: int = 0,
                     n_out: Optional[int] = None, scale: float = 1.0, out=None, work=None):
     """Frozen-flow frames (see pt_spectral_frames).

@@ -224,6 +224,39 @@ def test_sample_api_matches_reference_golden(hip):
     assert O.rel_rms(got, g["sample"]) < REL
 
 
+@pytest.mark.parametrize("n,count", [(128, 7), (96, 4)])
+def test_generate_groups_equals_sum_of_groups(hip, n, count):
+    """pt_generate_groups (one transform for all outer-scale groups) against
+    the oracle's per-group screens summed; n = 96 runs the general-size path."""
+    import torch
+    from pyturb_amd import ops
+
+    print(f"generate groups n={n} count={count}", flush=True)
+    rng = np.random.default_rng(n)
+    dx, P = 8.0 / n, 3
+    L0s = [20.0, 35.0, 60.0]
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    n_fft = (count + 1) // 2
+    noise = rng.standard_normal((3, 2, n_fft, n, n)).astype(np.float32)
+    shn = rng.standard_normal((3, P, 2, count, 3, 3)).astype(np.float32)
+    filts = [O.build_filters(n, dx, 0.12, L0, subharmonics=P) for L0 in L0s]
+    ref = sum(O.generate(f, f32(noise[g]), f32(shn[g]), count) for g, f in enumerate(filts))
+    ss = np.linspace(0.7, 1.3, count).astype(np.float32)
+    off, n_out, scale = 5, n - 10, 0.5
+    want = ref[:, off:off + n_out, off:off + n_out] * scale * ss[:, None, None]
+    dev = torch.device("cuda")
+    amp = torch.from_numpy(np.stack([f["amplitude"] for f in filts]).astype(np.float32)).to(dev)
+    sha = torch.from_numpy(np.stack([f["sh_amp"] for f in filts])).to(dev)
+    out = ops.generate_groups(torch.from_numpy(noise).to(dev), torch.from_numpy(shn).to(dev), amp,
+                              sha, count, crop_offset=off, n_out=n_out, scale=scale,
+                              screen_scale=torch.from_numpy(ss).to(dev))
+    assert O.rel_rms(_np(out), want) < REL
+    ops.generate_groups(torch.from_numpy(noise).to(dev), torch.from_numpy(shn).to(dev), amp, sha,
+                        count, crop_offset=off, n_out=n_out, scale=scale,
+                        screen_scale=torch.from_numpy(ss).to(dev), out=out, accumulate=True)
+    assert O.rel_rms(_np(out), 2 * want) < REL
+
+
 def test_phase_screen_api_matches_reference_golden(hip):
     import pyturb_amd as pa
 
