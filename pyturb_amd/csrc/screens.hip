@@ -304,18 +304,14 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
   }
 }
 
-// Pass A of the frame engine: FPB frame pairs per workgroup.  For each pair
-// p the Hermitian layer sums of frames 2p and 2p + 1 are packed as
-// Z = Herm X_2p + i Herm X_2p+1 and row-transformed.  The folded stack is
-// read once per workgroup for all 2 * FPB frames.  Shift phasors use the
-// exact fixed-point phases; the column factor is split as
+// Pass A of a frame pair: the Hermitian layer sums of frames 2p and 2p + 1
+// packed as Z = Herm X_t1 + i Herm X_t2, then the row transform.  Shift
+// phasors use the exact fixed-point phases; the column factor is split as
 // exp(2 pi i (8a + b - N/2) u) = t8[a] * t1[b], (k1 + N/2) mod N = 8a + b,
 // so the per-layer LDS tables are N/8 + 8 entries instead of N.
-constexpr int FPB = 1;  // frame pairs per workgroup
-
 template <int N>
 __host__ __device__ constexpr size_t pair_rows_lds(int L) {
-  return (size_t)2 * FPB * L * (N / 8 + 8 + RowsCfg<N>::M + 2) * 8 + (size_t)2 * FPB * L * 2 * 4;
+  return (size_t)2 * L * (N / 8 + 8 + RowsCfg<N>::M + 2) * 8 + (size_t)2 * L * 2 * 4;
 }
 
 template <int N>
@@ -323,31 +319,32 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
                                                          const float2* __restrict__ side, int L,
                                                          const double* __restrict__ disp, int F,
                                                          double inv_period, float2* __restrict__ work) {
-  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8, NF = 2 * FPB;
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8;
   static_assert(T == M * NB, "one row position per thread in the first stage");
   __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
   extern __shared__ __align__(16) unsigned char dyn[];
-  float2* t8 = (float2*)dyn;              // [NF][L][NB]
-  float2* t1 = t8 + NF * L * NB;          // [NF][L][8]
-  float2* px = t1 + NF * L * 8;           // [NF][L][M]
-  float2* ex = px + NF * L * M;           // [NF][L]
-  float2* ey = ex + NF * L;               // [NF][L]
-  uint32_t* U = (uint32_t*)(ey + NF * L); // [NF][L][2]
+  float2* t8 = (float2*)dyn;              // [2][L][NB]
+  float2* t1 = t8 + 2 * L * NB;           // [2][L][8]
+  float2* px = t1 + 2 * L * 8;            // [2][L][M]
+  float2* ex = px + 2 * L * M;            // [2][L]
+  float2* ey = ex + 2 * L;                // [2][L]
+  uint32_t* U = (uint32_t*)(ey + 2 * L);  // [2][L][2]
   const int tid = threadIdx.x;
-  const int t0 = blockIdx.y * NF;         // first frame of this workgroup
+  const int pr = blockIdx.y;
   const int row0 = blockIdx.x * M;
-  for (int q = tid; q < NF * 2 * L; q += T) {
-    const int t = t0 + q / (2 * L);
+  const bool two = 2 * pr + 1 < F;
+  for (int q = tid; q < 4 * L; q += T) {
+    const int t = 2 * pr + q / (2 * L);
     const int l = (q % (2 * L)) >> 1, ax = q & 1;
     U[q] = t < F ? cycles_fixed(disp[((size_t)t * L + l) * 2 + ax] * inv_period) : 0u;
   }
   init_twiddles<N, 1>(tw, tid, T);
   __syncthreads();
-  for (int q = tid; q < NF * L * NB; q += T) t8[q] = phasor_fixed((uint32_t)(8 * (q % NB) - N / 2) * U[(q / NB) * 2 + 1]);
-  for (int q = tid; q < NF * L * 8; q += T) t1[q] = phasor_fixed((uint32_t)(q & 7) * U[(q >> 3) * 2 + 1]);
-  for (int q = tid; q < NF * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[(q / M) * 2]);
-  for (int q = tid; q < NF * L; q += T) {
+  for (int q = tid; q < 2 * L * NB; q += T) t8[q] = phasor_fixed((uint32_t)(8 * (q % NB) - N / 2) * U[(q / NB) * 2 + 1]);
+  for (int q = tid; q < 2 * L * 8; q += T) t1[q] = phasor_fixed((uint32_t)(q & 7) * U[(q >> 3) * 2 + 1]);
+  for (int q = tid; q < 2 * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[(q / M) * 2]);
+  for (int q = tid; q < 2 * L; q += T) {
     ex[q] = phasor_fixed((uint32_t)N * U[q * 2]);
     ey[q] = phasor_fixed((uint32_t)N * U[q * 2 + 1]);
   }
@@ -357,32 +354,31 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
   const int k0 = row0 + m;
   const size_t plane = (size_t)N * N;
   const float2* fp = fold + (size_t)k0 * N + jq;
-  float2 z[NF][8];
+  float2 z0[8], z1[8];
 #pragma unroll
-  for (int f = 0; f < NF; ++f)
-#pragma unroll
-    for (int r = 0; r < 8; ++r) z[f][r] = make_float2(0.f, 0.f);
-  // For N >= 64 the eight columns share (k1 + N/2) mod 8: one t1 factor per
-  // layer and frame.
+  for (int r = 0; r < 8; ++r) z0[r] = z1[r] = make_float2(0.f, 0.f);
+  // For N >= 64 the eight columns jq + r*N/8 share (k1 + N/2) mod 8, so the
+  // row phasor times t1 is one product per layer and frame.
   constexpr bool BSAME = NB % 8 == 0;
-  const int b0 = (jq + N / 2) & 7;
   for (int l = 0; l < L; ++l) {
-    float2 c[NF];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) c[f] = cmul(px[(f * L + l) * M + m], t1[(f * L + l) * 8 + b0]);
+    const float2 pa = px[l * M + m], pb = px[(L + l) * M + m];
+    const int b0 = (jq + N / 2) & 7;
+    const float2 ca0 = cmul(pa, t1[l * 8 + b0]), cb0 = cmul(pb, t1[(L + l) * 8 + b0]);
     float2 g[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) g[r] = fp[(size_t)l * plane + r * NB];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int qq = (jq + r * NB + N / 2) & (N - 1);
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const float2 cf = BSAME ? c[f] : cmul(px[(f * L + l) * M + m], t1[(f * L + l) * 8 + (qq & 7)]);
-        const float2 u = cmul(g[r], cmul(cf, t8[(f * L + l) * NB + (qq >> 3)]));
-        z[f][r].x += u.x;
-        z[f][r].y += u.y;
-      }
+      const float2 ca = BSAME ? ca0 : cmul(pa, t1[l * 8 + (qq & 7)]);
+      const float2 cb = BSAME ? cb0 : cmul(pb, t1[(L + l) * 8 + (qq & 7)]);
+      const float2 p0 = cmul(ca, t8[l * NB + (qq >> 3)]);
+      const float2 p1 = cmul(cb, t8[(L + l) * NB + (qq >> 3)]);
+      const float2 u0 = cmul(g[r], p0), u1 = cmul(g[r], p1);
+      z0[r].x += u0.x;
+      z0[r].y += u0.y;
+      z1[r].x += u1.x;
+      z1[r].y += u1.y;
     }
   }
   if (k0 == N / 2 || (N / 2) % NB == jq) {  // Nyquist row, or the thread holding column N/2
@@ -393,37 +389,35 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
       const int j = k0 == N / 2 ? k1 : N + k0;
       const int qq = (k1 + N / 2) & (N - 1);
       for (int l = 0; l < L; ++l) {
-        const float2 bb = side[(size_t)l * 2 * N + j];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          const int fl = f * L + l;
-          float2 e = make_float2(1.f, 0.f);
-          if (k0 == N / 2) e = ex[fl];
-          if (k1 == N / 2) e = cmul(e, ey[fl]);
-          const float2 pf = cmul(cmul(px[fl * M + m], t1[fl * 8 + (qq & 7)]), t8[fl * NB + (qq >> 3)]);
-          const float2 u = cmul(bb, cmul(pf, e));
-          z[f][r].x += u.x;
-          z[f][r].y += u.y;
+        float2 e0 = make_float2(1.f, 0.f), e1 = e0;
+        if (k0 == N / 2) {
+          e0 = ex[l];
+          e1 = ex[L + l];
         }
+        if (k1 == N / 2) {
+          e0 = cmul(e0, ey[l]);
+          e1 = cmul(e1, ey[L + l]);
+        }
+        const float2 p0 = cmul(cmul(px[l * M + m], t1[l * 8 + (qq & 7)]), t8[l * NB + (qq >> 3)]);
+        const float2 p1 = cmul(cmul(px[(L + l) * M + m], t1[(L + l) * 8 + (qq & 7)]), t8[(L + l) * NB + (qq >> 3)]);
+        const float2 bb = side[(size_t)l * 2 * N + j];
+        const float2 u0 = cmul(bb, cmul(p0, e0)), u1 = cmul(bb, cmul(p1, e1));
+        z0[r].x += u0.x;
+        z0[r].y += u0.y;
+        z1[r].x += u1.x;
+        z1[r].y += u1.y;
       }
     }
   }
-#pragma unroll
-  for (int j = 0; j < FPB; ++j) {
-    const int t = t0 + 2 * j;
-    if (t >= F) break;  // workgroup-uniform
-    if (j > 0) __syncthreads();  // the previous transform's last LDS reads are done
-    const bool two = t + 1 < F;
-    int cnt = 0;
-    auto load = [&](int, int) {
-      const float2 a0 = z[2 * j][cnt], a1 = two ? z[2 * j + 1][cnt] : make_float2(0.f, 0.f);
-      ++cnt;
-      return make_float2(a0.x - a1.y, a0.y + a1.x);
-    };
-    float2* out = work + (size_t)(t / 2) * plane + (size_t)row0 * N;
-    auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
-    wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
-  }
+  int cnt = 0;
+  auto load = [&](int, int) {
+    const float2 a0 = z0[cnt], a1 = two ? z1[cnt] : make_float2(0.f, 0.f);
+    ++cnt;
+    return make_float2(a0.x - a1.y, a0.y + a1.x);
+  };
+  float2* out = work + (size_t)pr * plane + (size_t)row0 * N;
+  auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
+  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
 }
 
 // ----------------------------------------------------------------------------
@@ -518,9 +512,8 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   float2* side = work + w.side;
   k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
   PT_LAUNCH_CHECK();
-  const int groups = (F + 2 * FPB - 1) / (2 * FPB);
-  k_frame_pair_rows<N><<<dim3(N / MR, groups), RowsCfg<N>::T, dyn_r, st>>>(fold, side, L, disp, F, inv_period,
-                                                                          work);
+  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), RowsCfg<N>::T, dyn_r, st>>>(fold, side, L, disp, F,
+                                                                                inv_period, work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
