@@ -212,6 +212,24 @@ int pt_readout(const float* buf, int layers, long long cap, int width, int ld,
                int accumulate, float* out, void* stream);
 
 /*
+ * Adjoint of pt_readout with respect to each (direction, layer)'s readout
+ * geometry, for a cotangent grad = dLoss/d(out) (dirs, n, n) f32 device:
+ *   out[(d * layers + l) * 3 + {0, 1, 2}] = dLoss/d sa, dLoss/d sp, dLoss/d mag
+ * (f64 device; sa, sp, mag as in dir_geom).  Same taps and clamping as the
+ * forward; the interpolation weights are differentiated analytically
+ * (linear, Catmull-Rom, normalised Lanczos-3).  The reference computes no
+ * derivatives; this serves fitting wind travel and viewing directions
+ * through the extruder readout.  work: pt_readout_adjoint_work_elems(n)
+ * float32 elements.  Other arguments as pt_readout.
+ */
+long long pt_readout_adjoint_work_elems(int n);
+int pt_readout_adjoint(const float* buf, int layers, long long cap, int width,
+                       int ld, const int* start, const long long* fill,
+                       const double* wind_cs, const double* dir_geom, int dirs,
+                       int n, int interp, float scale, const float* grad,
+                       double* out, float* work, void* stream);
+
+/*
  * Differentiable frame model (profile and wind fitting).  The reference has
  * no derivatives; these entry points serve a fit of per-layer Cn2 weights,
  * r0, L0 and winds to OPD frames.  The layer spectra are written as

@@ -48,6 +48,9 @@ SIGNATURES = {
                              _vp, _vp, _vp]),
     "pt_readout": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _f,
                         _i, _vp, _vp]),
+    "pt_readout_adjoint_work_elems": (ctypes.c_longlong, [_i]),
+    "pt_readout_adjoint": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i,
+                                _f, _vp, _vp, _vp, _vp]),
     "pt_layer_spectra": (_i, [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp]),
     "pt_adjoint_work_elems": (ctypes.c_longlong, [_i, _i, _i]),
     "pt_spectral_frames_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i, _vp, _i, _i,

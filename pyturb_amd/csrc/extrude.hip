@@ -179,6 +179,139 @@ __device__ __forceinline__ void taps_weights(float t, float* w) {
   }
 }
 
+// Tap weights and their derivatives with respect to the fraction t.
+template <int TAPS>
+__device__ __forceinline__ void taps_dweights(float t, float* w, float* dw) {
+  if constexpr (TAPS == 2) {
+    w[0] = 1.f - t;
+    w[1] = t;
+    dw[0] = -1.f;
+    dw[1] = 1.f;
+  } else if constexpr (TAPS == 4) {
+    const float t2 = t * t, t3 = t2 * t;
+    w[0] = 0.5f * (-t + 2.f * t2 - t3);
+    w[1] = 0.5f * (2.f - 5.f * t2 + 3.f * t3);
+    w[2] = 0.5f * (t + 4.f * t2 - 3.f * t3);
+    w[3] = 0.5f * (t3 - t2);
+    dw[0] = 0.5f * (-1.f + 4.f * t - 3.f * t2);
+    dw[1] = 0.5f * (-10.f * t + 9.f * t2);
+    dw[2] = 0.5f * (1.f + 8.f * t - 9.f * t2);
+    dw[3] = 0.5f * (3.f * t2 - 2.f * t);
+  } else {  // normalised Lanczos-3: w_k = s_k / sum s, s_k = sinc(x) sinc(x/3), x = t - (k - 2)
+    float sv[6], ds[6], tot = 0.f, dtot = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float x = t - (float)(k - 2);
+      if (x == 0.f) {
+        sv[k] = 1.f;
+        ds[k] = 0.f;
+      } else {
+        const float x3 = x * (1.f / 3.f);
+        float s1, c1, s3, c3;
+        sincospif(x, &s1, &c1);
+        sincospif(x3, &s3, &c3);
+        const float pi = 3.14159265358979f;
+        const float a = s1 / (pi * x), b = s3 / (pi * x3);
+        const float da = (c1 - a) / x, db = (c3 - b) / x3;  // d sinc(y)/dy = (cos(pi y) - sinc(y)) / y
+        sv[k] = a * b;
+        ds[k] = da * b + a * db * (1.f / 3.f);
+      }
+      tot += sv[k];
+      dtot += ds[k];
+    }
+    const float inv = 1.f / tot;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      w[k] = sv[k] * inv;
+      dw[k] = (ds[k] - w[k] * dtot) * inv;
+    }
+  }
+}
+
+// Adjoint of the readout with respect to each (direction, layer)'s geometry
+// (row shift sa, column shift sp, magnification mag) for a cotangent grad:
+//   d/dsa  = scale * sum_px grad * d interp / d row,
+//   d/dsp  = scale * sum_px grad * d interp / d col,
+//   d/dmag = scale * sum_px grad * (d/drow (ci c + cj s) + d/dcol (-ci s + cj c)),
+// ci = i - (n-1)/2, cj = j - (n-1)/2.  Taps are the forward's (same clamping,
+// same FP32 coordinates); the per-block sums go to partial[((d * nblk + blk)
+// * L + l) * 3 + q] and are added in FP64 by k_readout_adj_final.
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_readout_adj(ReadoutArgs a, const float* __restrict__ grad,
+                                                     float* __restrict__ partial) {
+  constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
+  __shared__ float red[4 * 3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = blockIdx.x * 32 + (threadIdx.x & 31);
+  const int i = blockIdx.y * 8 + (threadIdx.x >> 5);
+  const bool in = i < a.n && j < a.n;
+  const int d = blockIdx.z;
+  const int nblk = gridDim.x * gridDim.y, blk = blockIdx.y * gridDim.x + blockIdx.x;
+  const float h = 0.5f * (float)(a.n - 1);
+  const float ci = in ? (float)i - h : 0.f, cj = in ? (float)j - h : 0.f;
+  const float G = in ? grad[((size_t)d * a.n + i) * a.n + j] * a.scale : 0.f;
+  for (int l = 0; l < a.L; ++l) {
+    float q3[3] = {0.f, 0.f, 0.f};
+    if (in) {
+      const float mg = a.mag[d][l];
+      const float gi = ci * mg, gj = cj * mg;
+      const float c = a.cosw[l], s = a.sinw[l];
+      const float row = fmaf(gi, c, fmaf(gj, s, a.r_frac[d][l]));
+      const float col = fmaf(-gi, s, fmaf(gj, c, a.c_frac[d][l]));
+      const float rf = floorf(row), cf = floorf(col);
+      const int r0 = (int)rf + a.r_int[d][l], c0 = (int)cf + a.c_int[d][l];
+      float wr[TAPS], dwr[TAPS], wc[TAPS], dwc[TAPS];
+      taps_dweights<TAPS>(row - rf, wr, dwr);
+      taps_dweights<TAPS>(col - cf, wc, dwc);
+      const float* bl = a.buf + (size_t)l * a.cap * a.ld;
+      const int fm = a.fillm1[l], st = a.start[l];
+      float vr = 0.f, vc = 0.f;
+#pragma unroll
+      for (int u = 0; u < TAPS; ++u) {
+        int ph = st + min(max(r0 + OFF + u, 0), fm);
+        ph = ph >= a.cap ? ph - a.cap : ph;
+        const float* br = bl + (size_t)ph * a.ld;
+        float rw = 0.f, rdw = 0.f;
+#pragma unroll
+        for (int v = 0; v < TAPS; ++v) {
+          const float b = br[min(max(c0 + OFF + v, 0), a.W - 1)];
+          rw = fmaf(wc[v], b, rw);
+          rdw = fmaf(dwc[v], b, rdw);
+        }
+        vr = fmaf(dwr[u], rw, vr);
+        vc = fmaf(wr[u], rdw, vc);
+      }
+      const float gr = G * vr, gc = G * vc;
+      q3[0] = gr;
+      q3[1] = gc;
+      q3[2] = gr * (ci * c + cj * s) + gc * (cj * c - ci * s);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float v = q3[q];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+      if (lane == 0) red[wave * 3 + q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int q = threadIdx.x;
+      partial[(((size_t)d * nblk + blk) * a.L + l) * 3 + q] = red[q] + red[3 + q] + red[6 + q] + red[9 + q];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void k_readout_adj_final(const float* __restrict__ partial, int D, int L, int nblk,
+                                                          int ldo, double* __restrict__ out) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= D * L * 3) return;
+  const int q = idx % 3, l = (idx / 3) % L, d = idx / (3 * L);
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b) acc += (double)partial[(((size_t)d * nblk + b) * L + l) * 3 + q];
+  out[((size_t)d * ldo + l) * 3 + q] = acc;
+}
+
 template <int TAPS>
 __global__ __launch_bounds__(256) void k_readout(ReadoutArgs a) {
   constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
@@ -399,51 +532,73 @@ extern "C" int pt_extrude_rows(const float* A, int width, int ld, int stencil_ro
   return PT_OK;
 }
 
+// Geometry of one chunk of up to RD directions x RL layers: host-split
+// integer and fractional shifts, FP32 rotation and magnification.
+static int pack_readout(ReadoutArgs& a, const float* buf, int layers, long long cap, int width, int ld,
+                        const int* start, const long long* fill, const double* wind_cs, const double* dir_geom,
+                        int dirs, int n, int d0, int l0) {
+  a.buf = buf + (size_t)l0 * cap * ld;
+  a.out = nullptr;
+  a.cap = (int)cap;
+  a.W = width;
+  a.ld = ld;
+  a.n = n;
+  a.L = layers - l0 < RL ? layers - l0 : RL;
+  a.D = dirs - d0 < RD ? dirs - d0 : RD;
+  a.accumulate = 0;
+  a.scale = 1.f;
+  for (int l = 0; l < RL; ++l) {
+    const int src = l0 + (l < a.L ? l : 0);
+    a.start[l] = start[src];
+    a.fillm1[l] = (int)(fill[src] - 1);
+    a.cosw[l] = (float)wind_cs[2 * src];
+    a.sinw[l] = (float)wind_cs[2 * src + 1];
+  }
+  for (int d = 0; d < RD; ++d)
+    for (int l = 0; l < RL; ++l) {
+      const int sd = d0 + (d < a.D ? d : 0), sl = l0 + (l < a.L ? l : 0);
+      const double* gp = dir_geom + ((size_t)sd * layers + sl) * 3;
+      const double rs = gp[0], cs = 0.5 * width + gp[1];  // row / column shifts
+      const double rfl = floor(rs), cfl = floor(cs);
+      if (!(fabs(rfl) <= 1e9) || !(fabs(cfl) <= 1e9)) return fail(PT_ERR_ARG, "readout: shift out of range");
+      a.r_int[d][l] = (int)rfl;
+      a.c_int[d][l] = (int)cfl;
+      a.r_frac[d][l] = (float)(rs - rfl);
+      a.c_frac[d][l] = (float)(cs - cfl);
+      a.mag[d][l] = (float)gp[2];
+    }
+  return PT_OK;
+}
+
+static int readout_check(const char* who, const float* buf, int layers, long long cap, int width, int ld,
+                         const int* start, const long long* fill, const double* wind_cs, const double* dir_geom,
+                         int dirs, int n, int interp) {
+  if (!buf || !start || !fill || !wind_cs || !dir_geom) return fail(PT_ERR_ARG, "%s: NULL array", who);
+  if (layers < 1 || dirs < 1 || n < 1 || width < 1 || cap < 1 || ld < width) return fail(PT_ERR_ARG, "%s: bad sizes", who);
+  if (cap * (long long)ld > (1LL << 30)) return fail(PT_ERR_UNSUPPORTED, "%s: ring slab over 2^30 floats", who);
+  if (interp < 0 || interp > 2) return fail(PT_ERR_ARG, "%s: interp must be 0 linear, 1 cubic, 2 lanczos", who);
+  for (int l = 0; l < layers; ++l) {
+    if (fill[l] < 1 || fill[l] > cap || start[l] < 0 || start[l] >= cap)
+      return fail(PT_ERR_ARG, "%s: layer %d ring state out of range", who, l);
+  }
+  return PT_OK;
+}
+
 extern "C" int pt_readout(const float* buf, int layers, long long cap, int width, int ld, const int* start,
                           const long long* fill, const double* wind_cs, const double* dir_geom, int dirs, int n,
                           int interp, float scale, int accumulate, float* out, void* stream) {
-  if (!buf || !start || !fill || !wind_cs || !dir_geom || !out) return fail(PT_ERR_ARG, "pt_readout: NULL array");
-  if (layers < 1 || dirs < 1 || n < 1 || width < 1 || cap < 1 || ld < width) return fail(PT_ERR_ARG, "pt_readout: bad sizes");
-  if (cap * (long long)ld > (1LL << 30)) return fail(PT_ERR_UNSUPPORTED, "pt_readout: ring slab over 2^30 floats");
-  if (interp < 0 || interp > 2) return fail(PT_ERR_ARG, "pt_readout: interp must be 0 linear, 1 cubic, 2 lanczos");
-  for (int l = 0; l < layers; ++l) {
-    if (fill[l] < 1 || fill[l] > cap || start[l] < 0 || start[l] >= cap)
-      return fail(PT_ERR_ARG, "pt_readout: layer %d ring state out of range", l);
-  }
+  if (!out) return fail(PT_ERR_ARG, "pt_readout: NULL array");
+  int rc = readout_check("pt_readout", buf, layers, cap, width, ld, start, fill, wind_cs, dir_geom, dirs, n, interp);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   for (int d0 = 0; d0 < dirs; d0 += RD) {
     for (int l0 = 0; l0 < layers; l0 += RL) {
       ReadoutArgs a;
-      a.buf = buf + (size_t)l0 * cap * ld;
+      rc = pack_readout(a, buf, layers, cap, width, ld, start, fill, wind_cs, dir_geom, dirs, n, d0, l0);
+      if (rc) return rc;
       a.out = out + (size_t)d0 * n * n;
-      a.cap = (int)cap;
-      a.W = width;
-      a.ld = ld;
-      a.n = n;
-      a.L = layers - l0 < RL ? layers - l0 : RL;
-      a.D = dirs - d0 < RD ? dirs - d0 : RD;
       a.accumulate = (accumulate || l0 > 0) ? 1 : 0;
       a.scale = scale;
-      for (int l = 0; l < RL; ++l) {
-        const int src = l0 + (l < a.L ? l : 0);
-        a.start[l] = start[src];
-        a.fillm1[l] = (int)(fill[src] - 1);
-        a.cosw[l] = (float)wind_cs[2 * src];
-        a.sinw[l] = (float)wind_cs[2 * src + 1];
-      }
-      for (int d = 0; d < RD; ++d)
-        for (int l = 0; l < RL; ++l) {
-          const int sd = d0 + (d < a.D ? d : 0), sl = l0 + (l < a.L ? l : 0);
-          const double* gp = dir_geom + ((size_t)sd * layers + sl) * 3;
-          const double rs = gp[0], cs = 0.5 * width + gp[1];  // row / column shifts
-          const double rfl = floor(rs), cfl = floor(cs);
-          if (fabs(rfl) > 1e9 || fabs(cfl) > 1e9) return fail(PT_ERR_ARG, "pt_readout: shift out of range");
-          a.r_int[d][l] = (int)rfl;
-          a.c_int[d][l] = (int)cfl;
-          a.r_frac[d][l] = (float)(rs - rfl);
-          a.c_frac[d][l] = (float)(cs - cfl);
-          a.mag[d][l] = (float)gp[2];
-        }
       const dim3 g((n + 31) / 32, (n + 7) / 8, a.D);
       if (n >= TB) {
         dim3 gb((n + TB - 1) / TB, (n + TB - 1) / TB, a.D);
@@ -460,6 +615,47 @@ extern "C" int pt_readout(const float* buf, int layers, long long cap, int width
       } else {
         k_readout<6><<<g, 256, 0, st>>>(a);
       }
+      PT_LAUNCH_CHECK();
+    }
+  }
+  return PT_OK;
+}
+
+static size_t readout_adj_blocks(int n) { return (size_t)((n + 31) / 32) * ((n + 7) / 8); }
+
+extern "C" long long pt_readout_adjoint_work_elems(int n) {
+  if (n < 1) return -1;
+  return (long long)((size_t)RD * readout_adj_blocks(n) * RL * 3);
+}
+
+extern "C" int pt_readout_adjoint(const float* buf, int layers, long long cap, int width, int ld, const int* start,
+                                  const long long* fill, const double* wind_cs, const double* dir_geom, int dirs,
+                                  int n, int interp, float scale, const float* grad, double* out, float* work,
+                                  void* stream) {
+  if (!grad || !out || !work) return fail(PT_ERR_ARG, "pt_readout_adjoint: NULL array");
+  int rc = readout_check("pt_readout_adjoint", buf, layers, cap, width, ld, start, fill, wind_cs, dir_geom, dirs,
+                         n, interp);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int nblk = (int)readout_adj_blocks(n);
+  for (int d0 = 0; d0 < dirs; d0 += RD) {
+    for (int l0 = 0; l0 < layers; l0 += RL) {
+      ReadoutArgs a;
+      rc = pack_readout(a, buf, layers, cap, width, ld, start, fill, wind_cs, dir_geom, dirs, n, d0, l0);
+      if (rc) return rc;
+      a.scale = scale;
+      const float* g = grad + (size_t)d0 * n * n;
+      const dim3 grid((n + 31) / 32, (n + 7) / 8, a.D);
+      if (interp == 0)
+        k_readout_adj<2><<<grid, 256, 0, st>>>(a, g, work);
+      else if (interp == 1)
+        k_readout_adj<4><<<grid, 256, 0, st>>>(a, g, work);
+      else
+        k_readout_adj<6><<<grid, 256, 0, st>>>(a, g, work);
+      PT_LAUNCH_CHECK();
+      const int cnt = a.D * a.L * 3;
+      k_readout_adj_final<<<(cnt + 63) / 64, 64, 0, st>>>(work, a.D, a.L, nblk, layers,
+                                                           out + ((size_t)d0 * layers + l0) * 3);
       PT_LAUNCH_CHECK();
     }
   }

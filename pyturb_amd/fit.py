@@ -32,7 +32,7 @@ from . import _hip, ops
 from .backend import resolve_device
 from .spectrum import build_filters_dL0
 
-__all__ = ["ProfileFrames"]
+__all__ = ["ProfileFrames", "readout_frames"]
 
 
 class ProfileFrames:
@@ -184,12 +184,81 @@ def _make_fn():
 
 
 class _LazyFn:
-    _fn = None
+    def __init__(self, make):
+        self._make = make
+        self._fn = None
 
     def apply(self, *args):
-        if _LazyFn._fn is None:
-            _LazyFn._fn = _make_fn()
-        return _LazyFn._fn.apply(*args)
+        if self._fn is None:
+            self._fn = self._make()
+        return self._fn.apply(*args)
 
 
-_SpectralFramesFn = _LazyFn()
+_SpectralFramesFn = _LazyFn(_make_fn)
+
+
+# ---------------------------------------------------------------------------
+# Extruder readout as a differentiable function of its geometry
+# ---------------------------------------------------------------------------
+def readout_frames(ext, thx, thy, mags=None, travel_offset=None, scale: float = 1.0):
+    """Summed phase of an ExtrudedAtmosphere for D directions, differentiable in
+    the viewing tangents ``thx``/``thy`` (D,), the per-layer extra along-wind
+    travel ``travel_offset`` (L,) [pixels] and the magnifications ``mags``
+    (D, L), with the ring buffers held fixed.
+
+    The geometry is the reference's (src/pyturb/extrude.py:458-469,807-822):
+    ``sa = travel - base + h (thx cos + thy sin)``, ``sp = h (-thx sin + thy cos)``
+    with h the layer altitude in pixels; the backward pass is
+    ``pt_readout_adjoint``.
+    """
+    t = _hip.torch()
+    dev = ext._buf.device
+    L = ext.n_layers
+    thx = t.as_tensor(thx, dtype=t.float64, device=dev).reshape(-1)
+    thy = t.as_tensor(thy, dtype=t.float64, device=dev).reshape(-1)
+    D = thx.numel()
+    alt = t.tensor([ly.alt_pix for ly in ext.layers], dtype=t.float64, device=dev)
+    cw = t.tensor([ly.cos for ly in ext.layers], dtype=t.float64, device=dev)
+    sw = t.tensor([ly.sin for ly in ext.layers], dtype=t.float64, device=dev)
+    base = t.tensor([ly.travel - ly.ring.base for ly in ext.layers], dtype=t.float64, device=dev)
+    if travel_offset is not None:
+        base = base + t.as_tensor(travel_offset, dtype=t.float64, device=dev)
+    dxp = alt[None, :] * thx[:, None]
+    dyp = alt[None, :] * thy[:, None]
+    sa = base[None, :] + dxp * cw + dyp * sw
+    sp = -dxp * sw + dyp * cw
+    if mags is None:
+        mg = t.tensor([ly.mag for ly in ext.layers], dtype=t.float64, device=dev).expand(D, L)
+    else:
+        mg = t.as_tensor(mags, dtype=t.float64, device=dev).expand(D, L)
+    geom = t.stack([sa, sp, mg], dim=-1)
+    return _ReadoutFn.apply(geom, ext, float(scale))
+
+
+def _make_readout_fn():
+    t = _hip.torch()
+    from .infinite import readout, readout_adjoint
+
+    class ReadoutFn(t.autograd.Function):
+        """Extruder readout with pt_readout_adjoint as its backward."""
+
+        @staticmethod
+        def forward(ctx, geom, ext, scale):
+            g = geom.detach().cpu().numpy()
+            ctx.ext, ctx.scale, ctx.geom = ext, scale, g
+            ctx.dev = geom.device
+            return readout(ext._buf, ext.capacity, ext.width, [ly.ring for ly in ext.layers],
+                           ext._wind_cs, g, ext.n, ext.interp, scale=scale)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            ext = ctx.ext
+            res = readout_adjoint(ext._buf, ext.capacity, ext.width, [ly.ring for ly in ext.layers],
+                                  ext._wind_cs, ctx.geom, ext.n, ext.interp,
+                                  grad_out.float().contiguous(), scale=ctx.scale)
+            return res.to(ctx.dev), None, None
+
+    return ReadoutFn
+
+
+_ReadoutFn = _LazyFn(_make_readout_fn)

@@ -192,6 +192,30 @@ def readout(buf, cap, width, rings, wind_cs, dir_geom, n, interp, scale=1.0, out
     return out
 
 
+def readout_adjoint(buf, cap, width, rings, wind_cs, dir_geom, n, interp, grad, scale=1.0):
+    """dLoss/d(sa, sp, mag) per (direction, layer) for dLoss/d(out) = grad (see
+    pt_readout_adjoint); returns a (D, L, 3) float64 device tensor."""
+    t = _hip.torch()
+    L = len(rings)
+    D = len(dir_geom)
+    if tuple(grad.shape) != (D, n, n) or grad.dtype != t.float32:
+        raise ValueError(f"grad must be ({D}, {n}, {n}) float32")
+    grad = grad.contiguous()
+    out = t.empty((D, L, 3), dtype=t.float64, device=buf.device)
+    work = t.empty((int(_hip.lib().pt_readout_adjoint_work_elems(n)),), dtype=t.float32,
+                   device=buf.device)
+    start = (ctypes_int * L)(*[r.start for r in rings])
+    fill = (ctypes_ll * L)(*[r.fill for r in rings])
+    cs = (ctypes_double * (2 * L))(*[v for c in wind_cs for v in c])
+    geom = (ctypes_double * (3 * D * L))(*np.asarray(dir_geom, dtype=np.float64).ravel())
+    rc = _hip.lib().pt_readout_adjoint(_hip.ptr(buf), L, cap, width, int(buf.shape[-1]), start,
+                                       fill, cs, geom, D, n, _INTERP_CODE[interp], float(scale),
+                                       _hip.ptr(grad), _hip.ptr(out), _hip.ptr(work),
+                                       _hip.stream_ptr())
+    _hip.check(rc, "pt_readout_adjoint")
+    return out
+
+
 class InfinitePhaseScreen:
     """Endless frozen-flow screen extruded row by row along axis 0.
 

@@ -180,3 +180,116 @@ def test_adam_fit_recovers_hidden_profile_direction(hip):
         losses.append(float(loss))
     print("loss first/last", losses[0], losses[-1])
     assert losses[-1] < 0.1 * losses[0]
+
+
+# ---------------------------------------------------------------------------
+# extruder readout adjoint
+# ---------------------------------------------------------------------------
+def readout_case(seed=5, n=24, W=40, cap=44, L=3):
+    """Random ring buffers and a 3-direction geometry whose taps all stay away
+    from integer positions (where linear interpolation has kinks) and from
+    the clamped rows and columns, so central differences are valid."""
+    rng = np.random.default_rng(seed)
+    buf = rng.standard_normal((L, cap, W)).astype(np.float32)
+    cs = [(np.cos(a), np.sin(a)) for a in rng.uniform(0, 2 * np.pi, L)]
+    c = 0.5 * cap
+    geom = np.empty((3, L, 3))
+    geom[:, :, 0] = c + rng.uniform(-2, 2, (3, L))
+    geom[:, :, 1] = rng.uniform(-2, 2, (3, L))
+    geom[:, :, 2] = rng.uniform(0.8, 1.05, (3, L))
+    G = rng.standard_normal((3, n, n)).astype(np.float32)
+    return buf, cs, geom, G
+
+
+def readout_oracle(buf, cs, geom, n, interp, fill):
+    W = buf.shape[-1]
+    out = []
+    for d in range(geom.shape[0]):
+        along, perp = zip(*[O.pupil_grids(n, c, s, W, geom[d, l, 2]) for l, (c, s) in enumerate(cs)])
+        out.append(O.readout(buf.astype(np.float64), fill, np.stack(along), np.stack(perp),
+                             geom[d, :, 0], geom[d, :, 1], interp))
+    return np.stack(out)
+
+
+def min_tap_distance(cs, geom, n, W):
+    """Smallest distance of any pixel's row/col position to an integer."""
+    worst = 1.0
+    for d in range(geom.shape[0]):
+        for l, (c, s) in enumerate(cs):
+            along, perp = O.pupil_grids(n, c, s, W, geom[d, l, 2])
+            for v in (along + geom[d, l, 0], perp + geom[d, l, 1]):
+                worst = min(worst, float(np.min(np.abs(v - np.round(v)))))
+    return worst
+
+
+@pytest.mark.parametrize("interp", ["linear", "cubic", "lanczos"])
+def test_readout_adjoint_matches_fp64_finite_differences(hip, interp):
+    import torch
+    from pyturb_amd.infinite import readout_adjoint
+    from pyturb_amd.ring import Ring
+
+    print(f"readout adjoint {interp}", flush=True)
+    n, h = 24, 1e-7
+    buf, cs, geom, G = readout_case(n=n)
+    L, cap, W = buf.shape
+    assert min_tap_distance(cs, geom, n, W) > 1e-5  # valid central differences
+    rings = [Ring(cap, 2, 0, cap) for _ in range(L)]
+    fill = np.full(L, cap)
+    dev = torch.device("cuda")
+    got = readout_adjoint(torch.from_numpy(buf).to(dev), cap, W, rings, cs, geom, n, interp,
+                          torch.from_numpy(G).to(dev), scale=0.5).cpu().numpy()
+
+    def loss(gm):
+        return 0.5 * float(np.sum(readout_oracle(buf, cs, gm, n, interp, fill) * G))
+
+    fd = np.zeros_like(geom)
+    for idx in np.ndindex(*geom.shape):
+        e = np.zeros_like(geom)
+        e[idx] = h
+        fd[idx] = (loss(geom + e) - loss(geom - e)) / (2 * h)
+    errs = [float(np.linalg.norm(got[..., q] - fd[..., q]) / np.linalg.norm(fd[..., q]))
+            for q in range(3)]
+    print("rel err sa/sp/mag", errs)
+    assert max(errs) < GRAD_REL, errs
+
+
+def test_readout_frames_chain_to_directions_and_travel(hip):
+    """Gradients of the extruder readout w.r.t. viewing tangents and along-wind
+    travel through readout_frames agree with central differences of the
+    device forward itself (FP32 forward, so a 1e-3 bar at h = 1e-3 px)."""
+    import torch
+    from pyturb_amd.extrude import ExtrudedAtmosphere
+    from pyturb_amd.fit import readout_frames
+
+    ext = ExtrudedAtmosphere(n=32, pixel_scale=0.25, layer_r0=[0.2, 0.35], layer_L0=[25.0, 25.0],
+                             layer_wind=[(6.0, 2.5), (-3.0, 11.0)],
+                             layer_altitude_los=[0.0, 8000.0], field_of_view_pix=[0.0, 3.0],
+                             interp="cubic", seeds=[11, 12])
+    ext.set_time(0.05)
+    dev = torch.device("cuda")
+    arc = np.pi / 180 / 3600
+    thx = torch.tensor([0.3 * arc, -1.1 * arc], dtype=torch.float64, device=dev, requires_grad=True)
+    thy = torch.tensor([0.7 * arc, 0.4 * arc], dtype=torch.float64, device=dev, requires_grad=True)
+    trav = torch.tensor([0.137, -0.211], dtype=torch.float64, device=dev, requires_grad=True)
+    G = torch.randn((2, 32, 32), device=dev, generator=torch.Generator(device=dev).manual_seed(1))
+    out = readout_frames(ext, thx, thy, travel_offset=trav)
+    (out * G).sum().backward()
+
+    def f(tx, ty, tr):
+        with torch.no_grad():
+            return float((readout_frames(ext, tx, ty, travel_offset=tr).double() * G.double()).sum())
+
+    h = 1e-3
+    fd = []
+    for k in range(2):
+        e = torch.zeros(2, dtype=torch.float64, device=dev)
+        e[k] = h
+        fd.append((f(thx.detach(), thy.detach(), trav.detach() + e)
+                   - f(thx.detach(), thy.detach(), trav.detach() - e)) / (2 * h))
+    fd = np.array(fd)
+    assert np.linalg.norm(trav.grad.cpu().numpy() - fd) < 1e-3 * np.linalg.norm(fd)
+    hp = 1e-3 / 8000.0 * 0.25  # about 1e-3 px at the 8 km layer
+    e = torch.tensor([hp, 0.0], dtype=torch.float64, device=dev)
+    fdx = (f(thx.detach() + e, thy.detach(), trav.detach())
+           - f(thx.detach() - e, thy.detach(), trav.detach())) / (2 * hp)
+    assert abs(thx.grad[0].item() - fdx) < 2e-3 * abs(fdx)
