@@ -14,6 +14,7 @@ from .atmosphere import Atmosphere, ExtrudeBoilingPerformanceWarning, PeriodicWr
 from .backend import resolve_device
 from .flow import FourierFlowScreen
 from .fourier import PhaseScreen
+from .infinite import InfinitePhaseScreen, phase_covariance
 from .profiles import (
     Layer,
     coherence_time,
@@ -38,6 +39,7 @@ __all__ = [
     "Atmosphere",
     "ExtrudeBoilingPerformanceWarning",
     "FourierFlowScreen",
+    "InfinitePhaseScreen",
     "Layer",
     "PeriodicWrapWarning",
     "PhaseScreen",
@@ -49,6 +51,7 @@ __all__ = [
     "list_profiles",
     "mean_turbulence_height",
     "opd_to_phase",
+    "phase_covariance",
     "phase_to_opd",
     "r0_at_wavelength",
     "r0_from_seeing",
