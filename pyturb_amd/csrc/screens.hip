@@ -311,14 +311,79 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
 // so the per-layer LDS tables are N/8 + 8 entries instead of N.
 template <int N>
 __host__ __device__ constexpr size_t pair_rows_lds(int L) {
-  return (size_t)2 * L * (N / 8 + 8 + RowsCfg<N>::M + 2) * 8 + (size_t)2 * L * 2 * 4;
+  return (size_t)2 * L * (N / 8 + 8 + RowsCfg<N>::M) * 8 + (size_t)2 * L * 4;
+}
+
+// Per frame pair, the row-independent phasor tables of pass A, computed once
+// instead of in each of the N/M row blocks: for frames f in {2p, 2p+1} and
+// layers l, t8[a] (a < N/8), t1[b] (b < 8) and the row phase U_x (bit
+// pattern in .x).  Layout per pair: [2][L][N/8 + 9] float2.
+template <int N>
+__host__ __device__ constexpr int pair_tab_stride() { return N / 8 + 9; }
+
+// It also sums the Nyquist-line terms of the pair, corr[j] = sum_l B_l(k) *
+// (P_l,1(k) E_l,1(k) + i P_l,2(k) E_l,2(k)), for the Nyquist row k = (N/2, j)
+// (j < N) and the Nyquist column k = (j - N, N/2) (N <= j < 2N, k0 != N/2),
+// with the phase of P taken exactly as one fixed-point sum.  Pass A adds
+// corr to the packed sum of those entries.
+template <int N>
+__global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restrict__ disp,
+                                                           const float2* __restrict__ side, int L, int F,
+                                                           double inv_period, float2* __restrict__ tab,
+                                                           float2* __restrict__ corr) {
+  constexpr int NB = N / 8, S = pair_tab_stride<N>();
+  const int pr = blockIdx.x;
+  float2* tp = tab + (size_t)pr * 2 * L * S;
+  for (int q = threadIdx.x; q < 2 * N; q += 256) {
+    const int k0 = q < N ? N / 2 : q - N, k1 = q < N ? q : N / 2;
+    float2 acc = make_float2(0.f, 0.f);
+    if (q < N || k0 != N / 2) {
+      for (int l = 0; l < L; ++l) {
+        float2 w[2];
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          const int t = 2 * pr + f;
+          w[f] = make_float2(0.f, 0.f);
+          if (t >= F) continue;
+          const uint32_t ux = cycles_fixed(disp[((size_t)t * L + l) * 2 + 0] * inv_period);
+          const uint32_t uy = cycles_fixed(disp[((size_t)t * L + l) * 2 + 1] * inv_period);
+          uint32_t ph = (uint32_t)signed_freq(k0, N) * ux + (uint32_t)signed_freq(k1, N) * uy;
+          if (k0 == N / 2) ph += (uint32_t)N * ux;  // E: exp(2 pi i s_0 / dx)
+          if (k1 == N / 2) ph += (uint32_t)N * uy;  // E: exp(2 pi i s_1 / dx)
+          w[f] = phasor_fixed(ph);
+        }
+        const float2 wv = make_float2(w[0].x - w[1].y, w[0].y + w[1].x);
+        const float2 u = cmul(side[(size_t)l * 2 * N + q], wv);
+        acc.x += u.x;
+        acc.y += u.y;
+      }
+    }
+    corr[(size_t)pr * 2 * N + q] = acc;
+  }
+  for (int q = threadIdx.x; q < 2 * L * S; q += 256) {
+    const int fl = q / S, e = q % S;
+    const int t = 2 * pr + fl / L, l = fl % L;
+    uint32_t ux = 0u, uy = 0u;
+    if (t < F) {
+      ux = cycles_fixed(disp[((size_t)t * L + l) * 2 + 0] * inv_period);
+      uy = cycles_fixed(disp[((size_t)t * L + l) * 2 + 1] * inv_period);
+    }
+    float2 v;
+    if (e < NB)
+      v = phasor_fixed((uint32_t)(8 * e - N / 2) * uy);
+    else if (e < NB + 8)
+      v = phasor_fixed((uint32_t)(e - NB) * uy);
+    else
+      v = make_float2(__uint_as_float(ux), 0.f);
+    tp[q] = v;
+  }
 }
 
 template <int N>
 __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restrict__ fold,
-                                                         const float2* __restrict__ side, int L,
-                                                         const double* __restrict__ disp, int F,
-                                                         double inv_period, float2* __restrict__ work) {
+                                                         const float2* __restrict__ corr, int L,
+                                                         const float2* __restrict__ ptab, int F,
+                                                         float2* __restrict__ work) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8;
   static_assert(T == M * NB, "one row position per thread in the first stage");
   __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
@@ -327,94 +392,89 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
   float2* t8 = (float2*)dyn;              // [2][L][NB]
   float2* t1 = t8 + 2 * L * NB;           // [2][L][8]
   float2* px = t1 + 2 * L * 8;            // [2][L][M]
-  float2* ex = px + 2 * L * M;            // [2][L]
-  float2* ey = ex + 2 * L;                // [2][L]
-  uint32_t* U = (uint32_t*)(ey + 2 * L);  // [2][L][2]
+  uint32_t* U = (uint32_t*)(px + 2 * L * M);  // [2][L] row phases
   const int tid = threadIdx.x;
   const int pr = blockIdx.y;
   const int row0 = blockIdx.x * M;
   const bool two = 2 * pr + 1 < F;
-  for (int q = tid; q < 4 * L; q += T) {
-    const int t = 2 * pr + q / (2 * L);
-    const int l = (q % (2 * L)) >> 1, ax = q & 1;
-    U[q] = t < F ? cycles_fixed(disp[((size_t)t * L + l) * 2 + ax] * inv_period) : 0u;
+  constexpr int S = pair_tab_stride<N>();
+  const float2* tp = ptab + (size_t)pr * 2 * L * S;
+  for (int q = tid; q < 2 * L * S; q += T) {
+    const int fl = q / S, e = q % S;
+    const float2 v = tp[q];
+    if (e < NB)
+      t8[fl * NB + e] = v;
+    else if (e < NB + 8)
+      t1[fl * 8 + e - NB] = v;
+    else
+      U[fl] = __float_as_uint(v.x);
   }
   init_twiddles<N, 1>(tw, tid, T);
   __syncthreads();
-  for (int q = tid; q < 2 * L * NB; q += T) t8[q] = phasor_fixed((uint32_t)(8 * (q % NB) - N / 2) * U[(q / NB) * 2 + 1]);
-  for (int q = tid; q < 2 * L * 8; q += T) t1[q] = phasor_fixed((uint32_t)(q & 7) * U[(q >> 3) * 2 + 1]);
-  for (int q = tid; q < 2 * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[(q / M) * 2]);
-  for (int q = tid; q < 2 * L; q += T) {
-    ex[q] = phasor_fixed((uint32_t)N * U[q * 2]);
-    ey[q] = phasor_fixed((uint32_t)N * U[q * 2 + 1]);
-  }
+  for (int q = tid; q < 2 * L * M; q += T) px[q] = phasor_fixed((uint32_t)signed_freq(row0 + q % M, N) * U[q / M]);
   __syncthreads();
   // first-stage positions of this thread: row m, columns jq + r * NB (r < 8)
   const int m = tid / NB, jq = tid % NB;
   const int k0 = row0 + m;
   const size_t plane = (size_t)N * N;
   const float2* fp = fold + (size_t)k0 * N + jq;
-  float2 z0[8], z1[8];
+  // z[r] accumulates the packed sum Herm X_t1 + i Herm X_t2 directly: per layer
+  // and element, Q = P_1 + i P_2 (two phasor products folded into 8 FMAs),
+  // then one complex multiply-accumulate z += g Q (4 FMAs).
+  float2 z[8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) z0[r] = z1[r] = make_float2(0.f, 0.f);
-  // For N >= 64 the eight columns jq + r*N/8 share (k1 + N/2) mod 8, so the
-  // row phasor times t1 is one product per layer and frame.
+  for (int r = 0; r < 8; ++r) z[r] = make_float2(0.f, 0.f);
+  // For N >= 64 the eight columns share (k1 + N/2) mod 8, so the row phasor
+  // times t1 is one product per layer and frame.
   constexpr bool BSAME = NB % 8 == 0;
+  const int b0 = (jq + N / 2) & 7;
   for (int l = 0; l < L; ++l) {
     const float2 pa = px[l * M + m], pb = px[(L + l) * M + m];
-    const int b0 = (jq + N / 2) & 7;
-    const float2 ca0 = cmul(pa, t1[l * 8 + b0]), cb0 = cmul(pb, t1[(L + l) * 8 + b0]);
+    const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
+    const float2 cb0 = two ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
     float2 g[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) g[r] = fp[(size_t)l * plane + r * NB];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int qq = (jq + r * NB + N / 2) & (N - 1);
-      const float2 ca = BSAME ? ca0 : cmul(pa, t1[l * 8 + (qq & 7)]);
-      const float2 cb = BSAME ? cb0 : cmul(pb, t1[(L + l) * 8 + (qq & 7)]);
-      const float2 p0 = cmul(ca, t8[l * NB + (qq >> 3)]);
-      const float2 p1 = cmul(cb, t8[(L + l) * NB + (qq >> 3)]);
-      const float2 u0 = cmul(g[r], p0), u1 = cmul(g[r], p1);
-      z0[r].x += u0.x;
-      z0[r].y += u0.y;
-      z1[r].x += u1.x;
-      z1[r].y += u1.y;
+      float2 ca = ca0, cb = cb0;
+      if (!BSAME) {
+        ca = cmul(pa, t1[l * 8 + (qq & 7)]);
+        cb = two ? cmul(pb, t1[(L + l) * 8 + (qq & 7)]) : make_float2(0.f, 0.f);
+      }
+      const float2 ta = t8[l * NB + (qq >> 3)], tb = t8[(L + l) * NB + (qq >> 3)];
+      float qx = ca.x * ta.x;
+      qx = fmaf(-ca.y, ta.y, qx);
+      qx = fmaf(-cb.x, tb.y, qx);
+      qx = fmaf(-cb.y, tb.x, qx);
+      float qy = ca.x * ta.y;
+      qy = fmaf(ca.y, ta.x, qy);
+      qy = fmaf(cb.x, tb.x, qy);
+      qy = fmaf(-cb.y, tb.y, qy);
+      z[r].x = fmaf(g[r].x, qx, z[r].x);
+      z[r].x = fmaf(-g[r].y, qy, z[r].x);
+      z[r].y = fmaf(g[r].x, qy, z[r].y);
+      z[r].y = fmaf(g[r].y, qx, z[r].y);
     }
   }
-  if (k0 == N / 2 || (N / 2) % NB == jq) {  // Nyquist row, or the thread holding column N/2
+  // Nyquist row / column terms of this pair (see k_frame_pair_tables)
+  const float2* cp = corr + (size_t)pr * 2 * N;
+  if (k0 == N / 2) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int k1 = jq + r * NB;
-      if (k0 != N / 2 && k1 != N / 2) continue;
-      const int j = k0 == N / 2 ? k1 : N + k0;
-      const int qq = (k1 + N / 2) & (N - 1);
-      for (int l = 0; l < L; ++l) {
-        float2 e0 = make_float2(1.f, 0.f), e1 = e0;
-        if (k0 == N / 2) {
-          e0 = ex[l];
-          e1 = ex[L + l];
-        }
-        if (k1 == N / 2) {
-          e0 = cmul(e0, ey[l]);
-          e1 = cmul(e1, ey[L + l]);
-        }
-        const float2 p0 = cmul(cmul(px[l * M + m], t1[l * 8 + (qq & 7)]), t8[l * NB + (qq >> 3)]);
-        const float2 p1 = cmul(cmul(px[(L + l) * M + m], t1[(L + l) * 8 + (qq & 7)]), t8[(L + l) * NB + (qq >> 3)]);
-        const float2 bb = side[(size_t)l * 2 * N + j];
-        const float2 u0 = cmul(bb, cmul(p0, e0)), u1 = cmul(bb, cmul(p1, e1));
-        z0[r].x += u0.x;
-        z0[r].y += u0.y;
-        z1[r].x += u1.x;
-        z1[r].y += u1.y;
-      }
+      const float2 c = cp[jq + r * NB];
+      z[r].x += c.x;
+      z[r].y += c.y;
     }
+  } else if ((N / 2) % NB == jq) {
+    const int r = (N / 2) / NB;
+    const float2 c = cp[N + k0];
+    z[r].x += c.x;
+    z[r].y += c.y;
   }
   int cnt = 0;
-  auto load = [&](int, int) {
-    const float2 a0 = z0[cnt], a1 = two ? z1[cnt] : make_float2(0.f, 0.f);
-    ++cnt;
-    return make_float2(a0.x - a1.y, a0.y + a1.x);
-  };
+  auto load = [&](int, int) { return z[cnt++]; };
   float2* out = work + (size_t)pr * plane + (size_t)row0 * N;
   auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
   wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
@@ -485,7 +545,7 @@ constexpr int FOLD_LMAX = 16;  // layers of the fused (paired) frame path
 // Workspace of the fused frame path (c64 elements): the paired intermediate,
 // the folded stack, its Nyquist side table, then the subharmonic tables.
 struct FramesLayout {
-  size_t fold, side, tab, total;
+  size_t fold, side, ptab, corr, tab, total;
 };
 static inline size_t r32(size_t v) { return (v + 31) & ~(size_t)31; }
 static FramesLayout frames_layout(int n, int F, int L, int P) {
@@ -493,7 +553,9 @@ static FramesLayout frames_layout(int n, int F, int L, int P) {
   const size_t plane = (size_t)n * n;
   w.fold = r32((size_t)((F + 1) / 2) * plane);
   w.side = w.fold + r32((size_t)L * plane);
-  w.tab = w.side + r32((size_t)L * 2 * n);
+  w.ptab = w.side + r32((size_t)L * 2 * n);
+  w.corr = w.ptab + r32((size_t)((F + 1) / 2) * 2 * L * (n / 8 + 9));
+  w.tab = w.corr + r32((size_t)((F + 1) / 2) * 2 * n);
   w.total = w.tab + sub_table_c64(P, n, F);
   return w;
 }
@@ -510,10 +572,13 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   const FramesLayout w = frames_layout(N, F, L, 0);
   float2* fold = work + w.fold;
   float2* side = work + w.side;
+  float2* ptab = work + w.ptab;
+  float2* corr = work + w.corr;
   k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
   PT_LAUNCH_CHECK();
-  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), RowsCfg<N>::T, dyn_r, st>>>(fold, side, L, disp, F,
-                                                                                inv_period, work);
+  k_frame_pair_tables<N><<<(F + 1) / 2, 256, 0, st>>>(disp, side, L, F, inv_period, ptab, corr);
+  PT_LAUNCH_CHECK();
+  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), RowsCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
