@@ -284,6 +284,11 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols_acc(GenArgs a) {
 //             = sum_l P_l(k) (A_l + B_l E_l(k))       on them.
 // fold[l] holds A_l + B_l (A_l on the Nyquist lines); side[l][j] holds B_l
 // on the Nyquist row (j = k1) and column (j = N + k0).
+//
+// Within each row the folded stack is stored in pass A's first-stage order:
+// column c = j + r * n/8 (j < n/8, r < 8) sits at position 8 j + r, so the
+// eight values a pass-A thread starts from are 64 contiguous bytes (four
+// 16-byte loads instead of eight 8-byte ones).
 __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ spectra, int n,
                                                     float2* __restrict__ fold, float2* __restrict__ side) {
   const size_t plane = (size_t)n * n;
@@ -295,12 +300,13 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
   const float2 sv = spectra[(size_t)l * plane + idx];
   const float2 sm = spectra[(size_t)l * plane + (size_t)m0 * n + m1];
   const float2 A = make_float2(0.5f * sv.x, 0.5f * sv.y), B = make_float2(0.5f * sm.x, -0.5f * sm.y);
-  const int h = n / 2;
+  const int h = n / 2, nb = n / 8;
+  const size_t dst = (size_t)l * plane + (size_t)k0 * n + (size_t)(k1 % nb) * 8 + k1 / nb;
   if (k0 == h || k1 == h) {
-    fold[(size_t)l * plane + idx] = A;
+    fold[dst] = A;
     side[(size_t)l * 2 * n + (k0 == h ? k1 : n + k0)] = B;
   } else {
-    fold[(size_t)l * plane + idx] = make_float2(A.x + B.x, A.y + B.y);
+    fold[dst] = make_float2(A.x + B.x, A.y + B.y);
   }
 }
 
@@ -417,7 +423,7 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
   const int m = tid / NB, jq = tid % NB;
   const int k0 = row0 + m;
   const size_t plane = (size_t)N * N;
-  const float2* fp = fold + (size_t)k0 * N + jq;
+  const float4* fp = (const float4*)(fold + (size_t)k0 * N + (size_t)jq * 8);  // first-stage order
   // z[r] accumulates the packed sum Herm X_t1 + i Herm X_t2 directly: per layer
   // and element, Q = P_1 + i P_2 (two phasor products folded into 8 FMAs),
   // then one complex multiply-accumulate z += g Q (4 FMAs).
@@ -434,7 +440,11 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
     const float2 cb0 = two ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
     float2 g[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) g[r] = fp[(size_t)l * plane + r * NB];
+    for (int h = 0; h < 4; ++h) {
+      const float4 v = fp[(size_t)l * (plane / 2) + h];
+      g[2 * h] = make_float2(v.x, v.y);
+      g[2 * h + 1] = make_float2(v.z, v.w);
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int qq = (jq + r * NB + N / 2) & (N - 1);
