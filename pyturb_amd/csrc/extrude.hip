@@ -7,9 +7,10 @@
 //
 //  * innovation terms for every new row of a step at once,
 //    BB = beta * B_unit^T (K x W), B streamed once per step;
-//  * one sequential round of the recurrence for up to 12 layers,
+//  * one sequential round of the recurrence for up to 16 layers,
 //    row = A * z + s_l * BB[k], A streamed once per round for all layers.
-//    Both are k_multi_gemv (a W x m*ld matrix against <= 12 vectors).
+//    Both are k_mfma_gemv (a W x m*ld matrix against <= 16 vectors on the
+//    FP32 matrix cores).
 // Rows have a pitch `ld` (multiple of 4 floats, zero padding) so every
 // matrix and ring row is 16-byte aligned.
 //  * k_readout    : rotated, sub-pixel, per-layer-shifted interpolation of
@@ -28,11 +29,11 @@ namespace pt {
 //                             (+ scale_e * add_e[w])
 // Both the innovation terms (M = B_unit, one segment, v = beta rows) and a
 // recurrence round (M = A, m segments, v = the m stencil rows of each layer)
-// are this shape: a W x (m*ld) matrix streamed once against up to 12 short
+// are this shape: a W x (m*ld) matrix streamed once against up to 16 short
 // vectors.  float4 loads (ld % 4 == 0, 16-byte aligned rows, zero padding
 // beyond the row width).
 // ---------------------------------------------------------------------------
-constexpr int VMAX = 12;  // vectors per launch
+constexpr int VMAX = 16;  // vectors per launch (one MFMA column tile)
 constexpr int MMAX = 4;   // stencil rows (segments)
 
 struct GemvArgs {
@@ -45,81 +46,74 @@ struct GemvArgs {
   float scale[VMAX];
 };
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+// C (W x nv) = M (W x m*ld) * Z (m*ld x nv) on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32: A lane l = M[l%16][l/16], B lane l = Z[l/16][l%16],
+// D lane l register r = C[4(l/16) + r][l%16]).  A workgroup owns 16 matrix
+// rows and all nv <= 16 vectors; its MF_WAVES waves split the K = m*ld
+// columns in chunks of 16.  In a chunk, lane l loads the float4 of row l%16
+// at k = 16c + 4(l/16) and the same float4 of vector l%16; the four MFMAs
+// (one per float4 component) then cover the chunk with k permuted inside it,
+// identically for both operands, so the products pair up exactly.  Each
+// vector value is reused for 16 matrix rows.  Partial tiles are summed
+// across waves through LDS, then out[e][row] = C + scale[e] * add[e][row].
+constexpr int MF_WAVES = 16;
+typedef float mfma_v4f __attribute__((ext_vector_type(4)));
 
-// A workgroup owns GR output rows: its 256 threads split the rows'
-// m*ld/4 float4 columns, so each vector float4 (from L2) is reused for GR
-// matrix rows and every thread has GR+1 independent loads per column in
-// flight; partial sums are reduced through LDS.
-constexpr int GR = 4;
-
-__global__ __launch_bounds__(256) void k_multi_gemv(GemvArgs a) {
-  __shared__ float part[4][GR][VMAX];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int row0 = blockIdx.x * GR;
-  const int nq = a.ld >> 2;  // float4 per segment
-  float acc[GR][VMAX];
-#pragma unroll
-  for (int r = 0; r < GR; ++r)
-#pragma unroll
-    for (int e = 0; e < VMAX; ++e) acc[r][e] = 0.f;
+__global__ __launch_bounds__(64 * MF_WAVES) void k_mfma_gemv(GemvArgs a) {
+  __shared__ float red[MF_WAVES][16][17];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row0 = blockIdx.x * 16;
+  const int ri = lane & 15, kq = lane >> 4;
+  const bool live = ri < a.nv;  // this lane's vector (B operand column)
+  const float* arow = a.M + (size_t)min(row0 + ri, a.W - 1) * a.ldm;  // tail rows recompute the last row
+  const int cps = (a.ld + 15) >> 4;  // 16-wide chunks per segment (ld % 4 == 0; k >= ld reads as zero)
+  mfma_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int s = 0; s < MMAX; ++s) {
     if (s >= a.m) break;
-    const float4* Ms[GR];
+    const float* zs = nullptr;
 #pragma unroll
-    for (int r = 0; r < GR; ++r) {
-      const int row = min(row0 + r, a.W - 1);  // tail rows recompute the last row (not stored)
-      Ms[r] = (const float4*)(a.M + (size_t)row * a.ldm + (size_t)s * a.ld);
-    }
-    for (int q = tid; q < nq; q += 256) {
-      float4 x[GR];
-#pragma unroll
-      for (int r = 0; r < GR; ++r) x[r] = Ms[r][q];
-#pragma unroll
-      for (int e = 0; e < VMAX; ++e) {
-        if (e < a.nv) {
-          const float4 z = ((const float4*)a.v[e][s])[q];
-#pragma unroll
-          for (int r = 0; r < GR; ++r)
-            acc[r][e] = fmaf(x[r].x, z.x, fmaf(x[r].y, z.y, fmaf(x[r].z, z.z, fmaf(x[r].w, z.w, acc[r][e]))));
-        }
-      }
+    for (int e = 0; e < VMAX; ++e)
+      if (e == ri) zs = a.v[e][s];
+    const float* as = arow + (size_t)s * a.ld;
+    for (int c = wave; c < cps; c += MF_WAVES) {
+      const int k = 16 * c + 4 * kq;
+      const bool in = k < a.ld;
+      const float4 x = in ? *(const float4*)(as + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 z = in && live ? *(const float4*)(zs + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z.w, acc, 0, 0, 0);
     }
   }
 #pragma unroll
-  for (int r = 0; r < GR; ++r)
-#pragma unroll
-    for (int e = 0; e < VMAX; ++e) {
-      if (e < a.nv) {
-        const float t = wave_sum(acc[r][e]);
-        if (lane == 0) part[wave][r][e] = t;
-      }
-    }
+  for (int r = 0; r < 4; ++r) red[wave][4 * kq + r][ri] = acc[r];
   __syncthreads();
-  const int r = tid / VMAX, e = tid % VMAX;
-  if (r < GR && e < a.nv && row0 + r < a.W) {
-    const float sum = (part[0][r][e] + part[1][r][e]) + (part[2][r][e] + part[3][r][e]);
-    float* out = nullptr;
-    const float* add = nullptr;
-    float sc = 0.f;
+  const int t = threadIdx.x;
+  if (t < 256) {
+    const int rr = t >> 4, e = t & 15;
+    if (e < a.nv && row0 + rr < a.W) {
+      float sum = 0.f;
 #pragma unroll
-    for (int k = 0; k < VMAX; ++k)
-      if (e == k) {
-        out = a.out[k];
-        add = a.add[k];
-        sc = a.scale[k];
-      }
-    out[row0 + r] = sum + (add ? sc * add[row0 + r] : 0.f);
+      for (int w = 0; w < MF_WAVES; ++w) sum += red[w][rr][e];
+      float* out = nullptr;
+      const float* add = nullptr;
+      float sc = 0.f;
+#pragma unroll
+      for (int k = 0; k < VMAX; ++k)
+        if (e == k) {
+          out = a.out[k];
+          add = a.add[k];
+          sc = a.scale[k];
+        }
+      out[row0 + rr] = sum + (add ? sc * add[row0 + rr] : 0.f);
+    }
   }
 }
 
 static int launch_gemv(GemvArgs& a, hipStream_t st) {
-  k_multi_gemv<<<(a.W + GR - 1) / GR, 256, 0, st>>>(a);
+  k_mfma_gemv<<<(a.W + 15) / 16, 64 * MF_WAVES, 0, st>>>(a);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -359,20 +353,57 @@ __global__ __launch_bounds__(256) void k_readout(ReadoutArgs a) {
   }
 }
 
-// Box-staged readout for n >= 32: a workgroup owns a 32 x 32 pixel tile.
-// For each (layer, direction) the tile's footprint in the ring buffer is an
-// affine image of the tile; its bounding box plus the tap margins is loaded
-// into LDS with coalesced row reads (rows clamped to the valid ring rows and
-// columns to [0, W-1] -- exactly the per-tap clamping), then the taps are
-// read from LDS.  A footprint larger than the box (magnification > ~1.1)
-// falls back to the direct gather for that (layer, direction).
 constexpr int TB = 32;    // tile edge
 constexpr int BOXE = 60;  // box edge: 31 * sqrt(2) + taps + 2 spare
+
+// Footprint box of one (layer, direction) for a tile: local ring rows rb0..
+// and buffer columns cb0.. (before clamping), extents nr x nc.
+struct BoxGeom {
+  int rb0, cb0, nr, nc;
+  bool fits;
+};
+
+template <int TAPS>
+__device__ __forceinline__ BoxGeom box_geom(const ReadoutArgs& a, int d, int l, int i0, int j0, int ti1, int tj1,
+                                            float h) {
+  constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
+  const float c = a.cosw[l], s = a.sinw[l];
+  const float mg = a.mag[d][l], rfr = a.r_frac[d][l], cfr = a.c_frac[d][l];
+  float rmin = 3e38f, rmax = -3e38f, cmin = 3e38f, cmax = -3e38f;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {  // affine map: the extremes are at the corners
+    const float gi = ((float)(q & 1 ? ti1 : i0) - h) * mg, gj = ((float)(q & 2 ? tj1 : j0) - h) * mg;
+    const float r = fmaf(gi, c, fmaf(gj, s, rfr)), cc = fmaf(-gi, s, fmaf(gj, c, cfr));
+    rmin = fminf(rmin, r);
+    rmax = fmaxf(rmax, r);
+    cmin = fminf(cmin, cc);
+    cmax = fmaxf(cmax, cc);
+  }
+  // one spare row/column each side absorbs last-bit rounding of interior pixels
+  BoxGeom b;
+  b.rb0 = (int)floorf(rmin) + OFF - 1;
+  b.cb0 = (int)floorf(cmin) + OFF - 1;
+  b.nr = (int)floorf(rmax) + OFF + TAPS + 1 - b.rb0;
+  b.nc = (int)floorf(cmax) + OFF + TAPS + 1 - b.cb0;
+  b.fits = b.nr <= BOXE && b.nc <= BOXE;
+  return b;
+}
+
+// Box-staged readout for n >= 32: a workgroup owns a 32 x 32 pixel tile of
+// one direction.  For each layer the tile's footprint in the ring buffer
+// (an affine image of the tile) plus the tap margins is staged in LDS with
+// coalesced row reads, rows clamped to the valid ring rows and columns to
+// [0, W-1] exactly as the per-tap clamp does; the taps are then read from
+// LDS.  The box of layer l + 1 is loaded into registers while layer l is
+// interpolated (two LDS buffers), so the gather latency overlaps the
+// arithmetic.  A footprint larger than the box (magnification > ~1.1) falls
+// back to the direct gather for that layer.
+constexpr int BOX_ROWS = (BOXE + 3) / 4;  // box rows per wave
 
 template <int TAPS>
 __global__ __launch_bounds__(256) void k_readout_box(ReadoutArgs a) {
   constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
-  __shared__ float box[BOXE][BOXE + 1];
+  __shared__ float box[2][BOXE][BOXE + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int i0 = blockIdx.y * TB, j0 = blockIdx.x * TB;
   const int jl = tid & 31, il = tid >> 5;  // pixel (i0 + il + 8k, j0 + jl), k < 4
@@ -380,78 +411,90 @@ __global__ __launch_bounds__(256) void k_readout_box(ReadoutArgs a) {
   const int ti1 = min(i0 + TB, a.n) - 1, tj1 = min(j0 + TB, a.n) - 1;
   const int d = blockIdx.z;  // one direction per block: short dependency chains
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int l = 0; l < a.L; ++l) {
-    const float c = a.cosw[l], s = a.sinw[l];
+  float pre[BOX_ROWS];
+  // box loads of layer l into registers (nothing when it does not fit)
+  auto fetch = [&](int l, const BoxGeom& b) {
+    if (!b.fits || lane >= b.nc) return;
     const float* bl = a.buf + (size_t)l * a.cap * a.ld;
     const int fm = a.fillm1[l], st = a.start[l];
-    {
-      const float mg = a.mag[d][l], rfr = a.r_frac[d][l], cfr = a.c_frac[d][l];
-      const int rint = a.r_int[d][l], cint = a.c_int[d][l];
-      // footprint of the tile (affine map: extremes at the corners)
-      float rmin = 3e38f, rmax = -3e38f, cmin = 3e38f, cmax = -3e38f;
+    const int cc = min(max(b.cb0 + a.c_int[d][l] + lane, 0), a.W - 1);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float gi = ((float)(q & 1 ? ti1 : i0) - h) * mg, gj = ((float)(q & 2 ? tj1 : j0) - h) * mg;
-        const float r = fmaf(gi, c, fmaf(gj, s, rfr)), cc = fmaf(-gi, s, fmaf(gj, c, cfr));
-        rmin = fminf(rmin, r);
-        rmax = fmaxf(rmax, r);
-        cmin = fminf(cmin, cc);
-        cmax = fmaxf(cmax, cc);
+    for (int k = 0; k < BOX_ROWS; ++k) {
+      const int br = wave + 4 * k;
+      if (br < b.nr) {
+        int ph = st + min(max(b.rb0 + a.r_int[d][l] + br, 0), fm);
+        ph = ph >= a.cap ? ph - a.cap : ph;
+        pre[k] = bl[(size_t)ph * a.ld + cc];
       }
-      // one spare row/column each side absorbs last-bit rounding of interior pixels
-      const int rb0 = (int)floorf(rmin) + OFF - 1, cb0 = (int)floorf(cmin) + OFF - 1;
-      const int nr = (int)floorf(rmax) + OFF + TAPS + 1 - rb0;
-      const int nc = (int)floorf(cmax) + OFF + TAPS + 1 - cb0;
-      const bool fits = nr <= BOXE && nc <= BOXE;  // uniform across the block
-      if (fits) {
-        __syncthreads();  // previous box fully consumed
-        if (lane < nc) {
-          const int cc = min(max(cb0 + cint + lane, 0), a.W - 1);
-          for (int br = wave; br < nr; br += 4) {
-            int ph = st + min(max(rb0 + rint + br, 0), fm);
-            ph = ph >= a.cap ? ph - a.cap : ph;
-            box[br][lane] = bl[ph * a.ld + cc];
-          }
+    }
+  };
+  auto commit = [&](int sel, const BoxGeom& b) {
+    if (!b.fits || lane >= b.nc) return;
+#pragma unroll
+    for (int k = 0; k < BOX_ROWS; ++k) {
+      const int br = wave + 4 * k;
+      if (br < b.nr) box[sel][br][lane] = pre[k];
+    }
+  };
+  BoxGeom cur = box_geom<TAPS>(a, d, 0, i0, j0, ti1, tj1, h);
+  fetch(0, cur);
+  commit(0, cur);
+  __syncthreads();
+  for (int l = 0; l < a.L; ++l) {
+    const int sel = l & 1;
+    BoxGeom nxt;
+    nxt.fits = false;
+    if (l + 1 < a.L) {
+      nxt = box_geom<TAPS>(a, d, l + 1, i0, j0, ti1, tj1, h);
+      fetch(l + 1, nxt);  // in flight while layer l is interpolated
+    }
+    const float c = a.cosw[l], s = a.sinw[l];
+    const float mg = a.mag[d][l], rfr = a.r_frac[d][l], cfr = a.c_frac[d][l];
+    const int rint = a.r_int[d][l], cint = a.c_int[d][l];
+    const float* bl = a.buf + (size_t)l * a.cap * a.ld;
+    const int fm = a.fillm1[l], st = a.start[l];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = i0 + il + 8 * k, j = j0 + jl;
+      if (i >= a.n || j >= a.n) continue;
+      const float gi = ((float)i - h) * mg, gj = ((float)j - h) * mg;
+      const float row = fmaf(gi, c, fmaf(gj, s, rfr));
+      const float col = fmaf(-gi, s, fmaf(gj, c, cfr));
+      const float rf = floorf(row), cf = floorf(col);
+      float wr[TAPS], wc[TAPS];
+      taps_weights<TAPS>(row - rf, wr);
+      taps_weights<TAPS>(col - cf, wc);
+      float val = 0.f;
+      if (cur.fits) {
+        const int br = min(max((int)rf + OFF - cur.rb0, 0), cur.nr - TAPS);
+        const int bc = min(max((int)cf + OFF - cur.cb0, 0), cur.nc - TAPS);
+#pragma unroll
+        for (int u = 0; u < TAPS; ++u) {
+          float rs = 0.f;
+#pragma unroll
+          for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], box[sel][br + u][bc + v], rs);
+          val = fmaf(wr[u], rs, val);
         }
-        __syncthreads();
-      }
+      } else {
+        const int r0 = (int)rf + rint, c0 = (int)cf + cint;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int i = i0 + il + 8 * k, j = j0 + jl;
-        if (i >= a.n || j >= a.n) continue;
-        const float gi = ((float)i - h) * mg, gj = ((float)j - h) * mg;
-        const float row = fmaf(gi, c, fmaf(gj, s, rfr));
-        const float col = fmaf(-gi, s, fmaf(gj, c, cfr));
-        const float rf = floorf(row), cf = floorf(col);
-        float wr[TAPS], wc[TAPS];
-        taps_weights<TAPS>(row - rf, wr);
-        taps_weights<TAPS>(col - cf, wc);
-        float val = 0.f;
-        if (fits) {
-          const int br = min(max((int)rf + OFF - rb0, 0), nr - TAPS);
-          const int bc = min(max((int)cf + OFF - cb0, 0), nc - TAPS);
+        for (int u = 0; u < TAPS; ++u) {
+          int ph = st + min(max(r0 + OFF + u, 0), fm);
+          ph = ph >= a.cap ? ph - a.cap : ph;
+          const float* brow = bl + (size_t)ph * a.ld;
+          float rs = 0.f;
 #pragma unroll
-          for (int u = 0; u < TAPS; ++u) {
-            float rs = 0.f;
-#pragma unroll
-            for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], box[br + u][bc + v], rs);
-            val = fmaf(wr[u], rs, val);
-          }
-        } else {
-          const int r0 = (int)rf + rint, c0 = (int)cf + cint;
-#pragma unroll
-          for (int u = 0; u < TAPS; ++u) {
-            int ph = st + min(max(r0 + OFF + u, 0), fm);
-            ph = ph >= a.cap ? ph - a.cap : ph;
-            const float* brow = bl + ph * a.ld;
-            float rs = 0.f;
-#pragma unroll
-            for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], brow[min(max(c0 + OFF + v, 0), a.W - 1)], rs);
-            val = fmaf(wr[u], rs, val);
-          }
+          for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], brow[min(max(c0 + OFF + v, 0), a.W - 1)], rs);
+          val = fmaf(wr[u], rs, val);
         }
-        acc[k] += val;
       }
+      acc[k] += val;
+    }
+    if (l + 1 < a.L) {
+      // buffer sel ^ 1 was last read in iteration l - 1, before the previous barrier
+      commit(sel ^ 1, nxt);
+      __syncthreads();
+      cur = nxt;
     }
   }
 #pragma unroll
