@@ -390,9 +390,10 @@ def main(argv=None):
     t_a = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) * 1e-3
     t_b = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) * 1e-3
     # per-frame pass figures: pass A reads the 8L n^2 B stack and writes an
-    # 8 n^2 B intermediate, pass B reads that and writes the 4 n^2 B frame
+    # 8 n^2 B intermediate; pass B reads that intermediate (one c64 grid per
+    # frame pair, 4 n^2 B per frame) and writes the 4 n^2 B frame
     bytes_a = (8.0 * L + 8.0) * N * N * N_FRAMES
-    bytes_b = 12.0 * N * N * N_FRAMES
+    bytes_b = 8.0 * N * N * N_FRAMES
     ach_a = bytes_a / t_a / 1e9
     frame_bytes = (8.0 * L + 4.0) * N * N
     print(f"bench: frames {fps:.0f}/s pass A {t_a*1e3:.3f} ms pass B {t_b*1e3:.3f} ms",

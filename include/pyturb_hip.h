@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 1
+#define PT_ABI_VERSION 2
 
 #define PT_OK 0
 #define PT_ERR_ARG 1         /* invalid argument (shape, count, NULL)      */
@@ -164,15 +164,18 @@ int pt_spectral_cols(void* work, int layers, int n, double pixel_scale,
  * of 4 floats, with zero padding past `width`; arrays are 16-byte aligned.
  *
  * Innovation terms of the Assemat-Wilson recurrence for a batch of new rows:
- *   out[k][w] = sum_j B_unit[w][j] * beta[k][j]      (out = beta * B^T)
- *   beta (rows, ld) f32 white noise, B_unit (width, ld) f32, out (rows, ld).
+ *   out[k][w] = sum_j B_unit[w][j] * beta_k[j]      (out = beta * B^T)
+ *   beta_rows: HOST array of `rows` device pointers, each to `width`
+ *   contiguous f32 white-noise values (4-byte aligned, no padding needed),
+ *   so every layer's innovations are read where the caller drew them;
+ *   B_unit (width, ld) f32, out (rows, ld) f32.
  *   The per-layer r0^(-5/6) factor is applied by pt_extrude_rows.
  * Replaces the B @ beta half of src/pyturb/extrude.py:427-429 and
  * src/pyturb/infinite.py:278-280, batched over every row of a step (as
  * extrude.py:686-689 does for boiling).
  */
-int pt_innovation_gemm(const float* beta, const float* B_unit, int rows,
-                       int width, int ld, float* out, void* stream);
+int pt_innovation_gemm(const float* const* beta_rows, const float* B_unit,
+                       int rows, int width, int ld, float* out, void* stream);
 
 /*
  * One round of the row recurrence for `nact` ring buffers:

@@ -26,6 +26,11 @@ Pinning (how each function is checked, see ``tests/test_oracle.py``):
 * ``compact_row_ring`` / ``OracleExtruder`` / ``readout``: golden readouts
   from a 2-layer n=32 extruder, exactness at integer travel and
   "one big jump == many small steps".
+* ``noll_to_zernike`` / ``zernike_basis`` / ``zernike_decompose`` /
+  ``noll_residual`` / ``noll_variance``: the reference's known-answer index
+  table, orthonormality over the pupil, exact recovery of a synthetic
+  combination of modes, the radial polynomials' closed forms, and Noll's
+  table values with the large-J asymptote.
 """
 
 from __future__ import annotations
@@ -67,6 +72,11 @@ __all__ = [
     "structure_function",
     "r0_from_seeing",
     "rel_rms",
+    "noll_to_zernike",
+    "zernike_basis",
+    "zernike_decompose",
+    "noll_residual",
+    "noll_variance",
 ]
 
 
@@ -661,3 +671,92 @@ class OracleInfinite:
         for d, w in zip(offs, ws):
             out = out + w * self.buf[np.clip(i0 + d, 0, self.fill - 1)]
         return out
+
+
+# ---------------------------------------------------------------------------
+# Zernike modes and Noll (1976) variances, for the validation statistics
+# ---------------------------------------------------------------------------
+# Noll (1976) Table IV: residual variance after removing the first J modes, in
+# (D/r0)^(5/3) rad^2 (analysis.py:40-49); beyond the table the asymptote
+# 0.2944 J^(-sqrt(3)/2) (analysis.py:51-58).
+_NOLL_TABLE = (1.0299, 0.582, 0.134, 0.111, 0.0880, 0.0648, 0.0587, 0.0525, 0.0463, 0.0401,
+               0.0377, 0.0352, 0.0328, 0.0304, 0.0279, 0.0267, 0.0255, 0.0243, 0.0232,
+               0.0220, 0.0208)
+
+
+def noll_to_zernike(j: int) -> Tuple[int, int]:
+    """analysis.py:61-72 -- Noll index j >= 1 to (radial n, azimuthal m).
+
+    Modes are counted row by row of the radial order n (n + 1 modes each);
+    inside a row |m| rises in steps of 2 from n mod 2, even j takes +|m|
+    (cosine), odd j takes -|m| (sine)."""
+    if j < 1:
+        raise ValueError("Noll index j must be >= 1")
+    n, first = 0, 1  # first Noll index of radial order n
+    while first + n + 1 <= j:
+        first += n + 1
+        n += 1
+    k = j - first  # position inside the row
+    mag = (n % 2) + 2 * ((k + (n + 1) % 2) // 2)
+    return n, (mag if j % 2 == 0 else -mag)
+
+
+def _zernike_radial(n: int, m: int, rho):
+    """analysis.py:75-86 -- R_n^|m|(rho) by its finite sum."""
+    m = abs(m)
+    out = np.zeros_like(rho)
+    for k in range((n - m) // 2 + 1):
+        c = (-1) ** k * math.factorial(n - k) / (
+            math.factorial(k) * math.factorial((n + m) // 2 - k) * math.factorial((n - m) // 2 - k))
+        out = out + c * rho ** (n - 2 * k)
+    return out
+
+
+def zernike_basis(n_modes: int, n_pixels: int, diameter_pixels: Optional[float] = None):
+    """analysis.py:89-136 -- (n_modes, n, n) Noll-normalised modes on the
+    inscribed disc (zero outside); pixel centres at (i - (n-1)/2) / radius."""
+    if n_modes < 1 or n_pixels < 2:
+        raise ValueError("n_modes >= 1 and n_pixels >= 2 required")
+    radius = 0.5 * (n_pixels if diameter_pixels is None else diameter_pixels)
+    c = (np.arange(n_pixels) - 0.5 * (n_pixels - 1)) / radius
+    x, y = c[:, None], c[None, :]
+    rho = np.sqrt(x * x + y * y)
+    theta = np.arctan2(np.broadcast_to(y, rho.shape), np.broadcast_to(x, rho.shape))
+    inside = rho <= 1.0
+    out = np.zeros((n_modes, n_pixels, n_pixels))
+    for i in range(n_modes):
+        n, m = noll_to_zernike(i + 1)
+        ang = np.cos(m * theta) if m >= 0 else np.sin(-m * theta)
+        scale = math.sqrt(n + 1.0) * (math.sqrt(2.0) if m != 0 else 1.0)
+        out[i] = np.where(inside, scale * _zernike_radial(n, m, rho) * ang, 0.0)
+    return out
+
+
+def zernike_decompose(phase, n_modes: int, basis=None):
+    """analysis.py:139-162 -- least-squares coefficients over the pupil
+    (the support of piston); (count, n_modes) for a stack."""
+    phase = np.asarray(phase, dtype=np.float64)
+    single = phase.ndim == 2
+    stack = phase[None] if single else phase
+    if basis is None:
+        basis = zernike_basis(n_modes, stack.shape[-1])
+    pupil = basis[0] != 0
+    design = basis[:, pupil].T
+    coeffs = np.linalg.lstsq(design, stack[:, pupil].T, rcond=None)[0].T
+    return coeffs[0] if single else coeffs
+
+
+def noll_residual(j: int) -> float:
+    """analysis.py:51-58 -- Delta_J in (D/r0)^(5/3) units."""
+    if j < 1:
+        raise ValueError("Noll index j must be >= 1")
+    if j <= len(_NOLL_TABLE):
+        return _NOLL_TABLE[j - 1]
+    return 0.2944 * j ** (-math.sqrt(3.0) / 2.0)
+
+
+def noll_variance(j: int, diameter: float, r0: float) -> float:
+    """analysis.py:165-174 -- Kolmogorov variance of mode j >= 2 [rad^2]."""
+    if j < 2:
+        raise ValueError("piston (j=1) has no finite variance; use j >= 2")
+    return (noll_residual(j - 1) - noll_residual(j)) * (diameter / r0) ** (5.0 / 3.0)

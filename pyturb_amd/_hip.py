@@ -24,6 +24,7 @@ PT_OK = 0
 PT_ERR_ARG = 1
 PT_ERR_HIP = 2
 PT_ERR_UNSUPPORTED = 3
+ABI_VERSION = 2  # PT_ABI_VERSION of include/pyturb_hip.h
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -76,6 +77,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         fn = getattr(so, name)
         fn.restype = res
         fn.argtypes = args
+    if so.pt_abi_version() != ABI_VERSION:
+        raise ImportError(f"{path} has ABI version {so.pt_abi_version()}, this package needs "
+                          f"{ABI_VERSION}: rebuild it with `python -m pyturb_amd.build`")
     return so
 
 

@@ -89,4 +89,15 @@ __device__ __forceinline__ double sh_block_mean(const double2 C[9], double2 mp) 
   return acc;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remapping the flat workgroup index with xcd_tile gives every XCD a
+// contiguous run of logical tiles, so tiles that share cache lines (column
+// tiles of one grid) meet in one L2 instead of being fetched by several.  A
+// bijection on [0, nx) for any nx.
+__device__ __forceinline__ int xcd_tile(int bx, int nx) {
+  const int xcd = bx & 7, k = bx >> 3;
+  const int q = nx >> 3, r = nx & 7;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
+}
+
 }  // namespace pt

@@ -7,9 +7,9 @@
 //
 //  * innovation terms for every new row of a step at once,
 //    BB = beta * B_unit^T (K x W), B streamed once per step;
-//  * one sequential round of the recurrence for up to 16 layers,
+//  * one sequential round of the recurrence for up to 32 layers,
 //    row = A * z + s_l * BB[k], A streamed once per round for all layers.
-//    Both are k_mfma_gemv (a W x m*ld matrix against <= 16 vectors on the
+//    Both are k_mfma_gemv (a W x m*ld matrix against <= 32 vectors on the
 //    FP32 matrix cores).
 // Rows have a pitch `ld` (multiple of 4 floats, zero padding) so every
 // matrix and ring row is 16-byte aligned.
@@ -29,17 +29,19 @@ namespace pt {
 //                             (+ scale_e * add_e[w])
 // Both the innovation terms (M = B_unit, one segment, v = beta rows) and a
 // recurrence round (M = A, m segments, v = the m stencil rows of each layer)
-// are this shape: a W x (m*ld) matrix streamed once against up to 16 short
+// are this shape: a W x (m*ld) matrix streamed once against up to 32 short
 // vectors.  float4 loads (ld % 4 == 0, 16-byte aligned rows, zero padding
 // beyond the row width).
 // ---------------------------------------------------------------------------
-constexpr int VMAX = 16;  // vectors per launch (one MFMA column tile)
-constexpr int MMAX = 4;   // stencil rows (segments)
+constexpr int VT = 16;         // vectors per MFMA column tile
+constexpr int VMAX = 2 * VT;   // vectors per launch
+constexpr int MMAX = 4;        // stencil rows (segments)
 
 struct GemvArgs {
   const float* M;
   long long ldm;  // matrix row pitch (m * ld)
   int W, ld, m, nv;
+  int zw;  // ZV4 == false: valid floats per vector (4-byte aligned, no padding)
   const float* v[VMAX][MMAX];
   float* out[VMAX];
   const float* add[VMAX];
@@ -49,54 +51,77 @@ struct GemvArgs {
 // C (W x nv) = M (W x m*ld) * Z (m*ld x nv) on the FP32 matrix cores
 // (v_mfma_f32_16x16x4f32: A lane l = M[l%16][l/16], B lane l = Z[l/16][l%16],
 // D lane l register r = C[4(l/16) + r][l%16]).  A workgroup owns 16 matrix
-// rows and all nv <= 16 vectors; its MF_WAVES waves split the K = m*ld
-// columns in chunks of 16.  In a chunk, lane l loads the float4 of row l%16
-// at k = 16c + 4(l/16) and the same float4 of vector l%16; the four MFMAs
-// (one per float4 component) then cover the chunk with k permuted inside it,
-// identically for both operands, so the products pair up exactly.  Each
-// vector value is reused for 16 matrix rows.  Partial tiles are summed
-// across waves through LDS, then out[e][row] = C + scale[e] * add[e][row].
+// rows and NT column tiles of 16 vectors (nv <= 16 NT); its MF_WAVES waves
+// split the K = m*ld columns in chunks of 16.  In a chunk, lane l loads the
+// float4 of row l%16 at k = 16c + 4(l/16) and the same float4 of vector
+// l%16 of each tile; the four MFMAs per tile (one per float4 component) then
+// cover the chunk with k permuted inside it, identically for both operands,
+// so the products pair up exactly.  Each matrix value is read once per
+// launch for up to 16 NT vectors, each vector value is reused for 16 matrix
+// rows.  Partial tiles are summed across waves through LDS, then
+// out[e][row] = C + scale[e] * add[e][row].
+// ZV4: vectors are 16-byte aligned and zero-padded to ld (ring rows), else
+// plain rows of zw floats read element by element (caller's innovations).
 constexpr int MF_WAVES = 16;
 typedef float mfma_v4f __attribute__((ext_vector_type(4)));
 
+template <bool ZV4, int NT>
 __global__ __launch_bounds__(64 * MF_WAVES) void k_mfma_gemv(GemvArgs a) {
-  __shared__ float red[MF_WAVES][16][17];
+  __shared__ float red[NT][MF_WAVES][16][17];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row0 = blockIdx.x * 16;
   const int ri = lane & 15, kq = lane >> 4;
-  const bool live = ri < a.nv;  // this lane's vector (B operand column)
   const float* arow = a.M + (size_t)min(row0 + ri, a.W - 1) * a.ldm;  // tail rows recompute the last row
   const int cps = (a.ld + 15) >> 4;  // 16-wide chunks per segment (ld % 4 == 0; k >= ld reads as zero)
-  mfma_v4f acc = {0.f, 0.f, 0.f, 0.f};
+  mfma_v4f acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int s = 0; s < MMAX; ++s) {
     if (s >= a.m) break;
-    const float* zs = nullptr;
+    const float* zs[NT];
 #pragma unroll
-    for (int e = 0; e < VMAX; ++e)
-      if (e == ri) zs = a.v[e][s];
+    for (int j = 0; j < NT; ++j) {
+      zs[j] = nullptr;
+#pragma unroll
+      for (int e = 0; e < VT; ++e)
+        if (e == ri) zs[j] = a.v[j * VT + e][s];
+    }
     const float* as = arow + (size_t)s * a.ld;
     for (int c = wave; c < cps; c += MF_WAVES) {
       const int k = 16 * c + 4 * kq;
       const bool in = k < a.ld;
       const float4 x = in ? *(const float4*)(as + k) : make_float4(0.f, 0.f, 0.f, 0.f);
-      const float4 z = in && live ? *(const float4*)(zs + k) : make_float4(0.f, 0.f, 0.f, 0.f);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z.x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z.y, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z.z, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z.w, acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const bool live = j * VT + ri < a.nv;  // this lane's vector (B operand column)
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ZV4) {
+          if (in && live) z = *(const float4*)(zs[j] + k);
+        } else if (live) {
+          if (k < a.zw) z.x = zs[j][k];
+          if (k + 1 < a.zw) z.y = zs[j][k + 1];
+          if (k + 2 < a.zw) z.z = zs[j][k + 2];
+          if (k + 3 < a.zw) z.w = zs[j][k + 3];
+        }
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z.x, acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z.y, acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z.z, acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z.w, acc[j], 0, 0, 0);
+      }
     }
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) red[wave][4 * kq + r][ri] = acc[r];
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[j][wave][4 * kq + r][ri] = acc[j][r];
   __syncthreads();
-  const int t = threadIdx.x;
-  if (t < 256) {
-    const int rr = t >> 4, e = t & 15;
+  for (int q = threadIdx.x; q < NT * 256; q += 64 * MF_WAVES) {
+    const int j = q >> 8, rr = (q >> 4) & 15, e = j * VT + (q & 15);
     if (e < a.nv && row0 + rr < a.W) {
       float sum = 0.f;
 #pragma unroll
-      for (int w = 0; w < MF_WAVES; ++w) sum += red[w][rr][e];
+      for (int w = 0; w < MF_WAVES; ++w) sum += red[j][w][rr][q & 15];
       float* out = nullptr;
       const float* add = nullptr;
       float sc = 0.f;
@@ -112,8 +137,19 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_mfma_gemv(GemvArgs a) {
   }
 }
 
-static int launch_gemv(GemvArgs& a, hipStream_t st) {
-  k_mfma_gemv<<<(a.W + 15) / 16, 64 * MF_WAVES, 0, st>>>(a);
+static int launch_gemv(GemvArgs& a, bool zv4, hipStream_t st) {
+  const int g = (a.W + 15) / 16, b = 64 * MF_WAVES;
+  if (a.nv > VT) {
+    if (zv4)
+      k_mfma_gemv<true, 2><<<g, b, 0, st>>>(a);
+    else
+      k_mfma_gemv<false, 2><<<g, b, 0, st>>>(a);
+  } else {
+    if (zv4)
+      k_mfma_gemv<true, 1><<<g, b, 0, st>>>(a);
+    else
+      k_mfma_gemv<false, 1><<<g, b, 0, st>>>(a);
+  }
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -511,11 +547,14 @@ __global__ __launch_bounds__(256) void k_readout_box(ReadoutArgs a) {
 
 using namespace pt;
 
-extern "C" int pt_innovation_gemm(const float* beta, const float* B, int rows, int width, int ld, float* out,
-                                  void* stream) {
-  if (!beta || !B || !out) return fail(PT_ERR_ARG, "pt_innovation_gemm: NULL array");
+extern "C" int pt_innovation_gemm(const float* const* beta_rows, const float* B, int rows, int width, int ld,
+                                  float* out, void* stream) {
+  if (!beta_rows || !B || !out) return fail(PT_ERR_ARG, "pt_innovation_gemm: NULL array");
   if (rows < 0 || width < 1 || ld < width || (ld & 3)) return fail(PT_ERR_ARG, "pt_innovation_gemm: bad sizes (ld %% 4)");
-  if (!aligned16(beta) || !aligned16(B) || !aligned16(out)) return fail(PT_ERR_ARG, "pt_innovation_gemm: arrays must be 16-byte aligned");
+  if (!aligned16(B) || !aligned16(out)) return fail(PT_ERR_ARG, "pt_innovation_gemm: B and out must be 16-byte aligned");
+  for (int k = 0; k < rows; ++k)
+    if (!beta_rows[k] || ((uintptr_t)beta_rows[k] & 3))
+      return fail(PT_ERR_ARG, "pt_innovation_gemm: innovation row %d NULL or not 4-byte aligned", k);
   hipStream_t st = (hipStream_t)stream;
   for (int k0 = 0; k0 < rows; k0 += VMAX) {
     GemvArgs a = {};
@@ -524,15 +563,16 @@ extern "C" int pt_innovation_gemm(const float* beta, const float* B, int rows, i
     a.W = width;
     a.ld = ld;
     a.m = 1;
+    a.zw = width;
     a.nv = rows - k0 < VMAX ? rows - k0 : VMAX;
     for (int e = 0; e < VMAX; ++e) {
       const int k = k0 + (e < a.nv ? e : 0);
-      a.v[e][0] = beta + (size_t)k * ld;
+      a.v[e][0] = beta_rows[k];
       a.out[e] = out + (size_t)k * ld;
       a.add[e] = nullptr;
       a.scale[e] = 0.f;
     }
-    int rc = launch_gemv(a, st);
+    int rc = launch_gemv(a, false, st);
     if (rc) return rc;
   }
   return PT_OK;
@@ -569,7 +609,7 @@ extern "C" int pt_extrude_rows(const float* A, int width, int ld, int stencil_ro
       a.add[e] = bb + (size_t)bbrow[src] * ld;
       a.scale[e] = bscale[src];
     }
-    int rc = launch_gemv(a, st);
+    int rc = launch_gemv(a, true, st);
     if (rc) return rc;
   }
   return PT_OK;
@@ -644,7 +684,7 @@ extern "C" int pt_readout(const float* buf, int layers, long long cap, int width
       a.scale = scale;
       const dim3 g((n + 31) / 32, (n + 7) / 8, a.D);
       if (n >= TB) {
-        dim3 gb((n + TB - 1) / TB, (n + TB - 1) / TB, a.D);
+        const dim3 gb((n + TB - 1) / TB, (n + TB - 1) / TB, a.D);
         if (interp == 0)
           k_readout_box<2><<<gb, 256, 0, st>>>(a);
         else if (interp == 1)

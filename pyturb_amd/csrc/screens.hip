@@ -95,18 +95,6 @@ int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, i
 // ----------------------------------------------------------------------------
 // plain 2-D transform passes (pt_fft2)
 // ----------------------------------------------------------------------------
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Column tiles of one grid share 128-byte lines (M * 4 or M * 8 bytes
-// per row per tile), so the flat tile index is remapped to give every XCD a
-// contiguous run of tiles: the partial lines of neighbouring tiles then meet
-// in one L2 instead of being fetched and written back by several.  A
-// bijection on [0, nx) for any nx.
-__device__ __forceinline__ int xcd_tile(int bx, int nx) {
-  const int xcd = bx & 7, k = bx >> 3;
-  const int q = nx >> 3, r = nx & 7;
-  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
-}
-
 template <int N, int S>
 __global__ __launch_bounds__(256) void k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, int n1) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;

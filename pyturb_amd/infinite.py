@@ -125,23 +125,32 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales):
     st = _hip.stream_ptr()
     As = A if isinstance(A, (list, tuple)) else [A] * L
     Bs = B if isinstance(B, (list, tuple)) else [B] * L
-    # innovation GEMM per distinct B (one per outer-scale group)
+    # innovation GEMM per distinct B (one per outer-scale group); the kernel
+    # reads every innovation row in place through a host array of pointers
     bb_of = {}
     bbrow = [0] * L
     for key in dict.fromkeys(id(b) for b in Bs):
         members = [l for l in range(L) if id(Bs[l]) == key and len(plans[l])]
         if not members:
             continue
-        total = sum(len(plans[l]) for l in members)
-        beta = t.zeros((total, ld), dtype=t.float32, device=buf.device)
-        off = 0
+        rows = []
+        keep = []
         for l in members:
-            k = len(plans[l])
-            beta[off:off + k, :width] = betas[l]
-            off += k
-        bb = t.empty_like(beta)
-        rc = lib.pt_innovation_gemm(_hip.ptr(beta), _hip.ptr(Bs[members[0]]), total,
-                                    width, ld, _hip.ptr(bb), st)
+            beta = betas[l]
+            if beta.dtype != t.float32 or beta.device != buf.device or beta.dim() != 2 \
+                    or beta.shape[1] != width or beta.stride(1) != 1:
+                beta = beta.to(device=buf.device, dtype=t.float32).reshape(-1, width).contiguous()
+            if beta.shape[0] != len(plans[l]):
+                raise ValueError(f"layer {l}: {beta.shape[0]} innovation rows for "
+                                 f"{len(plans[l])} new rows")
+            keep.append(beta)
+            base, pitch = beta.data_ptr(), beta.stride(0) * 4
+            rows.extend(base + i * pitch for i in range(beta.shape[0]))
+        total = len(rows)
+        bb = t.empty((total, ld), dtype=t.float32, device=buf.device)
+        ptrs = (_ct.c_void_p * total)(*rows)
+        rc = lib.pt_innovation_gemm(ptrs, _hip.ptr(Bs[members[0]]), total, width, ld,
+                                    _hip.ptr(bb), st)
         _hip.check(rc, "pt_innovation_gemm")
         off = 0
         for l in members:

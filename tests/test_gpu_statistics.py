@@ -12,7 +12,9 @@ tests/test_analysis.py:83-98, benchmarks/bench_compare.py:178-225):
   deficit that grows to -3 % at D/4 -- the method's own bias);
 * the same for the two-group (L0 = 25 m / 50 m) ensemble of pt_generate_groups;
 * frozen-flow temporal PSD slope of the spectral engine in the -8/3 band;
-* extruded structure function along the wind and isotropy after burn-in.
+* extruded structure function along the wind and isotropy after burn-in;
+* Kolmogorov Zernike-mode variances against Noll (1976), with the bands of
+  tests/test_analysis.py:42-61.
 """
 
 from __future__ import annotations
@@ -163,3 +165,24 @@ def test_extruded_screen_is_isotropic_after_burn_in(hip):
     assert np.all((0.85 < along / theory) & (along / theory < 1.15)), along / theory
     assert np.all((0.85 < cross / theory) & (cross / theory < 1.15)), cross / theory
     assert 0.85 < along.mean() / cross.mean() < 1.15
+
+
+def test_zernike_variances_match_noll(hip):
+    """Device screens at L0 = inf decomposed on Noll modes 1..15 over the
+    inscribed pupil: modes 4..15 together and the astigmatism pair against
+    Noll's variances (tip/tilt are under-sampled by a finite screen and are
+    not banded, as in the reference's own check)."""
+    import pyturb_amd as pa
+
+    print("Noll variances 1600 x 128^2", flush=True)
+    D, n, r0 = 4.0, 128, 0.4
+    gen = pa.PhaseScreen(n, D / n, r0, np.inf, seed=7, device="mi355x")
+    basis = O.zernike_basis(15, n)
+    coeffs = np.concatenate([O.zernike_decompose(gen.generate(400).double().cpu().numpy(), 15,
+                                                 basis=basis) for _ in range(4)])
+    measured = coeffs.var(axis=0)
+    mid = measured[3:15].sum() / sum(O.noll_variance(j, D, r0) for j in range(4, 16))
+    astig = 0.5 * (measured[4] + measured[5]) / O.noll_variance(5, D, r0)
+    print("modes 4-15 ratio", mid, "astigmatism ratio", astig)
+    assert 0.85 < mid < 1.2
+    assert 0.8 < astig < 1.25

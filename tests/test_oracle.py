@@ -292,3 +292,65 @@ def test_structure_function_of_oracle_screens_matches_theory():
     r, d = O.structure_function(scr, 0.02, max_separation=20)
     ratio = d[3:] / (6.88 * (r[3:] / 0.1) ** (5 / 3))
     assert np.all((ratio > 0.8) & (ratio < 1.2))
+
+
+# ---------------------------------------------------------------------------
+# Zernike modes and Noll variances (validation statistics)
+# ---------------------------------------------------------------------------
+def test_noll_index_table():
+    """Known answers of tests/test_analysis.py:10-14, plus the sine/cosine
+    parity rule: even j carries cos (m >= 0), odd j sin (m < 0), m = 0 apart."""
+    expected = {1: (0, 0), 2: (1, 1), 3: (1, -1), 4: (2, 0), 5: (2, -2),
+                6: (2, 2), 7: (3, -1), 8: (3, 1), 11: (4, 0)}
+    for j, nm in expected.items():
+        assert O.noll_to_zernike(j) == nm
+    seen = set()
+    for j in range(1, 67):
+        n, m = O.noll_to_zernike(j)
+        assert (n - abs(m)) % 2 == 0 and abs(m) <= n
+        if m:
+            assert (m > 0) == (j % 2 == 0)
+        seen.add((n, m))
+    assert len(seen) == 66  # radial orders 0..10, every (n, m) exactly once
+    with pytest.raises(ValueError):
+        O.noll_to_zernike(0)
+
+
+def test_zernike_radial_closed_forms():
+    rho = np.linspace(0.0, 1.0, 11)
+    from oracle.pyturb_oracle import _zernike_radial as R
+
+    assert np.allclose(R(2, 0, rho), 2 * rho ** 2 - 1)
+    assert np.allclose(R(3, 1, rho), 3 * rho ** 3 - 2 * rho)
+    assert np.allclose(R(4, 0, rho), 6 * rho ** 4 - 6 * rho ** 2 + 1)
+    assert np.allclose(R(4, 2, rho), 4 * rho ** 4 - 3 * rho ** 2)
+    for n in range(8):
+        for m in range(n % 2, n + 1, 2):
+            assert R(n, m, np.array([1.0]))[0] == pytest.approx(1.0)  # R_n^m(1) = 1
+
+
+def test_zernike_basis_orthonormal_and_decomposition_exact():
+    basis = O.zernike_basis(15, 256)
+    pupil = basis[0] != 0
+    vecs = basis[:, pupil]
+    gram = vecs @ vecs.T / pupil.sum()
+    assert np.allclose(np.diag(gram), 1.0, atol=0.02)
+    assert np.max(np.abs(gram - np.diag(np.diag(gram)))) < 0.02
+    rng = np.random.default_rng(0)
+    truth = rng.standard_normal((3, 15))
+    stack = np.einsum("cj,jxy->cxy", truth, basis)
+    assert np.allclose(O.zernike_decompose(stack, 15, basis=basis), truth, atol=1e-10)
+    assert np.allclose(O.zernike_decompose(stack[0], 15), truth[0], atol=1e-10)
+
+
+def test_noll_residuals_table_and_asymptote():
+    r = [O.noll_residual(j) for j in range(1, 40)]
+    assert all(a > b for a, b in zip(r, r[1:]))
+    assert O.noll_residual(1) == pytest.approx(1.0299)
+    assert O.noll_residual(3) == pytest.approx(0.134)
+    assert O.noll_residual(22) == pytest.approx(0.2944 * 22 ** (-np.sqrt(3) / 2))
+    # tip + tilt carry 0.448 (D/r0)^(5/3) together (Noll 1976, Delta_1 - Delta_3)
+    assert O.noll_variance(2, 8.0, 0.2) + O.noll_variance(3, 8.0, 0.2) == pytest.approx(
+        (1.0299 - 0.134) * 40.0 ** (5 / 3))
+    with pytest.raises(ValueError):
+        O.noll_variance(1, 8.0, 0.2)
