@@ -276,6 +276,45 @@ int pt_spectral_frames_adjoint(const float* noise, const float* amplitude,
                                const double* sh_rho, int levels, double scale,
                                double* out, void* work, void* stream);
 
+/* Workspace (c64 elements) of pt_boiled_frames; -1 on bad input. */
+long long pt_boil_work_elems(int n, int frames, int layers, int levels);
+
+/*
+ * Spectral frames with boiling: `frames` frames at the displacements disp,
+ * each followed by one boiling step of dt seconds, then steps - frames more
+ * steps (frames = 0: boil only).  Frame k is built from the state after k
+ * steps, as Atmosphere.frames does (src/pyturb/atmosphere.py:1128-1154),
+ * and the state is left after `steps` steps.  One boiling step is the
+ * per-mode AR(1) update of src/pyturb/atmosphere.py:1092-1126 with the time
+ * constants of :618-658:
+ *   S <- a S + sqrt(1 - a^2) (n0 + i n1) A,   a = exp(-dt / tau(f)),
+ *   tau(f) = tau_boil * max(|f| / f_ref, 1)^(-2/3),
+ * on the layer spectra and on the subharmonic coefficients.
+ *   spectra   : (layers, n, n) c64 state, in place
+ *   amplitude : (layers, n, n) f32 PSD amplitude of each layer
+ *   inv_tau   : (layers) f64 device, 1 / tau_boil (0 for a frozen layer)
+ *   f_ref     : (layers) f64 device, 1 / L0 (or 1 / (n dx) for L0 = inf)
+ *   disp      : (frames, layers, 2) f64 device, displacements [m]
+ *   sh_coeffs : (levels, layers, 3, 3) c128 state, in place
+ *   sh_amp    : (levels, layers, 3, 3) f64 subharmonic amplitudes
+ *   noise     : (steps, 2, layers, n, n) f32 standard normals in the
+ *               reference's draw order, or NULL for the device stream
+ *               (Philox4x32-10 keyed by `seed`, counted from step `step0`)
+ *   sh_noise  : (steps, 2, levels, layers, 3, 3) f64, or NULL likewise
+ *   out       : (frames, n_out, n_out) f32, crop and scale as in
+ *               pt_spectral_frames.
+ * Replaces Atmosphere._boil_step + _integrate per frame
+ * (src/pyturb/atmosphere.py:882-938, 1092-1154).  layers <= 16.
+ */
+int pt_boiled_frames(void* spectra, const float* amplitude,
+                     const double* inv_tau, const double* f_ref, int layers,
+                     int n, double pixel_scale, const double* disp, int frames,
+                     int steps, double dt, double* sh_coeffs,
+                     const double* sh_amp, int levels, const float* noise,
+                     const double* sh_noise, unsigned long long seed,
+                     unsigned long long step0, int crop_offset, int n_out,
+                     float scale, float* out, void* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

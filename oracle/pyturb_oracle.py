@@ -77,6 +77,7 @@ __all__ = [
     "zernike_decompose",
     "noll_residual",
     "noll_variance",
+    "device_boil_noise",
 ]
 
 
@@ -302,7 +303,7 @@ class OracleAtmosphere:
     def __init__(self, layers, r0, wavelength=500e-9, zenith_angle=0.0,
                  diameter=8.0, n=512, L0=None, power_law=11.0 / 3.0,
                  inner_scale=0.0, subharmonics=8, field_of_view=0.0,
-                 dtype="float32", seed=None):
+                 dtype="float32", seed=None, tau_boil=None):
         lay = [tuple(float(v) for v in ly) for ly in layers]
         frac = np.array([ly[1] for ly in lay])
         frac = frac / frac.sum()
@@ -330,8 +331,15 @@ class OracleAtmosphere:
         self.period = self.ns * self.dx
         master = np.random.SeedSequence(seed)
         children = master.spawn(len(lay))
-        master.spawn(1)  # spectral boiling stream (atmosphere.py:395)
-        master.spawn(1)  # extruder boiling stream (atmosphere.py:397)
+        # spectral boiling stream (atmosphere.py:395-396), then the
+        # extruder's boiling stream (atmosphere.py:397)
+        self.boil_seed = int(master.spawn(1)[0].generate_state(1)[0])
+        master.spawn(1)
+        self.dtype = dtype
+        if tau_boil is None:
+            self.tau_boil = np.full(len(lay), np.inf)
+        else:
+            self.tau_boil = np.broadcast_to(np.asarray(tau_boil, float), (len(lay),)).copy()
         self.filters, spectra, coeffs = [], [], []
         self.gen_seeds, self.ext_seeds = [], []
         kw = dict(subharmonics=subharmonics, power_law=power_law,
@@ -348,9 +356,13 @@ class OracleAtmosphere:
             coeffs.append(cf)
         self.spectra = np.stack(spectra)
         self.sh_coeffs = np.stack(coeffs, axis=1)  # (P, L, 3, 3)
+        self._initial = (self.spectra.copy(), self.sh_coeffs.copy())
         self.f = self.filters[0]["f"]
         self.sh_freq = self.filters[0]["sh_freq"]
         self.sh_basis = self.filters[0]["sh_basis"]
+        self.L0 = np.array([ly[4] for ly in lay])
+        self.t = 0.0
+        self.boil_rng = np.random.default_rng(self.boil_seed)
         # sample() generators, one per distinct round(L0, 9)
         groups: Dict[float, List[int]] = {}
         for i, ly in enumerate(lay):
@@ -379,6 +391,80 @@ class OracleAtmosphere:
             return self.phase(t) * scale
         return np.stack([self.phase(t, np.tan(a * ARCSEC), np.tan(b * ARCSEC))
                          for a, b in directions]) * scale
+
+    # ------------------------------------------------------------- boiling
+    def boil_tau_maps(self):
+        """atmosphere.py:618-658 -- per-mode boiling time constants.
+
+        tau(f) = tau_boil * max(f / f_ref, 1)^(-2/3) with f_ref = 1/L0, or
+        the grid fundamental 1/(n_screen dx) for L0 = inf; (L, ns, ns) for
+        the grid and (P, L, 3, 3) for the subharmonic cells."""
+        df = 1.0 / (self.ns * self.dx)
+        f_ref = np.array([1.0 / L0 if np.isfinite(L0) and L0 > 0 else df for L0 in self.L0])
+        fr = np.hypot(self.f[:, None], self.f[None, :])
+        ratio = np.maximum(fr[None] / f_ref[:, None, None], 1.0)
+        main = self.tau_boil[:, None, None] * ratio ** (-2.0 / 3.0)
+        fp = self.sh_freq  # (P, 3)
+        fr_sh = np.hypot(fp[:, :, None], fp[:, None, :])  # (P, 3, 3)
+        ratio_sh = np.maximum(fr_sh[:, None] / f_ref[None, :, None, None], 1.0)
+        sh = self.tau_boil[None, :, None, None] * ratio_sh ** (-2.0 / 3.0)
+        return main, sh
+
+    def draw_boil_noise(self):
+        """The draws of one boiling step, in the reference's order and dtype
+        (atmosphere.py:1114-1116, 1122-1124): main (2, L, ns, ns), then the
+        subharmonic (2, P, L, 3, 3)."""
+        L, P = self.spectra.shape[0], self.sh_coeffs.shape[0]
+        main = self.boil_rng.standard_normal((2, L, self.ns, self.ns), dtype=self.dtype)
+        sh = (self.boil_rng.standard_normal((2, P, L, 3, 3), dtype=self.dtype) if P else None)
+        return main, sh
+
+    def boil_step(self, dt, noise=None):
+        """atmosphere.py:1092-1126 -- one AR(1) update per mode.
+
+        S <- a S + sqrt(1 - a^2) (n0 + i n1) A with a = exp(-dt / tau(f)); the
+        same for the subharmonic coefficients.  Nothing is drawn when every
+        layer is frozen (infinite tau_boil).  ``noise``: an injected
+        ``(main, sh)`` pair instead of the boiling stream's draw."""
+        with np.errstate(divide="ignore"):
+            alpha = np.where(np.isfinite(self.tau_boil), np.exp(-dt / self.tau_boil), 1.0)
+        if np.all(alpha >= 1.0):
+            return
+        main_tau, sh_tau = self.boil_tau_maps()
+        main, sh = self.draw_boil_noise() if noise is None else noise
+        main = np.asarray(main, np.float64)
+        a = np.exp(-dt / main_tau)
+        b = np.sqrt(np.clip(1.0 - a * a, 0.0, None))
+        amp = np.stack([fl["amplitude"] for fl in self.filters])
+        self.spectra = a * self.spectra + b * ((main[0] + 1j * main[1]) * amp)
+        if self.sh_coeffs.shape[0]:
+            sh = np.asarray(sh, np.float64)
+            a_sh = np.exp(-dt / sh_tau)
+            b_sh = np.sqrt(np.clip(1.0 - a_sh * a_sh, 0.0, None))
+            amps = np.stack([fl["sh_amp"] for fl in self.filters], axis=1)  # (P, L, 3, 3)
+            self.sh_coeffs = a_sh * self.sh_coeffs + b_sh * ((sh[0] + 1j * sh[1]) * amps)
+
+    def frames(self, dt, steps, noises=None):
+        """atmosphere.py:1128-1154 -- yield (t, opd), then advance the clock
+        and boil once; ``noises``: optional per-step injected draws."""
+        for k in range(int(steps)):
+            t = self.t
+            yield t, self.opd(t)
+            self.t += float(dt)
+            self.boil_step(float(dt), None if noises is None else noises[k])
+
+    def evolve(self, dt, noise=None):
+        """atmosphere.py:1156-1189 -- advance, boil once, return the OPD."""
+        self.t += float(dt)
+        self.boil_step(float(dt), noise)
+        return self.opd(self.t)
+
+    def reset(self):
+        """atmosphere.py:1221-1243 -- clock to 0, initial turbulence and
+        boiling stream restored."""
+        self.t = 0.0
+        self.spectra, self.sh_coeffs = (a.copy() for a in self._initial)
+        self.boil_rng = np.random.default_rng(self.boil_seed)
 
     def sample(self, count, noises=None, dtype="float64"):
         """atmosphere.py:1191-1219 -- summed group screens, OPD metres.
@@ -760,3 +846,74 @@ def noll_variance(j: int, diameter: float, r0: float) -> float:
     if j < 2:
         raise ValueError("piston (j=1) has no finite variance; use j >= 2")
     return (noll_residual(j - 1) - noll_residual(j)) * (diameter / r0) ** (5.0 / 3.0)
+
+
+# ---------------------------------------------------------------------------
+# The device boiling stream: Philox4x32-10 (Salmon, Moraes, Dror & Shaw,
+# "Parallel random numbers: as easy as 1, 2, 3", SC 2011) + Box-Muller
+# ---------------------------------------------------------------------------
+_PHILOX_M = (np.uint64(0xD2511F53), np.uint64(0xCD9E8D57))
+_PHILOX_W = (np.uint32(0x9E3779B9), np.uint32(0xBB67AE85))
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32 with 10 rounds on uint32 arrays; ``ctr`` (..., 4), ``key`` (2,)."""
+    c = [np.asarray(ctr, dtype=np.uint32)[..., i].astype(np.uint64) for i in range(4)]
+    k0, k1 = np.uint32(key[0]), np.uint32(key[1])
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = _PHILOX_M[0] * c[0]
+        p1 = _PHILOX_M[1] * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c = [hi1 ^ c[1] ^ np.uint64(k0), lo1, hi0 ^ c[3] ^ np.uint64(k1), lo0]
+        k0 = np.uint32((int(k0) + int(_PHILOX_W[0])) & 0xFFFFFFFF)
+        k1 = np.uint32((int(k1) + int(_PHILOX_W[1])) & 0xFFFFFFFF)
+    return np.stack([v.astype(np.uint32) for v in c], axis=-1)
+
+
+def box_muller(a, b):
+    """Two standard normals from 32-bit words: u1 in (0, 1], u2 in [0, 1)."""
+    u1 = (np.asarray(a, np.float64) + 1.0) * 2.0 ** -32
+    u2 = np.asarray(b, np.float64) * 2.0 ** -32
+    r = np.sqrt(-2.0 * np.log(u1))
+    return r * np.cos(2.0 * np.pi * u2), r * np.sin(2.0 * np.pi * u2)
+
+
+def device_boil_noise(seed, step, L, n, P):
+    """The normals the device stream uses at one boiling step, in the
+    reference's layout: main (2, L, n, n) and subharmonic (2, P, L, 3, 3).
+
+    Mode e and its mirror m = -e mod n (e <= m) share one Philox call with
+    counter (e, layer, step low, step high): words 0-1 give the normals of
+    e, words 2-3 those of m; subharmonic cell q uses (q, 2^31, step)."""
+    key = (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+    k0, k1 = np.divmod(np.arange(n * n), n)
+    mirror = ((n - k0) % n) * n + (n - k1) % n
+    e = np.arange(n * n)
+    own = e <= mirror
+    main = np.zeros((2, L, n * n))
+    for l in range(L):
+        ctr = np.zeros((int(own.sum()), 4), dtype=np.uint32)
+        ctr[:, 0] = e[own]
+        ctr[:, 1] = l
+        ctr[:, 2] = step & 0xFFFFFFFF
+        ctr[:, 3] = (step >> 32) & 0xFFFFFFFF
+        r = philox4x32_10(ctr, key)
+        re_e, im_e = box_muller(r[:, 0], r[:, 1])
+        re_m, im_m = box_muller(r[:, 2], r[:, 3])
+        main[0, l, mirror[own]] = re_m
+        main[1, l, mirror[own]] = im_m
+        main[0, l, e[own]] = re_e  # self-conjugate modes take words 0-1
+        main[1, l, e[own]] = im_e
+    sh = None
+    if P:
+        count = P * L * 9
+        ctr = np.zeros((count, 4), dtype=np.uint32)
+        ctr[:, 0] = np.arange(count)
+        ctr[:, 1] = 0x80000000
+        ctr[:, 2] = step & 0xFFFFFFFF
+        ctr[:, 3] = (step >> 32) & 0xFFFFFFFF
+        r = philox4x32_10(ctr, key)
+        sh = np.stack(box_muller(r[:, 0], r[:, 1])).reshape(2, P, L, 3, 3)
+    return main.reshape(2, L, n, n), sh

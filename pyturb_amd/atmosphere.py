@@ -18,8 +18,10 @@ Device design:
 * ``sample`` -- one fused ``pt_generate`` call per outer-scale group,
   accumulating into the cropped OPD output.
 
-Boiling (``tau_boil``), the spectral LGS cone and chromatic dispersion
-models are outside this backend and raise ``NotImplementedError``.
+Spectral boiling (``tau_boil`` on ``engine="spectral"``) runs on the device
+(pt_boiled_frames).  Extruder boiling, the spectral LGS cone and chromatic
+dispersion models are outside this backend and raise
+``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -42,6 +44,7 @@ __all__ = ["Atmosphere", "PeriodicWrapWarning", "ExtrudeBoilingPerformanceWarnin
 
 _ARCSEC_TO_RAD = np.pi / (180.0 * 3600.0)
 _FRAME_CHUNK = 64
+_BOIL_CHUNK = 128  # boiled frames per device call
 
 
 class PeriodicWrapWarning(UserWarning):
@@ -84,8 +87,11 @@ class Atmosphere:
             layers, r0, seeing, wavelength, zenith_angle, diameter, n, L0, power_law,
             inner_scale, subharmonics, field_of_view, tau_boil, engine, interp,
             lgs_altitude, dispersion, wet_fraction, device, dtype, seed)
-        if config.has_tau_boil and np.any(np.isfinite(config.tau_boil)):
-            raise NotImplementedError("tau_boil (boiling) is not provided by the MI355X backend")
+        if (config.engine == "extrude" and config.has_tau_boil
+                and np.any(np.isfinite(config.tau_boil))):
+            raise NotImplementedError(
+                "tau_boil on engine='extrude' (extruder boiling) is not provided by the MI355X "
+                "backend; use engine='spectral', which boils per Fourier mode")
         self.config = config
         self.engine = config.engine
         self.interp = config.interp
@@ -138,8 +144,14 @@ class Atmosphere:
         # two boiling streams, then (gen, flow, ext) per layer.
         master = np.random.SeedSequence(self.seed)
         children = master.spawn(len(self.layers))
+        # spectral boiling stream, then the extruder's (atmosphere.py:395-397)
+        self._boil_seed = int(master.spawn(1)[0].generate_state(1)[0])
         master.spawn(1)
-        master.spawn(1)
+        self._boiling = self.engine == "spectral" and bool(np.any(np.isfinite(self.tau_boil)))
+        # injected boiling draws (tests): step index -> (main (2, L, ns, ns),
+        # subharmonic (2, P, L, 3, 3)) host arrays, replacing the device stream
+        self._boil_noise = None
+        self._epoch = 0
         self._layers: List[_LayerState] = []
         ext_r0, ext_L0, ext_wind, ext_alt, ext_seeds, flow_seeds = [], [], [], [], [], []
         for layer, child in zip(self.layers, children):
@@ -215,6 +227,20 @@ class Atmosphere:
         self._spectra = _hip.to_device(spectra, None, dev)
         self._sh_coeffs = _hip.to_device(coeffs, None, dev) if P else None
         self._spectra_host_c64 = None
+        self._boil_index = 0
+        if self._boiling:
+            t = _hip.torch()
+            gens = [st.generator for st in self._layers]
+            self._boil_amp = t.stack([g._amplitude for g in gens]).contiguous()
+            self._boil_sh_amp = (t.stack([g._sh_amp for g in gens], dim=1).contiguous()
+                                 if P else None)
+            df = 1.0 / (ns * self.pixel_scale)
+            L0s = np.array([float(ly.L0) for ly in self.layers])
+            f_ref = np.where(np.isfinite(L0s) & (L0s > 0), 1.0 / np.where(L0s > 0, L0s, 1.0), df)
+            with np.errstate(divide="ignore"):
+                inv_tau = np.where(np.isfinite(self.tau_boil), 1.0 / self.tau_boil, 0.0)
+            self._boil_f_ref = _hip.to_device(f_ref, np.float64, dev)
+            self._boil_inv_tau = _hip.to_device(inv_tau, np.float64, dev)
 
     @classmethod
     def from_profile(cls, name: str, **kwargs) -> "Atmosphere":
@@ -360,6 +386,9 @@ class Atmosphere:
             raise ValueError("steps must be >= 1")
         dt = float(dt)
         scale = self._scale(wavelength)
+        if self._boiling:
+            yield from self._boiled_frames(dt, int(steps), scale)
+            return
         left = int(steps)
         while left > 0:
             k = min(left, _FRAME_CHUNK) if self.engine == "spectral" else 1
@@ -385,7 +414,85 @@ class Atmosphere:
         if dt <= 0:
             raise ValueError("dt must be positive")
         self._t += float(dt)
+        self._boil_step(float(dt))  # no-op unless tau_boil is finite
         return self._cast(self._phase_scaled(self._t, 0.0, 0.0, self._scale(wavelength)))
+
+    # ------------------------------------------------------------------ boiling
+    def _boil_call(self, times, dt: float, steps: int, scale: float = 1.0):
+        """Frames at ``times`` (each followed by a boiling step), then
+        ``steps - len(times)`` more steps; advances the boiling state."""
+        t = _hip.torch()
+        F = len(times)
+        disp = None
+        if F:
+            times = np.asarray(times, dtype=np.float64)
+            dx = self._vx[None, :] * times[:, None]
+            dy = self._vy[None, :] * times[:, None]
+            if not self._wrap_warned:
+                self._check_wrap(dx, dy)
+            disp = _hip.to_device(np.stack([dx, dy], axis=-1), np.float64, self._dev)
+        noise = sh_noise = None
+        if self._boil_noise is not None:
+            draws = [self._boil_noise(self._boil_index + k) for k in range(steps)]
+            noise = _hip.to_device(np.stack([np.asarray(d[0]) for d in draws]), np.float32,
+                                   self._dev)
+            if self._sh_coeffs is not None:
+                sh_noise = _hip.to_device(np.stack([np.asarray(d[1]) for d in draws]),
+                                          np.float64, self._dev)
+        out = ops.boiled_frames(
+            self._spectra, self._boil_amp, self._boil_inv_tau, self._boil_f_ref, disp, dt,
+            self.pixel_scale, sh_coeffs=self._sh_coeffs, sh_amp=self._boil_sh_amp, steps=steps,
+            noise=noise, sh_noise=sh_noise, seed=self._boil_seed, step0=self._boil_index,
+            crop_offset=self.margin_pix, n_out=self.n, scale=scale)
+        self._boil_index += steps
+        del t
+        return out
+
+    def _boil_step(self, dt: float) -> None:
+        """One AR(1) boiling step of every mode (atmosphere.py:1092-1126)."""
+        if self._boiling:
+            self._boil_call([], dt, 1)
+
+    def _boiled_frames(self, dt: float, steps: int, scale: float):
+        """frames() with boiling: chunks of frames from one device call each.
+
+        A chunk advances the state by one step per frame.  If the caller
+        stops after fewer frames, the state is put back to the chunk's start
+        and advanced by the steps actually taken; the device stream is
+        counted by step, so this reproduces the state exactly.  A reset()
+        while a frame is out takes effect as in the reference: the clock
+        then advances and the fresh state boils once.
+        """
+        left = steps
+        while left > 0:
+            k = 1 if self._boil_noise is not None else min(left, _BOIL_CHUNK)
+            epoch = self._epoch
+            snap = (self._spectra.clone(),
+                    None if self._sh_coeffs is None else self._sh_coeffs.clone(),
+                    self._boil_index)
+            times = [self._t + i * dt for i in range(k)]
+            chunk = self._cast(self._boil_call(times, dt, k, scale))
+            used = 0
+            try:
+                for i in range(k):
+                    yield times[i], chunk[i]
+                    if self._epoch != epoch:  # reset while the frame was out
+                        break
+                    used += 1
+                    self._t += dt
+                    left -= 1
+            finally:
+                if used < k and self._epoch == epoch:
+                    self._spectra.copy_(snap[0])
+                    if snap[1] is not None:
+                        self._sh_coeffs.copy_(snap[1])
+                    self._boil_index = snap[2]
+                    if used:
+                        self._boil_call([], dt, used)
+            if self._epoch != epoch:
+                self._t += dt
+                self._boil_step(dt)
+                left -= 1
 
     # ------------------------------------------------------------------ Monte-Carlo
     def _generators(self) -> List[PhaseScreen]:
@@ -474,10 +581,14 @@ class Atmosphere:
     def reset(self) -> "Atmosphere":
         self._t = 0.0
         self._wrap_warned = False
+        self._epoch += 1
         if self.engine == "extrude":
             from .extrude import ExtrudedAtmosphere
 
             self._ext = ExtrudedAtmosphere(**self._ext_kwargs)
+        elif self._boiling:
+            # boiling changed the stored turbulence: rebuild it and rewind the stream
+            self._build_batched()
         return self
 
     @property
@@ -492,7 +603,10 @@ class Atmosphere:
             "n_layers": len(self.layers), "engine": self.engine, "interp": self.interp,
             "power_law": self.power_law, "inner_scale": self.inner_scale,
             "subharmonics": self.subharmonics, "field_of_view": self.field_of_view,
-            "tau_boil": None, "dispersion": self.dispersion,
+            # one rate for every layer, else None (atmosphere.py:1270-1274)
+            "tau_boil": (float(self.tau_boil[0]) if np.all(np.isfinite(self.tau_boil))
+                         and np.all(self.tau_boil == self.tau_boil[0]) else None),
+            "dispersion": self.dispersion,
             "wet_fraction": self.wet_fraction,
             "lgs_altitude": 0.0 if self.lgs_altitude is None else float(self.lgs_altitude),
             "device": self.device, "dtype": self.dtype, "seed": self.seed, "time": self._t,

@@ -172,6 +172,37 @@ __global__ __launch_bounds__(256) void k_frame_epi(FrameEpiArgs a) {
   a.out[((size_t)f * a.n_out + x) * a.n_out + y] = (v.x + lo) * a.scale;
 }
 
+// Frame-pair epilogue: pair b holds frame 2b in its real part and frame
+// 2b + 1 in its imaginary part.
+__global__ __launch_bounds__(256) void k_pair_epi(FrameEpiArgs a, int frames) {
+  const int n = a.n, P = a.levels, b = blockIdx.z;
+  const int y = blockIdx.x * 32 + (threadIdx.x & 31);
+  const int x = blockIdx.y * 8 + (threadIdx.x >> 5);
+  if (x >= a.n_out || y >= a.n_out) return;
+  const int i = x + a.off, j = y + a.off;
+  const float2 v = a.work[(size_t)b * n * n + (size_t)i * n + j];
+  const int f0 = 2 * b, f1 = f0 + 1;
+  const size_t o = (size_t)x * a.n_out + y, oplane = (size_t)a.n_out * a.n_out;
+  a.out[(size_t)f0 * oplane + o] = (v.x + (P > 0 ? sub_eval(a.tab, f0, i, j, n, P) : 0.f)) * a.scale;
+  if (f1 < frames) a.out[(size_t)f1 * oplane + o] = (v.y + (P > 0 ? sub_eval(a.tab, f1, i, j, n, P) : 0.f)) * a.scale;
+}
+
+int launch_pair_epi(const float2* z, float* out, SubTables t, int n, int frames, int levels, int off, int n_out,
+                    float scale, hipStream_t st) {
+  FrameEpiArgs a;
+  a.work = z;
+  a.out = out;
+  a.tab = t;
+  a.n = n;
+  a.levels = levels;
+  a.off = off;
+  a.n_out = n_out;
+  a.scale = scale;
+  k_pair_epi<<<dim3((n_out + 31) / 32, (n_out + 7) / 8, (frames + 1) / 2), 256, 0, st>>>(a, frames);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
 static int grid_stride_blocks(size_t total) {
   size_t b = (total + 255) / 256;
   return (int)(b < 65536 ? b : 65536);

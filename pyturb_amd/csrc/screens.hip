@@ -17,6 +17,7 @@
 
 #include "common.hpp"
 #include "fft_core.hpp"
+#include "boil.hpp"
 #include "sh_epilogue.hpp"
 
 namespace pt {
@@ -61,14 +62,14 @@ __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_n
 }
 
 __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ disp,
-                                                    const double2* __restrict__ coeffs, int L, int P, int n,
-                                                    double inv_period, SubTables t) {
+                                                    const double2* __restrict__ coeffs, size_t cstride, int L,
+                                                    int P, int n, double inv_period, SubTables t) {
   extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ double s_mean;
   double2* Q = (double2*)dyn;
   double2* SB = Q + 2 * (size_t)P * L;
   const int f = blockIdx.x, tid = threadIdx.x;
-  frame_sh_blocks(Q, SB, disp, coeffs, f, L, P, inv_period, tid, 256);
+  frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256);
   sub_fill_set(SB, P, n, f, t, f == 0, &s_mean, tid, 256);
 }
 
@@ -82,12 +83,12 @@ int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int 
 }
 
 int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
-                      SubTables t, hipStream_t st) {
+                      SubTables t, hipStream_t st, size_t cstride) {
   if (P == 0) return PT_OK;
   if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
   const size_t dyn = (size_t)(2 * P * L + 9 * P) * sizeof(double2);
   if (dyn > 60 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: %d layers x %d levels exceed LDS", L, P);
-  k_frame_prep<<<F, 256, dyn, st>>>(disp, coeffs, L, P, n, inv_period, t);
+  k_frame_prep<<<F, 256, dyn, st>>>(disp, coeffs, cstride, L, P, n, inv_period, t);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -810,4 +811,109 @@ extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double
   }
   return pt_frames_generic(spectra, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
                            scale, out, work, stream);
+}
+
+// ----------------------------------------------------------------------------
+// boiled spectral frames
+// ----------------------------------------------------------------------------
+// Workspace (c64 elements): frame-pair spectra, a transform scratch of the
+// same size (general n only), the phase table, the per-frame subharmonic
+// coefficient sets (c128, two elements each) and the subharmonic tables.
+struct BoilLayout {
+  size_t z, scratch, phase, cf, tab, total;
+};
+static BoilLayout boil_layout(int n, int F, int L, int P) {
+  BoilLayout w;
+  const size_t pairs_plane = (size_t)((F + 1) / 2) * n * n;
+  const bool fused = is_pow2_fast(n) && P <= PREG;
+  w.z = 0;
+  w.scratch = r32(pairs_plane);
+  w.phase = w.scratch + (fused ? 0 : r32(pairs_plane));
+  w.cf = w.phase + r32((size_t)F * L);
+  w.tab = w.cf + r32((size_t)F * P * L * 9 * 2);
+  w.total = w.tab + sub_table_c64(P, n, F);
+  return w;
+}
+
+extern "C" long long pt_boil_work_elems(int n, int frames, int layers, int levels) {
+  if (n < 2 || frames < 0 || layers < 1 || levels < 0) return -1;
+  return (long long)boil_layout(n, frames, layers, levels).total;
+}
+
+extern "C" int pt_boiled_frames(void* spectra, const float* amplitude, const double* inv_tau, const double* f_ref,
+                                int layers, int n, double pixel_scale, const double* disp, int frames, int steps,
+                                double dt, double* sh_coeffs, const double* sh_amp, int levels, const float* noise,
+                                const double* sh_noise, unsigned long long seed, unsigned long long step0,
+                                int crop_offset, int n_out, float scale, float* out, void* work, void* stream) {
+  if (!spectra || !amplitude || !inv_tau || !f_ref) return fail(PT_ERR_ARG, "pt_boiled_frames: NULL array");
+  if (layers < 1 || n < 2 || frames < 0 || steps < frames || levels < 0)
+    return fail(PT_ERR_ARG, "pt_boiled_frames: bad sizes (steps >= frames >= 0)");
+  if (layers > 16) return fail(PT_ERR_UNSUPPORTED, "pt_boiled_frames: more than 16 layers");
+  if (!(dt > 0.0) || !(pixel_scale > 0.0)) return fail(PT_ERR_ARG, "pt_boiled_frames: dt and pixel_scale must be > 0");
+  if (levels > 0 && (!sh_coeffs || !sh_amp)) return fail(PT_ERR_ARG, "pt_boiled_frames: NULL subharmonic arrays");
+  if (levels > PMAX) return fail(PT_ERR_UNSUPPORTED, "pt_boiled_frames: more than %d levels", PMAX);
+  if (frames > 65535) return fail(PT_ERR_ARG, "pt_boiled_frames: frames > 65535 per call");
+  if (frames > 0) {
+    if (!disp || !out || !work) return fail(PT_ERR_ARG, "pt_boiled_frames: NULL disp/out/work");
+    if (crop_offset < 0 || n_out < 1 || crop_offset + n_out > n) return fail(PT_ERR_ARG, "pt_boiled_frames: crop");
+  }
+  if (steps == 0) return PT_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const BoilLayout w = boil_layout(n, frames, layers, levels);
+  float2* base = (float2*)work;
+  BoilArgs b;
+  b.spectra = (float2*)spectra;
+  b.amp = amplitude;
+  b.inv_tau = inv_tau;
+  b.f_ref = f_ref;
+  b.coeffs = (double2*)sh_coeffs;
+  b.sh_amp = sh_amp;
+  b.coeff_frames = base ? (double2*)(base + w.cf) : nullptr;
+  b.noise = noise;
+  b.sh_noise = sh_noise;
+  b.phase = base ? (const uint2*)(base + w.phase) : nullptr;
+  b.z = base;
+  b.n = n;
+  b.L = layers;
+  b.P = levels;
+  b.frames = frames;
+  b.steps = steps;
+  b.dt = dt;
+  b.df = 1.0 / ((double)n * pixel_scale);
+  b.step0 = step0;
+  b.key0 = (uint32_t)seed;
+  b.key1 = (uint32_t)(seed >> 32);
+  const double inv_period = b.df;
+  int rc = launch_boil(b, disp, inv_period, st);
+  if (rc || frames == 0) return rc;
+  const SubTables tab = sub_tables(base + w.tab, levels, n, frames);
+  rc = launch_frame_prep(disp, (const double2*)(base + w.cf), frames, layers, levels, n, inv_period, tab, st,
+                         (size_t)levels * layers * 9);
+  if (rc) return rc;
+  const int pairs = (frames + 1) / 2;
+  if (is_pow2_fast(n) && levels <= PREG) {
+    GenArgs a;
+    a.work = base;
+    a.out = out;
+    a.screen_scale = nullptr;
+    a.tab = tab;
+    a.count = frames;
+    a.n_fft = pairs;
+    a.levels = levels;
+    a.off = crop_offset;
+    a.n_out = n_out;
+    a.accumulate = 0;
+    a.s_mul = 2;
+    a.s_add = 1;
+    a.scale = scale;
+#define CALL(N)                                                                               \
+  k_fft_rows<N, 1><<<dim3(N / RowsCfg<N>::M, pairs), RowsCfg<N>::T, 0, st>>>(base, base, N); \
+  PT_LAUNCH_CHECK();                                                                          \
+  return frames_cols_pow2<N>(a, st);
+    PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  }
+  rc = pt_fft2_generic(base, base, base + w.scratch, pairs, n, 1, stream);
+  if (rc) return rc;
+  return launch_pair_epi(base, out, tab, n, frames, levels, crop_offset, n_out, scale, st);
 }

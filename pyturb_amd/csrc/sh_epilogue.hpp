@@ -215,7 +215,12 @@ __device__ __forceinline__ float sub_eval(const SubTables& t, int s, int x, int 
 // Prep launchers (kernels in screens.hip).  Tables must not alias inputs.
 int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
                     hipStream_t st, int groups = 1);
+// cstride: coefficient sets per frame apart (0: one set for every frame).
 int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
-                      SubTables t, hipStream_t st);
+                      SubTables t, hipStream_t st, size_t cstride = 0);
+// Generic-size epilogue of frame-pair transforms: frame 2b = Re, 2b+1 = Im
+// of pair b, plus the subharmonic field, cropped and scaled.
+int launch_pair_epi(const float2* z, float* out, SubTables t, int n, int frames, int levels, int off, int n_out,
+                    float scale, hipStream_t st);
 
 }  // namespace pt

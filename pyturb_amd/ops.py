@@ -247,3 +247,59 @@ def spectral_frames_adjoint(noise, amplitude, dlnamp, group, gain, disp, grad, s
         _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
     _hip.check(rc, "pt_spectral_frames_adjoint")
     return out
+
+
+def boiled_frames(spectra, amplitude, inv_tau, f_ref, disp, dt: float, pixel_scale: float, *,
+                  sh_coeffs=None, sh_amp=None, steps: Optional[int] = None, noise=None,
+                  sh_noise=None, seed: int = 0, step0: int = 0, crop_offset: int = 0,
+                  n_out: Optional[int] = None, scale: float = 1.0, out=None, work=None):
+    """Frames with spectral boiling (pt_boiled_frames); updates the state.
+
+    ``spectra`` (L, n, n) complex64 and ``sh_coeffs`` (P, L, 3, 3)
+    complex128 are the boiling state, advanced in place by ``steps`` steps
+    (default: one per frame); ``amplitude`` (L, n, n) float32 and ``sh_amp``
+    (P, L, 3, 3) float64 the layers' PSD amplitudes; ``inv_tau`` and
+    ``f_ref`` (L,) float64 device tensors.  ``disp`` (F, L, 2) float64
+    metres, or None for boiling only.  ``noise`` (steps, 2, L, n, n) float32
+    and ``sh_noise`` (steps, 2, P, L, 3, 3) float64 inject the normals;
+    otherwise the device stream keyed by ``seed`` from step ``step0``.
+    Returns (F, n_out, n_out) float32 (None when F = 0).
+    """
+    t = _t()
+    L, n = int(spectra.shape[0]), int(spectra.shape[-1])
+    _require(spectra, t.complex64, (L, n, n), "spectra")
+    _require(amplitude, t.float32, (L, n, n), "amplitude")
+    _require(inv_tau, t.float64, (L,), "inv_tau")
+    _require(f_ref, t.float64, (L,), "f_ref")
+    F = 0 if disp is None else int(disp.shape[0])
+    if F:
+        _require(disp, t.float64, (F, L, 2), "disp")
+    steps = F if steps is None else int(steps)
+    P = 0 if sh_coeffs is None else int(sh_coeffs.shape[0])
+    if P:
+        _require(sh_coeffs, t.complex128, (P, L, 3, 3), "sh_coeffs")
+        _require(sh_amp, t.float64, (P, L, 3, 3), "sh_amp")
+    if noise is not None:
+        _require(noise, t.float32, (steps, 2, L, n, n), "noise")
+        if P:
+            _require(sh_noise, t.float64, (steps, 2, P, L, 3, 3), "sh_noise")
+    n_out = n if n_out is None else int(n_out)
+    if F:
+        if out is None:
+            out = t.empty((F, n_out, n_out), dtype=t.float32, device=spectra.device)
+        _require(out, t.float32, (F, n_out, n_out), "out")
+        need = int(_hip.lib().pt_boil_work_elems(n, F, L, P))
+        if need < 0:
+            raise ValueError("pt_boil_work_elems: bad arguments")
+        if work is None or work.numel() < need:
+            work = t.empty((need,), dtype=t.complex64, device=spectra.device)
+    rc = _hip.lib().pt_boiled_frames(
+        _hip.ptr(spectra), _hip.ptr(amplitude), _hip.ptr(inv_tau), _hip.ptr(f_ref), L, n,
+        float(pixel_scale), _hip.ptr(disp) if F else None, F, steps, float(dt),
+        _hip.ptr(sh_coeffs) if P else None, _hip.ptr(sh_amp) if P else None, P,
+        _hip.ptr(noise) if noise is not None else None,
+        _hip.ptr(sh_noise) if (noise is not None and P) else None,
+        int(seed) & 0xFFFFFFFFFFFFFFFF, int(step0), int(crop_offset), n_out, float(scale),
+        _hip.ptr(out) if F else None, _hip.ptr(work) if F else None, _hip.stream_ptr())
+    _hip.check(rc, "pt_boiled_frames")
+    return out if F else None

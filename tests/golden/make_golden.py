@@ -15,7 +15,14 @@ path that closed forms and identities do not fix on their own:
   882-998) on a 9-layer n=64 grid, on- and off-axis, and ``sample``;
 * the extrusion matrices (extrude.py:250-304) at W=32;
 * extruder readouts (extrude.py:754-915) for linear/cubic/lanczos on a
-  2-layer n=32 atmosphere, and InfinitePhaseScreen frames (infinite.py).
+  2-layer n=32 atmosphere, and InfinitePhaseScreen frames (infinite.py);
+* spectral boiling (atmosphere.py:618-658, 1092-1189, 1221-1243): the
+  per-mode time constants and the boiled frames of a 3-layer n=32
+  atmosphere through frames(), evolve() and reset(), which fix the boiling
+  stream's seed and draw order.
+
+``python tests/golden/make_golden.py /root/reference/src boil`` writes only
+the boiling fixture.
 
 Only data is written here; nothing from the reference is copied into the
 repository, and nothing at test time reads the reference.
@@ -42,10 +49,35 @@ def config2_layers(pyturb):
             for h, f, v, d in zip(heights, fracs, speeds, dirs)]
 
 
-def main(ref_src: str) -> None:
+BOIL_LAYERS = [(0.0, 0.5, 8.0, 30.0, 25.0), (3000.0, 0.3, 14.0, 110.0, 25.0),
+               (9000.0, 0.2, 21.0, 250.0, 25.0)]  # altitude, Cn2, speed, direction deg, L0
+BOIL_TAU = [0.02, float("inf"), 0.05]
+
+
+def write_boil(pyturb) -> None:
+    layers = [pyturb.Layer(*ly) for ly in BOIL_LAYERS]
+    atm = pyturb.Atmosphere(layers, r0=0.15, diameter=4.0, n=32, subharmonics=3,
+                            tau_boil=BOIL_TAU, dtype="float64", seed=6)
+    main_tau, sh_tau = atm._boil_main_tau, atm._boil_sh_tau
+    seq = [f.copy() for _, f in atm.frames(0.004, 5)]
+    seq.append(atm.evolve(0.004).copy())
+    seq.append(atm.opd(0.1).copy())  # opd does not boil
+    atm.reset()
+    again = [f.copy() for _, f in atm.frames(0.004, 2)]
+    np.savez_compressed(os.path.join(HERE, "boil_n32_seed6.npz"),
+                        main_tau=main_tau, sh_tau=sh_tau, frames=np.stack(seq),
+                        after_reset=np.stack(again))
+
+
+def main(ref_src: str, only: str = "") -> None:
     sys.path.insert(0, ref_src)
     import pyturb  # noqa: E402  (reference, this container only)
     from pyturb.extrude import ExtrudedAtmosphere, build_extrusion
+
+    if only == "boil":
+        write_boil(pyturb)
+        print("boiling fixture written to", HERE)
+        return
 
     # 1. generate: n=256, dx=1/32, r0=0.15, L0=25, 3 subharmonic levels, seed 1.
     gen = pyturb.PhaseScreen(n=256, pixel_scale=0.03125, r0=0.15, L0=25.0,
@@ -123,8 +155,12 @@ def main(ref_src: str) -> None:
                scr_inf.advance(40.0).copy()]
         inf[interp] = np.stack(seq)
     np.savez_compressed(os.path.join(HERE, "infinite_n32_seed5.npz"), **inf)
+
+    # 7. spectral boiling.
+    write_boil(pyturb)
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src",
+         sys.argv[2] if len(sys.argv) > 2 else "")

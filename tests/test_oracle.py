@@ -354,3 +354,90 @@ def test_noll_residuals_table_and_asymptote():
         (1.0299 - 0.134) * 40.0 ** (5 / 3))
     with pytest.raises(ValueError):
         O.noll_variance(1, 8.0, 0.2)
+
+
+# ---------------------------------------------------------------------------
+# spectral boiling (atmosphere.py:618-658, 1092-1189, 1221-1243)
+# ---------------------------------------------------------------------------
+_BOIL_LAYERS = [(0.0, 0.5, 8.0, 30.0, 25.0), (3000.0, 0.3, 14.0, 110.0, 25.0),
+                (9000.0, 0.2, 21.0, 250.0, 25.0)]
+_BOIL_TAU = [0.02, np.inf, 0.05]
+
+
+def _boil_oracle():
+    return O.OracleAtmosphere(_BOIL_LAYERS, 0.15, diameter=4.0, n=32, subharmonics=3,
+                              dtype="float64", seed=6, tau_boil=_BOIL_TAU)
+
+
+def test_boiling_matches_reference_golden():
+    """Seed tree, draw order, AR(1) update, frames/evolve/opd/reset."""
+    g = load_golden("boil_n32_seed6.npz")
+    atm = _boil_oracle()
+    main_tau, sh_tau = atm.boil_tau_maps()
+    assert np.allclose(main_tau, g["main_tau"], rtol=1e-12)
+    assert np.allclose(sh_tau, g["sh_tau"], rtol=1e-12)
+    seq = [f for _, f in atm.frames(0.004, 5)]
+    seq.append(atm.evolve(0.004))
+    seq.append(atm.opd(0.1))
+    assert O.rel_rms(np.stack(seq), g["frames"]) < 1e-12
+    atm.reset()
+    again = np.stack([f for _, f in atm.frames(0.004, 2)])
+    assert O.rel_rms(again, g["after_reset"]) < 1e-12
+
+
+def test_boil_tau_maps_closed_form():
+    """tau = tau_boil below f_ref, tau_boil (f/f_ref)^(-2/3) above; frozen
+    layers keep tau = inf (alpha = 1, nothing changes)."""
+    atm = _boil_oracle()
+    main_tau, _ = atm.boil_tau_maps()
+    f_ref = 1.0 / 25.0
+    fr = np.hypot(atm.f[:, None], atm.f[None, :])
+    expect = 0.02 * np.where(fr > f_ref, (fr / f_ref) ** (-2.0 / 3.0), 1.0)
+    assert np.allclose(main_tau[0], expect, rtol=1e-13)
+    assert np.all(np.isinf(main_tau[1]))
+    before = atm.spectra[1].copy()
+    atm.boil_step(0.01)
+    assert np.array_equal(atm.spectra[1], before)
+
+
+def test_boiling_is_stationary():
+    """The AR(1) step keeps each mode's variance: a stationary ensemble of
+    spectra stays stationary (mean |S|^2 / A^2 ~ 2 after many steps)."""
+    atm = _boil_oracle()
+    amp = np.stack([fl["amplitude"] for fl in atm.filters])
+    for _ in range(200):
+        atm.boil_step(0.01)
+    live = amp[0] > 0
+    ratio = np.mean(np.abs(atm.spectra[0][live]) ** 2 / amp[0][live] ** 2)
+    assert abs(ratio - 2.0) < 0.15
+
+
+def test_philox_known_answers_and_normals():
+    """Philox4x32-10 against the Random123 known-answer vectors (Salmon et
+    al. 2011), and the Box-Muller normals' first two moments."""
+    from oracle.pyturb_oracle import box_muller, philox4x32_10
+
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF),
+            (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, expect in kat:
+        got = philox4x32_10(np.array(ctr, dtype=np.uint32), key)
+        assert [int(v) for v in got] == list(expect)
+    ctr = np.zeros((200000, 4), dtype=np.uint32)
+    ctr[:, 0] = np.arange(200000)
+    r = philox4x32_10(ctr, (7, 9))
+    z = np.concatenate(box_muller(r[:, 0], r[:, 1]) + box_muller(r[:, 2], r[:, 3]))
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1.0) < 0.01
+
+
+def test_device_boil_noise_layout():
+    """Each mode and its mirror get distinct normals from one counter; the
+    stream depends only on (seed, step)."""
+    a, sa = O.device_boil_noise(99, 3, 2, 16, 2)
+    b, _ = O.device_boil_noise(99, 3, 2, 16, 2)
+    c, _ = O.device_boil_noise(99, 4, 2, 16, 2)
+    assert np.array_equal(a, b) and not np.allclose(a, c)
+    assert a.shape == (2, 2, 16, 16) and sa.shape == (2, 2, 2, 3, 3)
+    assert len(np.unique(a[0, 0])) == 256  # no mode shares a value with its mirror
