@@ -92,6 +92,33 @@ def run_frames(atm, disp, out, work, events=None):
         events[2].record()
 
 
+BOIL_TAU = 0.2  # s, every layer: the reference benchmark's boiling case (bench_suite.py:157)
+
+
+def build_boil(dev):
+    """The same 9-layer 512^2 atmosphere, boiling with tau_boil = 0.2 s."""
+    import pyturb_amd as pa
+    from pyturb_amd import _hip
+
+    atm = pa.Atmosphere(profile_layers(), r0=0.15, diameter=8.0, n=N, L0=25.0, subharmonics=8,
+                        seed=2, tau_boil=BOIL_TAU, device="mi355x")
+    times = np.arange(N_FRAMES) * DT
+    dx = atm._vx[None, :] * times[:, None]
+    dy = atm._vy[None, :] * times[:, None]
+    disp = _hip.to_device(np.stack([dx, dy], axis=-1), np.float64, dev)
+    return atm, disp
+
+
+def run_boil(atm, disp, out, work, step0):
+    """2000 boiled frames (a boiling step after each) from the device stream."""
+    from pyturb_amd import ops
+
+    ops.boiled_frames(atm._spectra, atm._boil_amp, atm._boil_inv_tau, atm._boil_f_ref, disp, DT,
+                      atm.pixel_scale, sh_coeffs=atm._sh_coeffs, sh_amp=atm._boil_sh_amp,
+                      seed=atm._boil_seed, step0=step0, scale=atm.wavelength / (2.0 * np.pi),
+                      out=out, work=work)
+
+
 def plain_frame(atm, t, ctx):
     """The reference's per-frame algorithm as plain torch ops, per layer."""
     import torch
@@ -343,6 +370,7 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte-Carlo ensemble")
     ap.add_argument("--no-extrude", action="store_true", help="skip the extrusion workload")
+    ap.add_argument("--no-boil", action="store_true", help="skip the boiling-frames workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (counter scaling)")
     args = ap.parse_args(argv)
@@ -424,6 +452,30 @@ def main(argv=None):
             extra["mc_plain_screens_per_s"] = round(nscr / tpm, 1)
             extra["fused_vs_plain_mc"] = round(mc_rate / (nscr / tpm), 1)
         del classes
+        torch.cuda.empty_cache()
+    if not args.no_boil:
+        print("bench: boiling frames", file=sys.stderr, flush=True)
+        batm, bdisp = build_boil(dev)
+        need = int(_hip.lib().pt_boil_work_elems(N, N_FRAMES, L, batm.subharmonics))
+        bwork = torch.empty((need,), dtype=torch.complex64, device=dev)
+        steps_done = [0]
+
+        def boil_once():
+            run_boil(batm, bdisp, out, bwork, steps_done[0])
+            steps_done[0] += N_FRAMES
+
+        tb = timed(boil_once, max(1, args.steps // 2), 1)
+        extra["boil_frames_per_s"] = round(N_FRAMES * max(1, args.steps // 2) / tb, 1)
+
+        def boil_api():  # the public iterator, chunked device calls
+            for _ in batm.frames(DT, N_FRAMES):
+                pass
+
+        ta = timed(boil_api, 1, 1)
+        extra["boil_api_frames_per_s"] = round(N_FRAMES / ta, 1)
+        extra["boil_tau_s"] = BOIL_TAU
+        extra["reference_published_rtx5090_boil_fps_512"] = 2134
+        del batm, bdisp, bwork
         torch.cuda.empty_cache()
     if not args.no_extrude:
         print("bench: building 9-layer 1024^2 extruder (host FP64 A/B)", file=sys.stderr, flush=True)

@@ -873,9 +873,10 @@ def philox4x32_10(ctr, key):
 
 
 def box_muller(a, b):
-    """Two standard normals from 32-bit words: u1 in (0, 1], u2 in [0, 1)."""
+    """Two standard normals from 32-bit words: u1 = (a + 1) 2^-32 in (0, 1],
+    u2 = (b >> 8) 2^-24 in [0, 1) (the angle takes the top 24 bits)."""
     u1 = (np.asarray(a, np.float64) + 1.0) * 2.0 ** -32
-    u2 = np.asarray(b, np.float64) * 2.0 ** -32
+    u2 = (np.asarray(b, np.uint32) >> np.uint32(8)).astype(np.float64) * 2.0 ** -24
     r = np.sqrt(-2.0 * np.log(u1))
     return r * np.cos(2.0 * np.pi * u2), r * np.sin(2.0 * np.pi * u2)
 
