@@ -248,6 +248,7 @@ def _plain_gen_ctx(grp, n):
 EXT_N = 1024
 EXT_STEPS = 1000
 LGS_ALT = 90e3
+LGS_FRAMES = 512  # spectral LGS-cone frames per timed call (9 per-layer transforms each)
 
 
 def build_extrude(dev):
@@ -371,6 +372,7 @@ def main(argv=None):
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte-Carlo ensemble")
     ap.add_argument("--no-extrude", action="store_true", help="skip the extrusion workload")
     ap.add_argument("--no-boil", action="store_true", help="skip the boiling-frames workload")
+    ap.add_argument("--no-lgs", action="store_true", help="skip the LGS-cone frames workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (counter scaling)")
     args = ap.parse_args(argv)
@@ -476,6 +478,18 @@ def main(argv=None):
         extra["boil_tau_s"] = BOIL_TAU
         extra["reference_published_rtx5090_boil_fps_512"] = 2134
         del batm, bdisp, bwork
+        torch.cuda.empty_cache()
+    if not args.no_lgs:
+        print("bench: LGS-cone frames", file=sys.stderr, flush=True)
+        import pyturb_amd as pa
+
+        latm = pa.Atmosphere(profile_layers(), r0=0.15, diameter=8.0, n=N, L0=25.0,
+                             subharmonics=8, seed=2, lgs_altitude=LGS_ALT, device="mi355x")
+        ltimes = np.arange(LGS_FRAMES) * DT
+        tl = timed(lambda: latm.opd_batch(ltimes), 1, 1)
+        extra["lgs_frames_per_s"] = round(LGS_FRAMES / tl, 1)
+        extra["reference_published_rtx5090_lgs_fps_512"] = 793
+        del latm
         torch.cuda.empty_cache()
     if not args.no_extrude:
         print("bench: building 9-layer 1024^2 extruder (host FP64 A/B)", file=sys.stderr, flush=True)

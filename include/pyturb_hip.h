@@ -315,6 +315,24 @@ int pt_boiled_frames(void* spectra, const float* amplitude,
                      unsigned long long step0, int crop_offset, int n_out,
                      float scale, float* out, void* work, void* stream);
 
+/*
+ * Spectral LGS cone readout: every layer's screen sampled at the pupil grid
+ * scaled about the screen centre by the layer's cone magnification, with
+ * separable taps, and the layers summed:
+ *   out[f][i][j] = scale * sum_l sum_a sum_b w[l][a][i] w[l][b][j]
+ *                  * screens[l][f][idx[l][a][i]][idx[l][b][j]]
+ *   screens : (layers, frames, ns, ns) f32 per-layer screens (each layer
+ *             transformed on its own, subharmonics included)
+ *   idx, w  : (layers, taps, n) int32 / f32 device tap tables, idx clipped
+ *             to [0, ns) and non-decreasing along both the pixel and the tap
+ *             axis (as src/pyturb/atmosphere.py:581-616 builds them)
+ *   taps    : 2 linear, 4 cubic, 6 lanczos;  out : (frames, n, n) f32.
+ * Replaces Atmosphere._lgs_zoom (src/pyturb/atmosphere.py:1047-1090).
+ */
+int pt_lgs_zoom(const float* screens, int layers, int frames, int ns,
+                const int* idx, const float* w, int taps, int n, float scale,
+                float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

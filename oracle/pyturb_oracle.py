@@ -58,6 +58,8 @@ __all__ = [
     "flow_translate",
     "integrate",
     "integrate_dirs",
+    "lgs_zoom_taps",
+    "integrate_lgs",
     "OracleAtmosphere",
     "phase_covariance",
     "build_extrusion",
@@ -275,6 +277,46 @@ def integrate(spectra, sh_coeffs, f, sh_freq, sh_basis, disp_x, disp_y,
     return total
 
 
+def lgs_zoom_taps(n, n_screen, mag, interp="cubic"):
+    """atmosphere.py:581-616 -- per-layer cone-zoom taps: screen index
+    (clipped to the screen) and weight, (L, T, n) each, for the pupil grid
+    scaled about the screen centre by each layer's magnification."""
+    centre = (n_screen - 1) / 2.0
+    base = np.arange(n, dtype=np.float64) - (n - 1) / 2.0
+    idx, wts = [], []
+    for m in np.atleast_1d(mag):
+        pos = centre + base * float(m)
+        p0 = np.floor(pos).astype(np.int64)
+        offs, w = _taps(interp, pos - p0)
+        idx.append(np.stack([np.clip(p0 + o, 0, n_screen - 1) for o in offs]))
+        wts.append(np.stack([np.asarray(v, float) for v in w]))
+    return np.stack(idx), np.stack(wts)
+
+
+def integrate_lgs(spectra, sh_coeffs, f, sh_freq, sh_basis, disp_x, disp_y,
+                  zoom_idx, zoom_w):
+    """atmosphere.py:1000-1090 -- LGS-cone frame, float64.
+
+    Each layer's shifted spectrum is transformed on its own, its own
+    subharmonic field (mean removed per layer) added, and the layer screen
+    sampled by the separable cone-zoom taps; the layers are then summed.
+    """
+    px = _shift_phasor(disp_x, f)  # (L, n)
+    py = _shift_phasor(disp_y, f)
+    screens = ifft2_unnormalised(spectra * px[:, :, None] * py[:, None, :]).real
+    P = sh_coeffs.shape[0]
+    if P:
+        qx = np.exp(2j * np.pi * np.asarray(disp_x)[None, :, None] * sh_freq[:, None, :])
+        qy = np.exp(2j * np.pi * np.asarray(disp_y)[None, :, None] * sh_freq[:, None, :])
+        shifted = sh_coeffs * qx[:, :, :, None] * qy[:, :, None, :]  # (P, L, 3, 3)
+        screens = screens + subharmonic_field(np.moveaxis(shifted, 0, 1), sh_basis)
+    total = 0.0
+    for l in range(screens.shape[0]):
+        rows = np.einsum("ti,tic->ic", zoom_w[l], screens[l][zoom_idx[l]])  # (n, ns)
+        total = total + np.einsum("tj,itj->ij", zoom_w[l], rows[:, zoom_idx[l]])
+    return total
+
+
 def integrate_dirs(spectra, sh_coeffs, f, sh_freq, sh_basis, disp_x, disp_y,
                    crop=None):
     """atmosphere.py:940-998 -- (D,L) displacements -> (D, n, n)."""
@@ -303,7 +345,8 @@ class OracleAtmosphere:
     def __init__(self, layers, r0, wavelength=500e-9, zenith_angle=0.0,
                  diameter=8.0, n=512, L0=None, power_law=11.0 / 3.0,
                  inner_scale=0.0, subharmonics=8, field_of_view=0.0,
-                 dtype="float32", seed=None, tau_boil=None):
+                 dtype="float32", seed=None, tau_boil=None, lgs_altitude=None,
+                 interp="cubic"):
         lay = [tuple(float(v) for v in ly) for ly in layers]
         frac = np.array([ly[1] for ly in lay])
         frac = frac / frac.sum()
@@ -362,6 +405,12 @@ class OracleAtmosphere:
         self.sh_basis = self.filters[0]["sh_basis"]
         self.L0 = np.array([ly[4] for ly in lay])
         self.t = 0.0
+        self.lgs_mag = None
+        if lgs_altitude is not None:  # atmosphere.py:484-498, 578-579
+            self.lgs_mag = 1.0 - self.alt / (lgs_altitude * self.airmass)
+            if np.any(self.lgs_mag <= 0.0):
+                raise ValueError("lgs_altitude must exceed every layer altitude")
+            self.zoom_idx, self.zoom_w = lgs_zoom_taps(self.n, self.ns, self.lgs_mag, interp)
         self.boil_rng = np.random.default_rng(self.boil_seed)
         # sample() generators, one per distinct round(L0, 9)
         groups: Dict[float, List[int]] = {}
@@ -382,6 +431,9 @@ class OracleAtmosphere:
 
     def phase(self, t, ox=0.0, oy=0.0):
         sx, sy = self.displacements(t, ox, oy)
+        if self.lgs_mag is not None:  # atmosphere.py:894-895
+            return integrate_lgs(self.spectra, self.sh_coeffs, self.f, self.sh_freq,
+                                 self.sh_basis, sx, sy, self.zoom_idx, self.zoom_w)
         return integrate(self.spectra, self.sh_coeffs, self.f, self.sh_freq,
                          self.sh_basis, sx, sy, self.crop)
 

@@ -18,10 +18,10 @@ Device design:
 * ``sample`` -- one fused ``pt_generate`` call per outer-scale group,
   accumulating into the cropped OPD output.
 
-Spectral boiling (``tau_boil`` on ``engine="spectral"``) runs on the device
-(pt_boiled_frames).  Extruder boiling, the spectral LGS cone and chromatic
-dispersion models are outside this backend and raise
-``NotImplementedError``.
+Spectral boiling (``tau_boil`` on ``engine="spectral"``, pt_boiled_frames)
+and the spectral LGS cone (``lgs_altitude``, per-layer transforms and
+pt_lgs_zoom) run on the device.  Extruder boiling and chromatic dispersion
+models are outside this backend and raise ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -44,6 +44,45 @@ __all__ = ["Atmosphere", "PeriodicWrapWarning", "ExtrudeBoilingPerformanceWarnin
 
 _ARCSEC_TO_RAD = np.pi / (180.0 * 3600.0)
 _FRAME_CHUNK = 64
+_LGS_CHUNK = 64  # frames per per-layer transform batch of the LGS cone
+
+
+def _lgs_zoom_taps(n: int, n_screen: int, mag, interp: str):
+    """Per-layer cone-zoom taps (atmosphere.py:581-616), FP64 on the host.
+
+    The pupil grid scaled about the screen centre by each layer's
+    magnification: positions ``(ns - 1)/2 + (i - (n - 1)/2) mag``, screen
+    indices clipped to the screen, weights of ``interp`` (linear; Catmull-Rom
+    of extrude.py:74-83; normalised Lanczos-3 of infinite.py:67-85).
+    Returns idx (L, T, n) int32 and w (L, T, n) float32.
+    """
+    centre = 0.5 * (n_screen - 1)
+    g = np.arange(n, dtype=np.float64) - 0.5 * (n - 1)
+    idx, wts = [], []
+    for m in mag:
+        pos = centre + g * float(m)
+        p0 = np.floor(pos)
+        t = pos - p0
+        if interp == "linear":
+            offs, w = (0, 1), (1.0 - t, t)
+        elif interp == "lanczos":
+            offs = tuple(range(-2, 4))
+            raw = [np.sinc(t - d) * np.sinc((t - d) / 3.0) for d in offs]
+            tot = sum(raw[1:], raw[0])
+            w = tuple(r / tot for r in raw)
+        else:
+            t2, t3 = t * t, t * t * t
+            offs = (-1, 0, 1, 2)
+            w = (0.5 * (-t + 2.0 * t2 - t3), 0.5 * (2.0 - 5.0 * t2 + 3.0 * t3),
+                 0.5 * (t + 4.0 * t2 - 3.0 * t3), 0.5 * (t3 - t2))
+        idx.append(np.stack([np.clip(p0 + o, 0, n_screen - 1) for o in offs]))
+        wts.append(np.stack(w))
+    idx = np.stack(idx).astype(np.int32)
+    # the device zoom stages windows [idx[:, 0, i0], idx[:, -1, i1]]: both
+    # axes must be non-decreasing
+    if np.any(np.diff(idx, axis=1) < 0) or np.any(np.diff(idx, axis=2) < 0):
+        raise ValueError("LGS zoom taps are not monotone")
+    return idx, np.stack(wts).astype(np.float32)
 _BOIL_CHUNK = 128  # boiled frames per device call
 
 
@@ -135,10 +174,9 @@ class Atmosphere:
                 raise ValueError(
                     f"lgs_altitude ({self.lgs_altitude:.0f} m) must exceed every layer "
                     "altitude (projected to the line of sight)")
-            if self.engine == "spectral":
-                raise NotImplementedError(
-                    "the spectral-engine LGS cone is not provided by the MI355X backend; "
-                    "use engine='extrude'")
+        # per-layer cone magnification of the spectral engine (atmosphere.py:484-498)
+        self._lgs_mag = mag if (self.lgs_altitude is not None and self.engine == "spectral") \
+            else None
 
         # Seed tree (atmosphere.py:393-437): master -> per-layer children,
         # two boiling streams, then (gen, flow, ext) per layer.
@@ -228,6 +266,10 @@ class Atmosphere:
         self._sh_coeffs = _hip.to_device(coeffs, None, dev) if P else None
         self._spectra_host_c64 = None
         self._boil_index = 0
+        if self._lgs_mag is not None:
+            idx, w = _lgs_zoom_taps(self.n, ns, self._lgs_mag, self.interp)
+            self._zoom_idx = _hip.to_device(idx, None, dev)
+            self._zoom_w = _hip.to_device(w, None, dev)
         if self._boiling:
             t = _hip.torch()
             gens = [st.generator for st in self._layers]
@@ -335,9 +377,34 @@ class Atmosphere:
         if not self._wrap_warned:
             self._check_wrap(dx, dy)
         disp = np.stack([dx, dy], axis=-1).reshape(-1, len(self._layers), 2)
+        if self._lgs_mag is not None:
+            return self._lgs_frames(disp, scale)
         disp_dev = _hip.to_device(disp, np.float64, self._dev)
         return ops.spectral_frames(self._spectra, disp_dev, self._sh_coeffs, self.pixel_scale,
                                    crop_offset=self.margin_pix, n_out=self.n, scale=scale)
+
+    def _lgs_frames(self, disp, scale):
+        """LGS-cone frames (atmosphere.py:1000-1090): each layer transformed
+        on its own over a chunk of frames (pt_spectral_frames with one
+        layer, subharmonics and their mean included), then the cone zoom and
+        the layer sum (pt_lgs_zoom)."""
+        t = _hip.torch()
+        F, L, ns = disp.shape[0], len(self._layers), self.n_screen
+        out = t.empty((F, self.n, self.n), dtype=t.float32, device=self._dev)
+        k = min(F, _LGS_CHUNK)
+        screens = t.empty((L, k, ns, ns), dtype=t.float32, device=self._dev)
+        for f0 in range(0, F, k):
+            m = min(k, F - f0)
+            for l in range(L):
+                d = _hip.to_device(np.ascontiguousarray(disp[f0:f0 + m, l:l + 1]), np.float64,
+                                   self._dev)
+                cf = (self._sh_coeffs[:, l:l + 1].contiguous() if self._sh_coeffs is not None
+                      else None)
+                ops.spectral_frames(self._spectra[l:l + 1], d, cf, self.pixel_scale,
+                                    n_out=ns, out=screens[l, :m])
+            ops.lgs_zoom(screens[:, :m] if m == k else screens[:, :m].contiguous(),
+                         self._zoom_idx, self._zoom_w, self.n, scale=scale, out=out[f0:f0 + m])
+        return out
 
     def opd(self, t: float = 0.0, directions=None, wavelength: Optional[float] = None) -> Any:
         """OPD [m] (or phase at ``wavelength``) at time ``t`` [s]."""
@@ -463,6 +530,15 @@ class Atmosphere:
         while a frame is out takes effect as in the reference: the clock
         then advances and the fresh state boils once.
         """
+        if self._lgs_mag is not None:
+            # the cone zooms every layer on its own, so each frame is an LGS
+            # frame of the current state followed by one boiling step
+            for _ in range(steps):
+                t0 = self._t
+                yield t0, self._cast(self._spectral([t0], np.zeros(1), np.zeros(1), scale)[0])
+                self._t += dt
+                self._boil_step(dt)
+            return
         left = steps
         while left > 0:
             k = 1 if self._boil_noise is not None else min(left, _BOIL_CHUNK)

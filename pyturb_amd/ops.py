@@ -303,3 +303,24 @@ def boiled_frames(spectra, amplitude, inv_tau, f_ref, disp, dt: float, pixel_sca
         _hip.ptr(out) if F else None, _hip.ptr(work) if F else None, _hip.stream_ptr())
     _hip.check(rc, "pt_boiled_frames")
     return out if F else None
+
+
+def lgs_zoom(screens, idx, w, n: int, *, scale: float = 1.0, out=None):
+    """Cone-zoomed layer sum (pt_lgs_zoom).
+
+    ``screens`` (L, F, ns, ns) float32; ``idx`` (L, T, n) int32 and ``w``
+    (L, T, n) float32 device tap tables.  Returns (F, n, n) float32.
+    """
+    t = _t()
+    L, F, ns = int(screens.shape[0]), int(screens.shape[1]), int(screens.shape[-1])
+    _require(screens, t.float32, (L, F, ns, ns), "screens")
+    T = int(idx.shape[1])
+    _require(idx, t.int32, (L, T, n), "idx")
+    _require(w, t.float32, (L, T, n), "w")
+    if out is None:
+        out = t.empty((F, n, n), dtype=t.float32, device=screens.device)
+    _require(out, t.float32, (F, n, n), "out")
+    rc = _hip.lib().pt_lgs_zoom(_hip.ptr(screens), L, F, ns, _hip.ptr(idx), _hip.ptr(w), T, n,
+                                float(scale), _hip.ptr(out), _hip.stream_ptr())
+    _hip.check(rc, "pt_lgs_zoom")
+    return out

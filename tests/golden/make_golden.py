@@ -21,8 +21,12 @@ path that closed forms and identities do not fix on their own:
   atmosphere through frames(), evolve() and reset(), which fix the boiling
   stream's seed and draw order.
 
-``python tests/golden/make_golden.py /root/reference/src boil`` writes only
-the boiling fixture.
+* the spectral LGS cone (atmosphere.py:581-616, 1000-1090): frames of a
+  3-layer n=32 atmosphere for a 20 km beacon, on- and off-axis, with each
+  interpolation.
+
+``python tests/golden/make_golden.py /root/reference/src boil`` (or ``lgs``)
+writes only that fixture.
 
 Only data is written here; nothing from the reference is copied into the
 repository, and nothing at test time reads the reference.
@@ -69,14 +73,28 @@ def write_boil(pyturb) -> None:
                         after_reset=np.stack(again))
 
 
+LGS_DIRS = [(0.0, 0.0), (6.0, -4.0)]  # arcsec, inside the 10" field
+
+
+def write_lgs(pyturb) -> None:
+    out = {}
+    for interp in ("linear", "cubic", "lanczos"):
+        atm = pyturb.Atmosphere([pyturb.Layer(*ly) for ly in BOIL_LAYERS], r0=0.15,
+                                diameter=4.0, n=32, subharmonics=3, field_of_view=10.0,
+                                lgs_altitude=20e3, interp=interp, dtype="float64", seed=7)
+        out[interp] = np.stack([atm.opd(t, directions=LGS_DIRS) for t in (0.0, 0.013)])
+        out[interp + "_mag"] = atm._lgs_mag
+    np.savez_compressed(os.path.join(HERE, "lgs_n32_seed7.npz"), **out)
+
+
 def main(ref_src: str, only: str = "") -> None:
     sys.path.insert(0, ref_src)
     import pyturb  # noqa: E402  (reference, this container only)
     from pyturb.extrude import ExtrudedAtmosphere, build_extrusion
 
-    if only == "boil":
-        write_boil(pyturb)
-        print("boiling fixture written to", HERE)
+    if only in ("boil", "lgs"):
+        (write_boil if only == "boil" else write_lgs)(pyturb)
+        print(only, "fixture written to", HERE)
         return
 
     # 1. generate: n=256, dx=1/32, r0=0.15, L0=25, 3 subharmonic levels, seed 1.
@@ -156,8 +174,9 @@ def main(ref_src: str, only: str = "") -> None:
         inf[interp] = np.stack(seq)
     np.savez_compressed(os.path.join(HERE, "infinite_n32_seed5.npz"), **inf)
 
-    # 7. spectral boiling.
+    # 7. spectral boiling; 8. spectral LGS cone.
     write_boil(pyturb)
+    write_lgs(pyturb)
     print("golden fixtures written to", HERE)
 
 

@@ -441,3 +441,32 @@ def test_device_boil_noise_layout():
     assert np.array_equal(a, b) and not np.allclose(a, c)
     assert a.shape == (2, 2, 16, 16) and sa.shape == (2, 2, 2, 3, 3)
     assert len(np.unique(a[0, 0])) == 256  # no mode shares a value with its mirror
+
+
+# ---------------------------------------------------------------------------
+# spectral LGS cone (atmosphere.py:581-616, 1000-1090)
+# ---------------------------------------------------------------------------
+LGS_DIRS = [(0.0, 0.0), (6.0, -4.0)]
+
+
+def test_lgs_frames_match_reference_golden():
+    g = load_golden("lgs_n32_seed7.npz")
+    for interp in ("linear", "cubic", "lanczos"):
+        atm = O.OracleAtmosphere(_BOIL_LAYERS, 0.15, diameter=4.0, n=32, subharmonics=3,
+                                 field_of_view=10.0, dtype="float64", seed=7,
+                                 lgs_altitude=20e3, interp=interp)
+        assert np.allclose(atm.lgs_mag, g[interp + "_mag"], rtol=1e-14)
+        got = np.stack([atm.opd(t, directions=LGS_DIRS) for t in (0.0, 0.013)])
+        assert O.rel_rms(got, g[interp]) < 1e-12, interp
+
+
+def test_lgs_zoom_taps_identities():
+    """mag = 1 on an unpadded grid samples the screen exactly (identity
+    taps); the weights of every output pixel sum to 1."""
+    for interp in ("linear", "cubic", "lanczos"):
+        idx, w = O.lgs_zoom_taps(16, 16, [1.0, 0.7], interp)
+        assert np.allclose(w.sum(axis=1), 1.0)
+        screen = np.random.default_rng(0).standard_normal((16, 16))
+        rows = np.einsum("ti,tic->ic", w[0], screen[idx[0]])
+        out = np.einsum("tj,itj->ij", w[0], rows[:, idx[0]])
+        assert np.allclose(out, screen, atol=1e-12)
