@@ -42,8 +42,11 @@ __device__ __forceinline__ Ctr4 philox4x32_10(Ctr4 c, uint32_t k0, uint32_t k1) 
   constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;  // Weyl key increments
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-    const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+    // one 32 x 32 -> 64-bit multiply each (v_mad_u64_u32) instead of a
+    // separate high and low product
+    const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     c = Ctr4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
     k0 += W0;
     k1 += W1;
@@ -99,22 +102,29 @@ __global__ void k_boil_phase(const double* __restrict__ disp, int count, double 
 // ---------------------------------------------------------------------------
 // the fused boil + layer-sum kernel
 // ---------------------------------------------------------------------------
-// Thread (k0, k1) with k0 <= n/2 owns mode e = (k0, k1) and its mirror
-// m = (-k0, -k1) mod n when e < m, or e alone when e = m (the four
-// self-conjugate modes); threads with e > m (half of rows 0 and n/2) idle.
-// |f| and hence a = exp(-dt/tau) are equal at e and m.  Instantiated per
-// layer count so every per-layer array stays in registers (a run-time
-// bound leaves them in scratch memory beyond 8 layers).
-template <int LB>
+// A quad of lanes (LP = 4 for 3+ layers) owns mode e = (k0, k1), k0 <= n/2,
+// and its mirror m = (-k0, -k1) mod n when e < m, or e alone when e = m
+// (the four self-conjugate modes); quads with e > m (half of rows 0 and
+// n/2) idle.  Lane g of the quad holds layers g, g + LP, ...: each frame's
+// layer sums are reduced across the quad with lane shuffles, so per-lane
+// state is a quarter of the stack and four times as many waves hide the
+// latency of the per-layer transcendental chain.  |f| and hence
+// a = exp(-dt/tau) are equal at e and m.  Instantiated per layer count, so
+// every per-layer array stays in registers.
+template <int L>
 __global__ __launch_bounds__(256) void k_boil_pairs(BoilArgs b) {
+  constexpr int LP = L >= 3 ? 4 : L;         // lanes per mode pair
+  constexpr int JB = (L + LP - 1) / LP;      // layers per lane
   const int n = b.n;
   const size_t plane = (size_t)n * n;
   const size_t tidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tidx >= (size_t)(n / 2 + 1) * n) return;
-  const int k0 = (int)(tidx / n), k1 = (int)(tidx % n);
+  if (tidx >= (size_t)(n / 2 + 1) * n * LP) return;  // whole quads
+  const int g = (int)(tidx % LP);
+  const size_t pidx = tidx / LP;
+  const int k0 = (int)(pidx / n), k1 = (int)(pidx % n);
   const int mr0 = (n - k0) % n, m1 = (n - k1) % n;
   const size_t e = (size_t)k0 * n + k1, m = (size_t)mr0 * n + m1;
-  if (e > m) return;
+  if (e > m) return;  // the whole quad
   const bool self = e == m;
   const uint32_t k0s = (uint32_t)signed_freq(k0, n), k1s = (uint32_t)signed_freq(k1, n);
   const uint32_t m0s = (uint32_t)signed_freq(mr0, n), m1s = (uint32_t)signed_freq(m1, n);
@@ -123,75 +133,90 @@ __global__ __launch_bounds__(256) void k_boil_pairs(BoilArgs b) {
   const bool nyq = 2 * k0 == n || 2 * k1 == n;
   const double fx = (double)(int32_t)k0s * b.df, fy = (double)(int32_t)k1s * b.df;
   const double fr = sqrt(fx * fx + fy * fy);
-  float2 se[LB], sm[LB];
-  float a[LB], ce[LB], cm[LB];
+  float2 se[JB], sm[JB];
+  float a[JB], ce[JB], cm[JB];
 #pragma unroll
-  for (int l = 0; l < LB; ++l) {
-    se[l] = b.spectra[(size_t)l * plane + e];
-    sm[l] = b.spectra[(size_t)l * plane + m];
-    // a = exp(-dt / tau), 1/tau = inv_tau * max(fr / f_ref, 1)^(2/3); FP64 setup once per call
-    const double ratio = fmax(fr / b.f_ref[l], 1.0);
-    const double al = exp(-b.dt * b.inv_tau[l] * cbrt(ratio * ratio));
-    const double bl = sqrt(fmax(1.0 - al * al, 0.0));
-    a[l] = (float)al;
-    ce[l] = (float)(bl * b.amp[(size_t)l * plane + e]);
-    cm[l] = (float)(bl * b.amp[(size_t)l * plane + m]);
+  for (int j = 0; j < JB; ++j) {
+    const int l = g + LP * j;
+    se[j] = sm[j] = make_float2(0.f, 0.f);
+    a[j] = 1.f;
+    ce[j] = cm[j] = 0.f;
+    if (l < L) {
+      se[j] = b.spectra[(size_t)l * plane + e];
+      sm[j] = b.spectra[(size_t)l * plane + m];
+      // a = exp(-dt / tau), 1/tau = inv_tau * max(fr / f_ref, 1)^(2/3); FP64 setup once per call
+      const double ratio = fmax(fr / b.f_ref[l], 1.0);
+      const double al = exp(-b.dt * b.inv_tau[l] * cbrt(ratio * ratio));
+      const double bl = sqrt(fmax(1.0 - al * al, 0.0));
+      a[j] = (float)al;
+      ce[j] = (float)(bl * b.amp[(size_t)l * plane + e]);
+      cm[j] = (float)(bl * b.amp[(size_t)l * plane + m]);
+    }
   }
   float2 h1 = make_float2(0.f, 0.f);
   const uint32_t mode = (uint32_t)e;
   for (int s = 0; s < b.steps; ++s) {
     if (s < b.frames) {
-      const uint2* ph = b.phase + (size_t)s * b.L;
+      const uint2* ph = b.phase + (size_t)s * L;
       float2 xe = make_float2(0.f, 0.f), xm = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int l = 0; l < LB; ++l) {
-        const uint2 u = ph[l];
-        const float2 pe = phasor_rev(k0s * u.x + k1s * u.y);
-        xe = cadd(xe, cmul(se[l], pe));
-        if (!self) {
+      for (int j = 0; j < JB; ++j) {
+        const int l = g + LP * j;
+        if (l < L) {
+          const uint2 u = ph[l];
+          const float2 pe = phasor_rev(k0s * u.x + k1s * u.y);
+          xe = cadd(xe, cmul(se[j], pe));
           const float2 pm = nyq ? phasor_rev(m0s * u.x + m1s * u.y) : make_float2(pe.x, -pe.y);
-          xm = cadd(xm, cmul(sm[l], pm));
+          xm = cadd(xm, cmul(sm[j], pm));
         }
+      }
+#pragma unroll
+      for (int o = 1; o < LP; o <<= 1) {
+        xe.x += __shfl_xor(xe.x, o, LP);
+        xe.y += __shfl_xor(xe.y, o, LP);
+        xm.x += __shfl_xor(xm.x, o, LP);
+        xm.y += __shfl_xor(xm.y, o, LP);
       }
       if (self) xm = xe;
       // Herm X(e) = (X(e) + conj X(m)) / 2, Herm X(m) = conj Herm X(e)
       const float2 h = make_float2(0.5f * (xe.x + xm.x), 0.5f * (xe.y - xm.y));
-      if ((s & 1) == 0) {
-        h1 = h;
-      }
-      if ((s & 1) == 1 || s == b.frames - 1) {
+      if ((s & 1) == 0) h1 = h;
+      if (g == 0 && ((s & 1) == 1 || s == b.frames - 1)) {
         const float2 h2 = (s & 1) ? h : make_float2(0.f, 0.f);
         float2* z = b.z + (size_t)(s >> 1) * plane;
         z[e] = make_float2(h1.x - h2.y, h1.y + h2.x);  // h1 + i h2
         if (!self) z[m] = make_float2(h1.x + h2.y, h2.x - h1.y);  // conj h1 + i conj h2
       }
     }
-    // AR(1) step of both modes of every layer
-    const unsigned long long g = b.step0 + (unsigned long long)s;
+    // AR(1) step of both modes of this lane's layers
+    const unsigned long long gs = b.step0 + (unsigned long long)s;
 #pragma unroll
-    for (int l = 0; l < LB; ++l) {
-      if (ce[l] == 0.f && cm[l] == 0.f && a[l] == 1.f) continue;  // frozen layer
+    for (int j = 0; j < JB; ++j) {
+      const int l = g + LP * j;
+      if (l >= L || (ce[j] == 0.f && cm[j] == 0.f && a[j] == 1.f)) continue;  // frozen layer
       float2 ne, nm;
       if (b.noise) {
-        const float* nz = b.noise + (size_t)s * 2 * b.L * plane + (size_t)l * plane;
-        ne = make_float2(nz[e], nz[(size_t)b.L * plane + e]);
-        nm = make_float2(nz[m], nz[(size_t)b.L * plane + m]);
+        const float* nz = b.noise + (size_t)s * 2 * L * plane + (size_t)l * plane;
+        ne = make_float2(nz[e], nz[(size_t)L * plane + e]);
+        nm = make_float2(nz[m], nz[(size_t)L * plane + m]);
       } else {
-        const Ctr4 r = philox4x32_10(Ctr4{mode, (uint32_t)l, (uint32_t)g, (uint32_t)(g >> 32)}, b.key0, b.key1);
+        const Ctr4 r = philox4x32_10(Ctr4{mode, (uint32_t)l, (uint32_t)gs, (uint32_t)(gs >> 32)}, b.key0, b.key1);
         ne = box_muller(r.x, r.y);
         nm = box_muller(r.z, r.w);
       }
-      se[l] = make_float2(fmaf(a[l], se[l].x, ce[l] * ne.x), fmaf(a[l], se[l].y, ce[l] * ne.y));
+      se[j] = make_float2(fmaf(a[j], se[j].x, ce[j] * ne.x), fmaf(a[j], se[j].y, ce[j] * ne.y));
       if (self)
-        sm[l] = se[l];
+        sm[j] = se[j];
       else
-        sm[l] = make_float2(fmaf(a[l], sm[l].x, cm[l] * nm.x), fmaf(a[l], sm[l].y, cm[l] * nm.y));
+        sm[j] = make_float2(fmaf(a[j], sm[j].x, cm[j] * nm.x), fmaf(a[j], sm[j].y, cm[j] * nm.y));
     }
   }
 #pragma unroll
-  for (int l = 0; l < LB; ++l) {
-    b.spectra[(size_t)l * plane + e] = se[l];
-    if (!self) b.spectra[(size_t)l * plane + m] = sm[l];
+  for (int j = 0; j < JB; ++j) {
+    const int l = g + LP * j;
+    if (l >= L) continue;
+    b.spectra[(size_t)l * plane + e] = se[j];
+    if (!self) b.spectra[(size_t)l * plane + m] = sm[j];
   }
 }
 
@@ -238,7 +263,7 @@ int launch_boil(const BoilArgs& b, const double* disp, double inv_period, hipStr
     k_boil_phase<<<(count + 255) / 256, 256, 0, st>>>(disp, count, inv_period, (uint2*)b.phase);
     PT_LAUNCH_CHECK();
   }
-  const size_t threads = (size_t)(b.n / 2 + 1) * b.n;
+  const size_t threads = (size_t)(b.n / 2 + 1) * b.n * (b.L >= 3 ? 4 : b.L);  // lanes per mode pair
   const int blocks = (int)((threads + 255) / 256);
   switch (b.L) {
 #define PT_BOIL_CASE(L) \
