@@ -333,6 +333,25 @@ int pt_lgs_zoom(const float* screens, int layers, int frames, int ns,
                 const int* idx, const float* w, int taps, int n, float scale,
                 float* out, void* stream);
 
+/*
+ * `count` standard normals times `scale` into out (f32): element i is
+ * component i % 2 of Box-Muller pair (i % 4) / 2 of the Philox4x32-10 call
+ * with counter (i / 4, stream_id) and key `seed`.  The extruder's boiling
+ * noise (the reference draws it from one generator, extrude.py:648-650).
+ */
+int pt_normals(float* out, long long count, unsigned long long seed,
+               unsigned long long stream_id, float scale, void* stream);
+
+/*
+ * dst row (dst_start + r) mod dst_cap = a * itself + b * src row r, for
+ * r < rows and `width` floats (a = 0: dst is not read).  Pitches in floats.
+ * The boiling blend of src/pyturb/extrude.py:740-752 on a ring buffer, and
+ * the seeding of fresh windows.
+ */
+int pt_rows_blend(float* dst, long long dst_pitch, int dst_start, int dst_cap,
+                  const float* src, long long src_pitch, int rows, int width,
+                  float a, float b, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,7 @@ __all__ = [
     "pupil_grids",
     "along_extent_and_span",
     "OracleExtruder",
+    "extrusion_seed_factor",
     "OracleInfinite",
     "structure_function",
     "r0_from_seeing",
@@ -672,6 +673,21 @@ def along_extent_and_span(n, vx, vy, mag, fov):
     return ext, span
 
 
+def extrusion_seed_factor(width, stencil_rows, dx, L0, r0=1.0):
+    """extrude.py:284-303 -- S with S S^T = C_zz (Cholesky, else the
+    symmetric square root), the seed of a fresh stencil block."""
+    m, W = int(stencil_rows), int(width)
+    zy, zx = np.divmod(np.arange(m * W, dtype=np.float64), W)
+    sep = np.hypot(zx[:, None] - zx[None, :], zy[:, None] - zy[None, :])
+    c_zz = phase_covariance(sep * dx, r0, L0)
+    c_zz = 0.5 * (c_zz + c_zz.T)
+    try:
+        return np.linalg.cholesky(c_zz)
+    except np.linalg.LinAlgError:
+        w, v = np.linalg.eigh(c_zz)
+        return v * np.sqrt(np.clip(w, 0.0, None))
+
+
 class OracleExtruder:
     """extrude.py:307-469, 520-659 -- frozen-flow multi-layer extruder, FP64.
 
@@ -682,9 +698,11 @@ class OracleExtruder:
     """
 
     def __init__(self, n, dx, layer_r0, layer_L0, layer_wind, layer_alt,
-                 fov_pix=0.0, stencil_rows=2, lgs_alt=None, A=None, B_unit=None):
+                 fov_pix=0.0, stencil_rows=2, lgs_alt=None, A=None, B_unit=None,
+                 tau_boil=None, boil_seed=None):
         self.n, self.dx, self.m = int(n), float(dx), int(stencil_rows)
         L = len(layer_r0)
+        self.tau_boil = [np.inf] * L if tau_boil is None else [float(t) for t in tau_boil]
         fov = [float(fov_pix)] * L if np.ndim(fov_pix) == 0 else list(fov_pix)
         mags = [1.0 - a / lgs_alt for a in layer_alt] if lgs_alt else [1.0] * L
         ext, spans = zip(*[along_extent_and_span(self.n, vx, vy, mg, fv)
@@ -719,6 +737,12 @@ class OracleExtruder:
             self.along_max.append(float(a.max()))
             self.base[l] = int(math.floor(a.min() - fov[l])) - 2
         self.innov_count = np.zeros(L, np.int64)
+        # boiling (extrude.py:586-654): S = S_unit r0^(-5/6), one stream
+        self.S = None
+        if any(np.isfinite(t) for t in self.tau_boil):
+            s_unit = extrusion_seed_factor(self.W, self.m, self.dx, layer_L0[0])
+            self.S = [s_unit * r ** (-5.0 / 6.0) for r in layer_r0]
+            self.boil_rng = np.random.default_rng(boil_seed)
 
     def seed(self, seed_rows):
         for l in range(self.L):
@@ -767,6 +791,45 @@ class OracleExtruder:
         sa, sp = self.shifts(thx, thy)
         return readout(self.buf, self.fill, np.stack(self.along),
                        np.stack(self.perp), sa, sp, interp)
+
+    def boil_region(self, l, dt):
+        """extrude.py:472-496 -- (lo, hi, a, b) of layer l's readable rows."""
+        if not np.isfinite(self.tau_boil[l]):
+            return None
+        a = math.exp(-dt / self.tau_boil[l])
+        b = math.sqrt(max(0.0, 1.0 - a * a))
+        lo = max(0, int(math.floor(self.along_min[l] + self.travel[l])) - int(self.base[l]) - 3)
+        hi = int(self.fill[l])
+        return None if hi <= lo else (lo, hi, a, b)
+
+    def fresh_window(self, l, height, draw=None):
+        """extrude.py:695-716 -- an independent (height, W) window: S w
+        seeds m rows, A/B extrude the rest.  ``draw(k)`` returns the k-th
+        white-noise vector (k = 0: the seed, m W values; then one row of W
+        per extruded row); by default the boiling stream, in that order."""
+        draw = draw or (lambda k, size: self.boil_rng.standard_normal(size))
+        W, m = self.W, self.m
+        out = np.empty((height, W))
+        seed = (self.S[l] @ np.asarray(draw(0, m * W), float)).reshape(m, W)
+        rows = min(m, height)
+        out[:rows] = seed[:rows]
+        for k in range(m, height):
+            out[k] = self.A @ out[k - m:k].ravel() + self.B[l] @ np.asarray(draw(k - m + 1, W), float)
+        return out
+
+    def boil_step(self, dt, draws=None):
+        """extrude.py:718-752 (NumPy path) -- every boiling layer's readable
+        window blended toward a fresh window, in layer order.  ``draws``:
+        optional per-layer ``draw`` callables (see fresh_window)."""
+        if self.S is None:
+            return
+        for l in range(self.L):
+            region = self.boil_region(l, dt)
+            if region is None:
+                continue
+            lo, hi, a, b = region
+            fresh = self.fresh_window(l, hi - lo, None if draws is None else draws[l])
+            self.buf[l, lo:hi] = a * self.buf[l, lo:hi] + b * fresh
 
 
 class OracleInfinite:

@@ -57,6 +57,9 @@ SIGNATURES = {
     "pt_spectral_frames_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i, _vp, _i, _i,
                                         _vp, _vp, _i, _d, _vp, _vp, _vp]),
     "pt_lgs_zoom": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _f, _vp, _vp]),
+    "pt_normals": (_i, [_vp, ctypes.c_longlong, ctypes.c_ulonglong, ctypes.c_ulonglong, _f, _vp]),
+    "pt_rows_blend": (_i, [_vp, ctypes.c_longlong, _i, _i, _vp, ctypes.c_longlong, _i, _i, _f, _f,
+                           _vp]),
     "pt_boil_work_elems": (ctypes.c_longlong, [_i, _i, _i, _i]),
     "pt_boiled_frames": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i, _i, _d, _vp, _vp, _i, _vp,
                               _vp, ctypes.c_ulonglong, ctypes.c_ulonglong, _i, _i, _f, _vp, _vp,

@@ -18,10 +18,11 @@ Device design:
 * ``sample`` -- one fused ``pt_generate`` call per outer-scale group,
   accumulating into the cropped OPD output.
 
-Spectral boiling (``tau_boil`` on ``engine="spectral"``, pt_boiled_frames)
-and the spectral LGS cone (``lgs_altitude``, per-layer transforms and
-pt_lgs_zoom) run on the device.  Extruder boiling and chromatic dispersion
-models are outside this backend and raise ``NotImplementedError``.
+Spectral boiling (``tau_boil`` on ``engine="spectral"``, pt_boiled_frames),
+extruder boiling (fresh windows on the matrix cores, pt_rows_blend) and the
+spectral LGS cone (``lgs_altitude``, per-layer transforms and pt_lgs_zoom)
+run on the device.  Chromatic dispersion models are outside this backend
+and raise ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -91,7 +92,8 @@ class PeriodicWrapWarning(UserWarning):
 
 
 class ExtrudeBoilingPerformanceWarning(UserWarning):
-    """Boiling on the extrude engine (not provided by this backend)."""
+    """``tau_boil`` on ``engine="extrude"`` is much slower than on "spectral":
+    every step re-extrudes a fresh window row by row (atmosphere.py:69-86)."""
 
 
 class _LayerState:
@@ -126,11 +128,12 @@ class Atmosphere:
             layers, r0, seeing, wavelength, zenith_angle, diameter, n, L0, power_law,
             inner_scale, subharmonics, field_of_view, tau_boil, engine, interp,
             lgs_altitude, dispersion, wet_fraction, device, dtype, seed)
-        if (config.engine == "extrude" and config.has_tau_boil
-                and np.any(np.isfinite(config.tau_boil))):
-            raise NotImplementedError(
-                "tau_boil on engine='extrude' (extruder boiling) is not provided by the MI355X "
-                "backend; use engine='spectral', which boils per Fourier mode")
+        if config.engine == "extrude" and config.has_tau_boil:
+            warnings.warn(
+                "boiling (tau_boil) on engine='extrude' re-extrudes an independent fresh screen "
+                "every step and is markedly slower than the spectral engine's per-mode AR(1) "
+                "boiling. Prefer engine='spectral' for boiling-heavy runs unless genuine "
+                "non-periodicity is required.", ExtrudeBoilingPerformanceWarning, stacklevel=2)
         self.config = config
         self.engine = config.engine
         self.interp = config.interp
@@ -184,7 +187,7 @@ class Atmosphere:
         children = master.spawn(len(self.layers))
         # spectral boiling stream, then the extruder's (atmosphere.py:395-397)
         self._boil_seed = int(master.spawn(1)[0].generate_state(1)[0])
-        master.spawn(1)
+        self._ext_boil_seed = int(master.spawn(1)[0].generate_state(1)[0])
         self._boiling = self.engine == "spectral" and bool(np.any(np.isfinite(self.tau_boil)))
         # injected boiling draws (tests): step index -> (main (2, L, ns, ns),
         # subharmonic (2, P, L, 3, 3)) host arrays, replacing the device stream
@@ -245,7 +248,9 @@ class Atmosphere:
                 interp=self.interp,
                 lgs_altitude_los=(None if self.lgs_altitude is None
                                   else self.lgs_altitude * self.airmass),
-                device=self.device, dtype=self.dtype, seeds=ext_seeds)
+                device=self.device, dtype=self.dtype, seeds=ext_seeds,
+                tau_boil=(list(self.tau_boil) if config.has_tau_boil else None),
+                boil_seed=self._ext_boil_seed)
             self._ext = ExtrudedAtmosphere(**self._ext_kwargs)
 
     # ------------------------------------------------------------------
@@ -456,18 +461,23 @@ class Atmosphere:
         if self._boiling:
             yield from self._boiled_frames(dt, int(steps), scale)
             return
+        if self.engine == "extrude":
+            # frame, then advance the clock and boil once (atmosphere.py:1146-1154)
+            for _ in range(int(steps)):
+                t0 = self._t
+                yield t0, self._cast(self._phase_scaled(t0, 0.0, 0.0, scale))
+                self._t += dt
+                self._ext.boil_step(dt)  # no-op unless tau_boil is finite
+            return
         left = int(steps)
         while left > 0:
-            k = min(left, _FRAME_CHUNK) if self.engine == "spectral" else 1
+            k = min(left, _FRAME_CHUNK)
             times = []
             t = self._t
             for _ in range(k):
                 times.append(t)
                 t += dt
-            if self.engine == "spectral":
-                chunk = self._cast(self._spectral(times, np.zeros(1), np.zeros(1), scale))
-            else:
-                chunk = [self._cast(self._phase_scaled(times[0], 0.0, 0.0, scale))]
+            chunk = self._cast(self._spectral(times, np.zeros(1), np.zeros(1), scale))
             for i in range(k):
                 if self._t != times[i]:  # clock moved under us (reset): recompute
                     break
@@ -482,6 +492,8 @@ class Atmosphere:
             raise ValueError("dt must be positive")
         self._t += float(dt)
         self._boil_step(float(dt))  # no-op unless tau_boil is finite
+        if self.engine == "extrude":
+            self._ext.boil_step(float(dt))
         return self._cast(self._phase_scaled(self._t, 0.0, 0.0, self._scale(wavelength)))
 
     # ------------------------------------------------------------------ boiling

@@ -255,6 +255,24 @@ __global__ void k_boil_coeffs(BoilArgs b) {
 }
 
 // ---------------------------------------------------------------------------
+// plain normals (extruder boiling): element i is word pair (i % 4) / 2,
+// component i % 2, of Philox call (i / 4, stream)
+// ---------------------------------------------------------------------------
+__global__ void k_normals(float* __restrict__ out, long long count, uint32_t key0, uint32_t key1,
+                          unsigned long long stream, float scale) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (4 * c >= count) return;
+  const Ctr4 r = philox4x32_10(Ctr4{(uint32_t)c, (uint32_t)((unsigned long long)c >> 32), (uint32_t)stream,
+                                    (uint32_t)(stream >> 32)},
+                               key0, key1);
+  const float2 p = box_muller(r.x, r.y), q = box_muller(r.z, r.w);
+  const float v[4] = {p.x, p.y, q.x, q.y};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (4 * c + k < count) out[4 * c + k] = scale * v[k];
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int launch_boil(const BoilArgs& b, const double* disp, double inv_period, hipStream_t st) {
@@ -287,3 +305,16 @@ int launch_boil(const BoilArgs& b, const double* disp, double inv_period, hipStr
 }
 
 }  // namespace pt
+
+using namespace pt;
+
+extern "C" int pt_normals(float* out, long long count, unsigned long long seed, unsigned long long stream_id,
+                          float scale, void* stream) {
+  if (!out || count < 0) return fail(PT_ERR_ARG, "pt_normals: bad arguments");
+  if (count == 0) return PT_OK;
+  const long long calls = (count + 3) / 4;
+  k_normals<<<(unsigned)((calls + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      out, count, (uint32_t)seed, (uint32_t)(seed >> 32), stream_id, scale);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}

@@ -744,3 +744,33 @@ extern "C" int pt_readout_adjoint(const float* buf, int layers, long long cap, i
   }
   return PT_OK;
 }
+
+// Row blend into a ring: dst row (dst_start + r) mod dst_cap =
+// a * itself + b * src row r, for r < rows, over `width` floats (a = 0
+// writes b * src without reading dst).  The extruder's boiling blend
+// (src/pyturb/extrude.py:740-752) and the seeding of fresh windows.
+__global__ void k_rows_blend(float* __restrict__ dst, long long dst_pitch, int dst_start, int dst_cap,
+                             const float* __restrict__ src, long long src_pitch, int width, float a, float b) {
+  const int r = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= width) return;
+  int pr = dst_start + r;
+  pr %= dst_cap;
+  float* d = dst + (size_t)pr * dst_pitch + c;
+  const float s = b * src[(size_t)r * src_pitch + c];
+  *d = a == 0.f ? s : fmaf(a, *d, s);
+}
+
+extern "C" int pt_rows_blend(float* dst, long long dst_pitch, int dst_start, int dst_cap, const float* src,
+                             long long src_pitch, int rows, int width, float a, float b, void* stream) {
+  if (!dst || !src) return fail(PT_ERR_ARG, "pt_rows_blend: NULL array");
+  if (rows < 0 || width < 1 || dst_cap < 1 || dst_start < 0 || dst_start >= dst_cap || rows > dst_cap ||
+      dst_pitch < width || src_pitch < width)
+    return fail(PT_ERR_ARG, "pt_rows_blend: bad sizes");
+  if (rows == 0) return PT_OK;
+  if (rows > 65535) return fail(PT_ERR_ARG, "pt_rows_blend: more than 65535 rows");
+  k_rows_blend<<<dim3((width + 255) / 256, rows), 256, 0, (hipStream_t)stream>>>(dst, dst_pitch, dst_start, dst_cap,
+                                                                                  src, src_pitch, width, a, b);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}

@@ -20,6 +20,7 @@ tensor`` injects them instead.  Boiling is not provided by this backend.
 
 from __future__ import annotations
 
+import ctypes as _ct
 import math
 from typing import Any, List, Optional, Sequence, Tuple, Union
 
@@ -29,7 +30,7 @@ from . import _hip
 from .backend import resolve_device
 from .config import ExtrusionConfig
 from .fourier import PhaseScreen
-from .infinite import device_extrusion, extrude_rounds, readout, row_pitch
+from .infinite import device_extrusion, device_seed_factor, extrude_rounds, readout, row_pitch
 from .ring import Ring
 
 __all__ = ["ExtrudedAtmosphere", "along_extent_and_span"]
@@ -47,7 +48,7 @@ def along_extent_and_span(n, vx, vy, magnification, fov_margin_pix) -> Tuple[flo
 
 class _Layer:
     __slots__ = ("speed", "cos", "sin", "alt_pix", "along_min", "along_max", "fov", "mag",
-                 "ring", "travel", "rng", "bscale", "A", "B")
+                 "ring", "travel", "rng", "bscale", "A", "B", "L0", "tau_boil")
 
 
 class ExtrudedAtmosphere:
@@ -60,7 +61,8 @@ class ExtrudedAtmosphere:
                  stencil_rows: int = 2, interp: str = "cubic",
                  lgs_altitude_los: Optional[float] = None, device: str = "mi355x",
                  dtype: Union[str, np.dtype] = "float32", seeds=None, innovations=None,
-                 seed_screens=None) -> None:
+                 seed_screens=None, tau_boil: Optional[Sequence[float]] = None,
+                 boil_seed: Optional[int] = None) -> None:
         config = ExtrusionConfig.create(n, pixel_scale, layer_r0, layer_L0, layer_wind,
                                         layer_altitude_los, field_of_view_pix, stencil_rows,
                                         interp, lgs_altitude_los, device, dtype, seeds)
@@ -108,6 +110,8 @@ class ExtrudedAtmosphere:
             ly.A, B = device_extrusion(W, self.m, self.dx, L0, 1.0, self._dev)
             ly.B = B
             ly.bscale = float(r0) ** (-5.0 / 6.0)
+            ly.L0 = float(L0)
+            ly.tau_boil = float("inf") if tau_boil is None else float(tau_boil[i])
             ly.rng = np.random.default_rng(config.seeds[i])
             if seed_screens is not None:
                 init = seed_screens[i]
@@ -120,6 +124,14 @@ class ExtrudedAtmosphere:
             ly.travel = 0.0
             self.layers.append(ly)
         self._wind_cs = [(ly.cos, ly.sin) for ly in self.layers]
+        # boiling (extrude.py:586-654): one stream for every layer; the device
+        # draws it from Philox keyed by boil_seed, counted by (step, layer, kind)
+        self.boiling = any(np.isfinite(ly.tau_boil) for ly in self.layers)
+        self._boil_seed = 0 if boil_seed is None else int(boil_seed)
+        self._boil_index = 0
+        # injected boiling draws (tests): (step, layer, window rows h) ->
+        # (seed (m W,), rows (h - m, W)) host arrays
+        self._boil_draws = None
 
     # ------------------------------------------------------------------
     def _betas(self, counts):
@@ -149,6 +161,80 @@ class ExtrudedAtmosphere:
             extrude_rounds([ly.A for ly in self.layers], [ly.B for ly in self.layers],
                            self._buf, self.capacity, self.width, self.m, plans, betas,
                            [ly.bscale for ly in self.layers])
+
+    def boil_step(self, dt: float) -> None:
+        """One AR(1) blend of every boiling layer's readable window toward a
+        fresh independent window (extrude.py:472-496, 660-752).
+
+        A fresh window of h rows is S w on its first m rows (S S^T = C_zz,
+        scaled by r0^(-5/6)) and the layer's own recurrence A z + B beta on
+        the rest: the seed product and the recurrence run on the matrix
+        cores (pt_innovation_gemm, pt_extrude_rows), the blend
+        buf = a buf + sqrt(1 - a^2) fresh in pt_rows_blend.  A no-op unless
+        some layer has a finite tau_boil.
+        """
+        if not self.boiling:
+            return
+        t = _hip.torch()
+        lib = _hip.lib()
+        st = _hip.stream_ptr()
+        W, m, ld = self.width, self.m, int(self._buf.shape[-1])
+        regions = []
+        for l, ly in enumerate(self.layers):
+            if not np.isfinite(ly.tau_boil):
+                continue
+            a = math.exp(-dt / ly.tau_boil)
+            b = math.sqrt(max(0.0, 1.0 - a * a))
+            lo = max(0, int(math.floor(ly.along_min + ly.travel)) - ly.ring.base - 3)
+            hi = ly.ring.fill
+            if hi > lo:
+                regions.append((l, lo, hi, a, b))
+        if not regions:
+            return
+        H = max(hi - lo for _, lo, hi, _, _ in regions)
+        # zero padding past W: the recurrence's 16-byte loads read it
+        fresh = t.zeros((len(regions), H, ld), dtype=t.float32, device=self._dev)
+        As, Bs, plans, betas, scales = [], [], [], [], []
+        mw = m * W
+        for j, (l, lo, hi, a, b) in enumerate(regions):
+            ly, h = self.layers[l], hi - lo
+            S = device_seed_factor(W, m, self.dx, ly.L0, self._dev)
+            ldS = int(S.shape[-1])
+            inj = None if self._boil_draws is None else self._boil_draws(self._boil_index, l, h)
+            if inj is None:
+                w = t.empty((mw,), dtype=t.float32, device=self._dev)
+                _hip.check(lib.pt_normals(_hip.ptr(w), mw, self._boil_seed,
+                                          self._boil_stream(l, 0), ly.bscale, st), "pt_normals")
+                beta = t.empty((max(h - m, 0), W), dtype=t.float32, device=self._dev)
+                if h > m:
+                    _hip.check(lib.pt_normals(_hip.ptr(beta), (h - m) * W, self._boil_seed,
+                                              self._boil_stream(l, 1), 1.0, st), "pt_normals")
+            else:
+                w = _hip.to_device(np.asarray(inj[0], np.float64) * ly.bscale, np.float32, self._dev)
+                beta = _hip.to_device(np.asarray(inj[1], np.float64).reshape(-1, W)[:max(h - m, 0)],
+                                      np.float32, self._dev)
+            seed = t.empty((1, ldS), dtype=t.float32, device=self._dev)
+            ptrs = (_ct.c_void_p * 1)(_hip.ptr(w))
+            _hip.check(lib.pt_innovation_gemm(ptrs, _hip.ptr(S), 1, mw, ldS, _hip.ptr(seed), st),
+                       "pt_innovation_gemm")
+            _hip.check(lib.pt_rows_blend(_hip.ptr(fresh[j]), ld, 0, H, _hip.ptr(seed), W,
+                                         min(m, h), W, 0.0, 1.0, st), "pt_rows_blend")
+            A, B = device_extrusion(W, m, self.dx, ly.L0, 1.0, self._dev)
+            As.append(A)
+            Bs.append(B)
+            plans.append([(k, tuple(range(k - m, k))) for k in range(m, h)])
+            betas.append(beta)
+            scales.append(ly.bscale)
+        extrude_rounds(As, Bs, fresh, H, W, m, plans, betas, scales)
+        for j, (l, lo, hi, a, b) in enumerate(regions):
+            ring = self.layers[l].ring
+            _hip.check(lib.pt_rows_blend(_hip.ptr(self._buf[l]), ld, ring.phys(lo), self.capacity,
+                                         _hip.ptr(fresh[j]), ld, hi - lo, W, float(a), float(b), st),
+                       "pt_rows_blend")
+        self._boil_index += 1
+
+    def _boil_stream(self, layer: int, kind: int) -> int:
+        return (self._boil_index * 64 + layer) * 2 + kind
 
     def dir_geometry(self, thx, thy, mags=None) -> np.ndarray:
         """(D, L, 3) readout geometry: along/perp shifts and magnification."""

@@ -110,6 +110,33 @@ def device_extrusion(width, stencil_rows, pixel_scale, L0, r0, dev):
     return hit
 
 
+_SEED_CACHE: Dict[tuple, Any] = {}
+
+
+def device_seed_factor(width, stencil_rows, pixel_scale, L0, dev):
+    """S_unit (m W, round_up(m W, 64)) float32 on the device, S S^T = C_zz at
+    r0 = 1: Cholesky of the stencil-block covariance, else its symmetric
+    square root (extrude.py:284-303).  Cached per geometry and device."""
+    key = (int(width), int(stencil_rows), float(pixel_scale), round(float(L0), 9), str(dev))
+    hit = _SEED_CACHE.get(key)
+    if hit is None:
+        m, W = int(stencil_rows), int(width)
+        zy, zx = np.divmod(np.arange(m * W, dtype=np.float64), W)
+        sep = np.hypot(zx[:, None] - zx[None, :], zy[:, None] - zy[None, :])
+        c_zz = phase_covariance(sep * pixel_scale, 1.0, L0)
+        c_zz = 0.5 * (c_zz + c_zz.T)
+        try:
+            s_unit = np.linalg.cholesky(c_zz)
+        except np.linalg.LinAlgError:
+            w, v = np.linalg.eigh(c_zz)
+            s_unit = v * np.sqrt(np.clip(w, 0.0, None))
+        sp = np.zeros((m * W, row_pitch(m * W)), dtype=np.float32)
+        sp[:, :m * W] = s_unit
+        hit = _hip.to_device(sp, None, dev)
+        _SEED_CACHE[key] = hit
+    return hit
+
+
 def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales):
     """Run the planned rows of several layers on the device.
 

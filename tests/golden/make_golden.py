@@ -25,8 +25,11 @@ path that closed forms and identities do not fix on their own:
   3-layer n=32 atmosphere for a 20 km beacon, on- and off-axis, with each
   interpolation.
 
-``python tests/golden/make_golden.py /root/reference/src boil`` (or ``lgs``)
-writes only that fixture.
+* extruder boiling (extrude.py:472-496, 660-752, NumPy path): readouts of a
+  2-layer n=32 extruder boiling at two rates, between set_time steps.
+
+``python tests/golden/make_golden.py /root/reference/src boil`` (or ``lgs``,
+``extboil``) writes only that fixture.
 
 Only data is written here; nothing from the reference is copied into the
 repository, and nothing at test time reads the reference.
@@ -87,13 +90,31 @@ def write_lgs(pyturb) -> None:
     np.savez_compressed(os.path.join(HERE, "lgs_n32_seed7.npz"), **out)
 
 
+def write_extboil(pyturb) -> None:
+    from pyturb.extrude import ExtrudedAtmosphere
+
+    ext = ExtrudedAtmosphere(
+        n=32, pixel_scale=0.25, layer_r0=[0.2, 0.35], layer_L0=[25.0, 25.0],
+        layer_wind=[(6.0, 2.5), (-3.0, 11.0)], layer_altitude_los=[0.0, 8000.0],
+        field_of_view_pix=[0.0, 3.0], interp="cubic", dtype="float64", seeds=[11, 12],
+        tau_boil=[0.03, 0.08], boil_seed=77)
+    frames = []
+    for k in range(5):
+        ext.set_time(k * 0.01)
+        frames.append(ext.integrate(0.0, 0.0))
+        frames.append(ext.integrate(2e-5, -1e-5))
+        ext.boil_step(0.01)
+    np.savez_compressed(os.path.join(HERE, "extrude_boil_n32.npz"), frames=np.stack(frames))
+
+
 def main(ref_src: str, only: str = "") -> None:
     sys.path.insert(0, ref_src)
     import pyturb  # noqa: E402  (reference, this container only)
     from pyturb.extrude import ExtrudedAtmosphere, build_extrusion
 
-    if only in ("boil", "lgs"):
-        (write_boil if only == "boil" else write_lgs)(pyturb)
+    writers = {"boil": write_boil, "lgs": write_lgs, "extboil": write_extboil}
+    if only in writers:
+        writers[only](pyturb)
         print(only, "fixture written to", HERE)
         return
 
@@ -174,9 +195,10 @@ def main(ref_src: str, only: str = "") -> None:
         inf[interp] = np.stack(seq)
     np.savez_compressed(os.path.join(HERE, "infinite_n32_seed5.npz"), **inf)
 
-    # 7. spectral boiling; 8. spectral LGS cone.
+    # 7. spectral boiling; 8. spectral LGS cone; 9. extruder boiling.
     write_boil(pyturb)
     write_lgs(pyturb)
+    write_extboil(pyturb)
     print("golden fixtures written to", HERE)
 
 

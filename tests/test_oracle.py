@@ -470,3 +470,48 @@ def test_lgs_zoom_taps_identities():
         rows = np.einsum("ti,tic->ic", w[0], screen[idx[0]])
         out = np.einsum("tj,itj->ij", w[0], rows[:, idx[0]])
         assert np.allclose(out, screen, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# extruder boiling (extrude.py:472-496, 660-752)
+# ---------------------------------------------------------------------------
+def test_extruder_boiling_matches_reference_golden():
+    g = load_golden("extrude_boil_n32.npz")["frames"]
+    kw = dict(n=32, dx=0.25, layer_r0=[0.2, 0.35], layer_L0=[25.0, 25.0],
+              layer_wind=[(6.0, 2.5), (-3.0, 11.0)], layer_alt=[0.0, 8000.0],
+              fov_pix=[0.0, 3.0], tau_boil=[0.03, 0.08], boil_seed=77)
+    ext = O.OracleExtruder(**kw)
+    rngs = [np.random.default_rng(s) for s in (11, 12)]
+    seeds = []
+    for l, rng in enumerate(rngs):
+        filt = O.build_filters(ext.W, 0.25, kw["layer_r0"][l], 25.0, subharmonics=8)
+        main, sh = O.draw_generate_noise(rng, ext.W, 1, 8)
+        seeds.append(O.generate(filt, main, sh, 1)[0])
+    ext.seed(seeds)
+    innov = lambda l, k: rngs[l].standard_normal(ext.W)
+    got = []
+    for k in range(5):
+        ext.set_time(k * 0.01, innov)
+        got.append(ext.integrate(0.0, 0.0))
+        got.append(ext.integrate(2e-5, -1e-5))
+        ext.boil_step(0.01)
+    assert O.rel_rms(np.stack(got), g) < 1e-9
+
+
+def test_extruder_boiling_preserves_covariance():
+    """A fresh window has the stencil covariance: S S^T = C_zz (seed) and
+    every blend keeps a^2 + b^2 = 1."""
+    S = O.extrusion_seed_factor(24, 2, 0.25, 25.0)
+    zy, zx = np.divmod(np.arange(48.0), 24)
+    czz = O.phase_covariance(np.hypot(zx[:, None] - zx, zy[:, None] - zy) * 0.25, 1.0, 25.0)
+    assert np.allclose(S @ S.T, czz, atol=1e-9 * czz.max())
+    ext = O.OracleExtruder(n=16, dx=0.25, layer_r0=[0.2], layer_L0=[25.0],
+                           layer_wind=[(5.0, 0.0)], layer_alt=[0.0], tau_boil=[0.05])
+    rng = np.random.default_rng(2)
+    ext.seed([rng.standard_normal((ext.spans[0], ext.W))])
+    ext.set_time(0.3, lambda l, k: rng.standard_normal(ext.W))
+    lo, hi, a, b = ext.boil_region(0, 0.01)
+    assert np.isclose(a * a + b * b, 1.0) and np.isclose(a, np.exp(-0.01 / 0.05))
+    # the window starts 3 rows below the lowest row the pupil reads
+    assert lo == int(np.floor(ext.along_min[0] + ext.travel[0])) - ext.base[0] - 3
+    assert hi == ext.fill[0]
