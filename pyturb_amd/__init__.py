@@ -15,6 +15,7 @@ from .backend import resolve_device
 from .flow import FourierFlowScreen
 from .fourier import PhaseScreen
 from .infinite import InfinitePhaseScreen, phase_covariance
+from .io import load, save
 from .profiles import (
     Layer,
     coherence_time,
@@ -49,6 +50,7 @@ __all__ = [
     "greenwood_frequency",
     "isoplanatic_angle",
     "list_profiles",
+    "load",
     "mean_turbulence_height",
     "opd_to_phase",
     "phase_covariance",
@@ -56,6 +58,7 @@ __all__ = [
     "r0_at_wavelength",
     "r0_from_seeing",
     "resolve_device",
+    "save",
     "seeing_from_r0",
     "structure_function",
     "to_numpy",
