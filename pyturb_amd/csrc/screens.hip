@@ -274,10 +274,13 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_gen_cols_acc(GenArgs a) {
 // fold[l] holds A_l + B_l (A_l on the Nyquist lines); side[l][j] holds B_l
 // on the Nyquist row (j = k1) and column (j = N + k0).
 //
-// Within each row the folded stack is stored in pass A's first-stage order:
-// column c = j + r * n/8 (j < n/8, r < 8) sits at position 8 j + r, so the
-// eight values a pass-A thread starts from are 64 contiguous bytes (four
-// 16-byte loads instead of eight 8-byte ones).
+// Within each row the folded stack is stored chunk-major for pass A: the
+// thread for j < n/8 starts from columns j + r * n/8 (r < 8), and column
+// c = j + r * n/8 sits at position (r / 2) * (n/4) + 2 j + r % 2.  Chunk
+// h = r / 2 of every thread is then one contiguous 16-byte-per-lane run, so
+// each of a thread's four 16-byte loads is a fully coalesced wave load
+// (with the eight values contiguous per thread, lanes 64 bytes apart, every
+// load touched 32 cache lines and the texture-data path saturated).
 __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ spectra, int n,
                                                     float2* __restrict__ fold, float2* __restrict__ side) {
   const size_t plane = (size_t)n * n;
@@ -290,7 +293,8 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
   const float2 sm = spectra[(size_t)l * plane + (size_t)m0 * n + m1];
   const float2 A = make_float2(0.5f * sv.x, 0.5f * sv.y), B = make_float2(0.5f * sm.x, -0.5f * sm.y);
   const int h = n / 2, nb = n / 8;
-  const size_t dst = (size_t)l * plane + (size_t)k0 * n + (size_t)(k1 % nb) * 8 + k1 / nb;
+  const int j = k1 % nb, r = k1 / nb;
+  const size_t dst = (size_t)l * plane + (size_t)k0 * n + (size_t)(r >> 1) * 2 * nb + 2 * j + (r & 1);
   if (k0 == h || k1 == h) {
     fold[dst] = A;
     side[(size_t)l * 2 * n + (k0 == h ? k1 : n + k0)] = B;
@@ -305,8 +309,10 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
 // exp(2 pi i (8a + b - N/2) u) = t8[a] * t1[b], (k1 + N/2) mod N = 8a + b,
 // so the per-layer LDS tables are N/8 + 8 entries instead of N.
 template <int N>
+struct PairCfg;
+template <int N>
 __host__ __device__ constexpr size_t pair_rows_lds(int L) {
-  return (size_t)2 * L * (N / 8 + 8 + RowsCfg<N>::M) * 8 + (size_t)2 * L * 4;
+  return (size_t)2 * L * (N / 8 + 8 + PairCfg<N>::M) * 8 + (size_t)2 * L * 4;
 }
 
 // Per frame pair, the row-independent phasor tables of pass A, computed once
@@ -374,12 +380,21 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
   }
 }
 
+// Pass-A workgroups: 256 threads (4 rows at n = 512), four per CU by LDS.
+// 512-thread, 8-row workgroups (24 waves per CU instead of 16) measured the
+// same: occupancy does not bound this kernel.
 template <int N>
-__global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restrict__ fold,
-                                                         const float2* __restrict__ corr, int L,
-                                                         const float2* __restrict__ ptab, int F,
-                                                         float2* __restrict__ work) {
-  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M, NB = N / 8;
+struct PairCfg {
+  static constexpr int T = (N * N / 8 < 256) ? N * N / 8 : 256;
+  static constexpr int M = 8 * T / N;
+};
+
+template <int N>
+__global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2* __restrict__ fold,
+                                                                   const float2* __restrict__ corr, int L,
+                                                                   const float2* __restrict__ ptab, int F,
+                                                                   float2* __restrict__ work) {
+  constexpr int T = PairCfg<N>::T, M = PairCfg<N>::M, NB = N / 8;
   static_assert(T == M * NB, "one row position per thread in the first stage");
   __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
   __shared__ float2 tw[N];
@@ -412,7 +427,7 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
   const int m = tid / NB, jq = tid % NB;
   const int k0 = row0 + m;
   const size_t plane = (size_t)N * N;
-  const float4* fp = (const float4*)(fold + (size_t)k0 * N + (size_t)jq * 8);  // first-stage order
+  const float4* fp = (const float4*)(fold + (size_t)k0 * N) + jq;  // chunk h at float4 h * NB + jq
   // z[r] accumulates the packed sum Herm X_t1 + i Herm X_t2 directly: per layer
   // and element, Q = P_1 + i P_2 (two phasor products folded into 8 FMAs),
   // then one complex multiply-accumulate z += g Q (4 FMAs).
@@ -430,7 +445,7 @@ __global__ __launch_bounds__(256) void k_frame_pair_rows(const float2* __restric
     float2 g[8];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-      const float4 v = fp[(size_t)l * (plane / 2) + h];
+      const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * NB];
       g[2 * h] = make_float2(v.x, v.y);
       g[2 * h + 1] = make_float2(v.z, v.w);
     }
@@ -562,9 +577,9 @@ static FramesLayout frames_layout(int n, int F, int L, int P) {
 template <int N>
 static int frames_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
                             float2* work, hipStream_t st) {
-  constexpr int MR = RowsCfg<N>::M;
+  constexpr int MR = PairCfg<N>::M;
   const size_t dyn_r = pair_rows_lds<N>(L);
-  if (dyn_r + (size_t)Layout<N, RowsCfg<N>::T, MR, false>::SIZE * 8 + (size_t)N * 8 > kLdsMax)
+  if (dyn_r + (size_t)Layout<N, PairCfg<N>::T, MR, false>::SIZE * 8 + (size_t)N * 8 > kLdsMax)
     return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor tables at n=%d", L, N);
   int rc = set_dyn_lds(k_frame_pair_rows<N>, dyn_r);
   if (rc) return rc;
@@ -577,7 +592,7 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   PT_LAUNCH_CHECK();
   k_frame_pair_tables<N><<<(F + 1) / 2, 256, 0, st>>>(disp, side, L, F, inv_period, ptab, corr);
   PT_LAUNCH_CHECK();
-  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), RowsCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, work);
+  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), PairCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
