@@ -15,7 +15,14 @@ Also reported on the same line:
     everywhere -- the reference's array-module algorithm on ROCm);
   * roofline of the dominant kernel (pass A of the frame engine) from HIP
     events on the launch stream;
-  * the CPU baseline: the FP64 NumPy oracle timed on this host (1 thread).
+  * the CPU baseline: the FP64 NumPy oracle timed on this host (1 thread);
+  * a closed-loop call pattern, one frame per call, eager against a
+    HIP-graph replay (pyturb_amd.graph.CapturedFrames), and the ensemble
+    replayed from a HIP graph;
+  * the launch shapes the autotuner picked (pyturb_amd.tune) for the
+    row and column passes at each size;
+  * 16 atmospheres called in turn, 125 frames each, so that their 302 MB of
+    layer stacks stream from HBM instead of staying in the MALL.
 
 ``python bench.py [--gpus 1] [--steps K] [--warmup W]`` prints one JSON line.
 """
@@ -70,7 +77,7 @@ def build_frames_workload(dev):
     return atm, times, disp
 
 
-def run_frames(atm, disp, out, work, events=None):
+def run_frames(atm, disp, out, work, events=None, frames=N_FRAMES):
     from pyturb_amd import _hip
 
     lib = _hip.lib()
@@ -81,11 +88,11 @@ def run_frames(atm, disp, out, work, events=None):
     if events:
         events[0].record()
     rc = lib.pt_spectral_rows(_hip.ptr(atm._spectra), L, N, atm.pixel_scale, _hip.ptr(disp),
-                              N_FRAMES, _hip.ptr(work), st)
+                              frames, _hip.ptr(work), st)
     _hip.check(rc, "pt_spectral_rows")
     if events:
         events[1].record()
-    rc = lib.pt_spectral_cols(_hip.ptr(work), L, N, atm.pixel_scale, _hip.ptr(disp), N_FRAMES,
+    rc = lib.pt_spectral_cols(_hip.ptr(work), L, N, atm.pixel_scale, _hip.ptr(disp), frames,
                               _hip.ptr(atm._sh_coeffs), P, 0, N, float(scale), _hip.ptr(out), st)
     _hip.check(rc, "pt_spectral_cols")
     if events:
@@ -373,6 +380,12 @@ def main(argv=None):
     ap.add_argument("--no-extrude", action="store_true", help="skip the extrusion workload")
     ap.add_argument("--no-boil", action="store_true", help="skip the boiling-frames workload")
     ap.add_argument("--no-lgs", action="store_true", help="skip the LGS-cone frames workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in FFT launch shapes (skip the autotuner)")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the 16-atmosphere HBM-streamed frames variant")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the one-frame-per-call loop (eager vs HIP-graph replay)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (counter scaling)")
     args = ap.parse_args(argv)
@@ -397,6 +410,15 @@ def main(argv=None):
     from pyturb_amd import _hip
 
     dev = _hip.device(0)
+    tuned = None
+    if not args.no_tune:
+        from pyturb_amd import tune
+
+        print("bench: autotuning the FFT launch shapes", file=sys.stderr, flush=True)
+        sizes = sorted({N} | {n for n, _ in MC_SIZES})
+        tune.autotune(sizes)
+        tuned = {str(n): {k: tune.describe(k, n) for k in tune.KNOBS} for n in sizes}
+        torch.cuda.empty_cache()
     print("bench: building 9-layer 512^2 spectral workload", file=sys.stderr, flush=True)
     atm, times, disp = build_frames_workload(dev)
     out = torch.empty((N_FRAMES, N, N), dtype=torch.float32, device=dev)
@@ -431,6 +453,17 @@ def main(argv=None):
     par = parity(atm, out)
 
     extra = {}
+    if not args.no_graph:
+        print("bench: one frame per call, eager vs HIP-graph replay", file=sys.stderr, flush=True)
+        from pyturb_amd.graph import CapturedFrames
+
+        cap = CapturedFrames(atm, 1)
+        ncall = 500
+        tg = timed(lambda: [cap([float(t)]) for t in times[:ncall]], 1, 1)
+        te = timed(lambda: [atm.opd(float(t)) for t in times[:ncall]], 1, 1)
+        extra["loop_eager_frames_per_s"] = round(ncall / te, 1)
+        extra["loop_graph_frames_per_s"] = round(ncall / tg, 1)
+        del cap
     if not args.no_plain:
         ctx = plain_context(atm)
         nplain = 200
@@ -449,11 +482,45 @@ def main(argv=None):
         extra["mc_ms_per_ensemble"] = round(1e3 * tm / max(1, args.steps // 2), 3)
         extra["mc_algorithmic_GBs"] = round(mc_bytes * mc_rate / nscr / 1e9, 1)
         extra["mc_frac_hbm_peak"] = round(mc_bytes * mc_rate / nscr / 1e9 / HBM_PEAK_GBS, 4)
+        if not args.no_graph:
+            from pyturb_amd.graph import Captured
+
+            cap = Captured(lambda: run_mc(classes))
+            tg = timed(cap.replay, max(1, args.steps // 2), 1)
+            extra["mc_graph_screens_per_s"] = round(nscr * max(1, args.steps // 2) / tg, 1)
+            del cap
         if not args.no_plain:
             tpm = timed(lambda: run_mc_plain(classes), 1, 1)
             extra["mc_plain_screens_per_s"] = round(nscr / tpm, 1)
             extra["fused_vs_plain_mc"] = round(mc_rate / (nscr / tpm), 1)
         del classes
+        torch.cuda.empty_cache()
+    if not args.no_stream:
+        # The 18.9 MB layer stack of one atmosphere stays in the 256 MB MALL
+        # across steps.  Sixteen atmospheres (302 MB of stacks) called in
+        # turn, 125 frames each, make every call's fold read its stack from
+        # HBM; pass A then reads the folded copy that the call just wrote.
+        print("bench: 16 atmospheres in turn (HBM-streamed stacks)", file=sys.stderr, flush=True)
+        import pyturb_amd as pa
+
+        K = 16
+        per = N_FRAMES // K
+        ws = []
+        for k in range(K):
+            a_k = pa.Atmosphere(profile_layers(), r0=0.15, diameter=8.0, n=N, L0=25.0,
+                                subharmonics=8, seed=100 + k, device="mi355x")
+            ws.append(a_k)
+        sdisp = disp[:per].contiguous()
+        sout = out[:per]
+
+        def stream_once():
+            for a_k in ws:
+                run_frames(a_k, sdisp, sout, work, frames=per)
+
+        ts = timed(stream_once, max(1, args.steps // 2), 1)
+        extra["stream16_frames_per_s"] = round(K * per * max(1, args.steps // 2) / ts, 1)
+        extra["stream16_stack_MB"] = round(K * 8.0 * L * N * N / 1e6, 1)
+        del ws
         torch.cuda.empty_cache()
     if not args.no_boil:
         print("bench: boiling frames", file=sys.stderr, flush=True)
@@ -539,6 +606,7 @@ def main(argv=None):
                      "frame_algorithmic_GBs": round(frame_bytes * fps / 1e9, 1),
                      "pass_b_GBs": round(bytes_b / t_b / 1e9, 1)},
         "cpu_baseline": cpu,
+        "autotuned_shapes": tuned,
         "parity_rel_rms_vs_fp64": float(f"{par:.3g}"),
         "reference_published_rtx5090_fps_512": 3133,
     }

@@ -24,12 +24,17 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 2
+#define PT_ABI_VERSION 3
 
 #define PT_OK 0
 #define PT_ERR_ARG 1         /* invalid argument (shape, count, NULL)      */
 #define PT_ERR_HIP 2         /* HIP runtime error (launch, copy)           */
 #define PT_ERR_UNSUPPORTED 3 /* size outside what the kernels implement    */
+
+/* Launch-shape knobs of the autotuner (pt_tune_*). */
+#define PT_TUNE_GEN_ROWS 0 /* row pass of pt_generate*: threads per workgroup     */
+#define PT_TUNE_GEN_COLS 1 /* column pass of pt_generate* and frame pass B:       */
+                           /* threads per workgroup x workgroup order             */
 
 /* Last error message of the calling thread ("" if none). */
 const char* pt_last_error(void);
@@ -351,6 +356,29 @@ int pt_normals(float* out, long long count, unsigned long long seed,
 int pt_rows_blend(float* dst, long long dst_pitch, int dst_start, int dst_cap,
                   const float* src, long long src_pitch, int rows, int width,
                   float a, float b, void* stream);
+
+/*
+ * Autotuner state: the launch shape of the generate row pass
+ * (PT_TUNE_GEN_ROWS) and of the column pass shared by pt_generate* and the
+ * frame engine's pass B (PT_TUNE_GEN_COLS), selected per power-of-two grid
+ * size n in 32..2048.  No counterpart in the reference, whose transforms are
+ * library ifft2 calls (fourier.py:259, atmosphere.py:918).  Variant 0 is
+ * the built-in default; every variant gives bit-identical results.  The
+ * selection is process-wide host state read when a call launches (not
+ * synchronised with calls from other threads).
+ *
+ * pt_tune_variants: number of variants of a knob (negative error code on a
+ * bad knob).  pt_tune_info: effective threads per workgroup, rows or
+ * columns per workgroup (tile) and XCD-contiguous order (0/1) of a variant
+ * at size n; any output pointer may be NULL.  pt_tune_set / pt_tune_get:
+ * select / read the variant for size n (pt_tune_get returns it, or a
+ * negative error code).
+ */
+int pt_tune_variants(int knob);
+int pt_tune_info(int knob, int n, int variant, int* threads, int* tile,
+                 int* xcd_order);
+int pt_tune_set(int knob, int n, int variant);
+int pt_tune_get(int knob, int n);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,9 @@ PT_OK = 0
 PT_ERR_ARG = 1
 PT_ERR_HIP = 2
 PT_ERR_UNSUPPORTED = 3
-ABI_VERSION = 2  # PT_ABI_VERSION of include/pyturb_hip.h
+PT_TUNE_GEN_ROWS = 0
+PT_TUNE_GEN_COLS = 1
+ABI_VERSION = 3  # PT_ABI_VERSION of include/pyturb_hip.h
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -64,6 +66,10 @@ SIGNATURES = {
     "pt_boiled_frames": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i, _i, _d, _vp, _vp, _i, _vp,
                               _vp, ctypes.c_ulonglong, ctypes.c_ulonglong, _i, _i, _f, _vp, _vp,
                               _vp]),
+    "pt_tune_variants": (_i, [_i]),
+    "pt_tune_info": (_i, [_i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "pt_tune_set": (_i, [_i, _i, _i]),
+    "pt_tune_get": (_i, [_i, _i]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -95,6 +101,11 @@ def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         _lib = load()
+        cache = os.environ.get("PYTURB_AMD_TUNE_CACHE")
+        if cache and os.path.exists(cache):  # launch shapes chosen by pyturb_amd.tune
+            from . import tune
+
+            tune.load_cache(cache)
     return _lib
 
 

@@ -1,0 +1,109 @@
+"""HIP-graph replay of fixed-shape hot-path calls.
+
+The library's entry points launch kernels only: no memset, copy or
+allocation happens inside a call, so a sequence of calls over static
+buffers records into a HIP graph (``torch.cuda.CUDAGraph``) and replays with
+one launch from the host.
+
+* ``Captured(fn)`` records any such sequence, e.g. a Monte-Carlo ensemble of
+  mixed screen sizes (one ``ops.generate_groups`` call per size class over
+  noise that the caller refreshes in place between replays).
+* ``CapturedFrames(atm, frames, directions)`` is ``atm.opd_batch`` for a
+  fixed number of times.  A closed-loop simulation asks for a few frames at
+  a time, over and over, and each eager call is then dominated by launch and
+  host overhead: five launches through ctypes plus a displacement upload.
+  Each call refreshes only the displacement table, with one host-to-device
+  copy, and replays the graph.
+"""
+
+from __future__ import annotations
+
+from typing import Any, Optional, Sequence
+
+import numpy as np
+
+from . import _hip, ops
+
+
+class Captured:
+    """A HIP graph of the launches ``fn()`` makes on the current stream.
+
+    ``fn`` runs once eagerly on a side stream (so that lazily built state
+    such as LDS limits is set up outside the capture), then once under
+    capture.  Every buffer it touches must stay allocated and at the same
+    address while the graph is replayed.
+    """
+
+    def __init__(self, fn, device=None) -> None:
+        t = _hip.torch()
+        dev = device if device is not None else t.cuda.current_device()
+        side = t.cuda.Stream(device=dev)
+        side.wait_stream(t.cuda.current_stream(dev))
+        with t.cuda.stream(side):
+            fn()
+        t.cuda.current_stream(dev).wait_stream(side)
+        self.graph = t.cuda.CUDAGraph()
+        with t.cuda.graph(self.graph):
+            fn()
+
+    def replay(self) -> None:
+        """Launch the recorded work on the current stream."""
+        self.graph.replay()
+
+
+class CapturedFrames:
+    """``atm.opd_batch(times, directions)`` for a fixed number of times and
+    directions, replayed from a HIP graph.  The returned tensor is the graph's
+    static output: copy it before the next call if it must be kept."""
+
+    def __init__(self, atm, frames: int, directions: Optional[Sequence] = None,
+                 wavelength: Optional[float] = None) -> None:
+        t = _hip.torch()
+        if atm.engine != "spectral" or atm._lgs_mag is not None or atm._boiling:
+            raise ValueError("CapturedFrames needs a frozen-flow spectral atmosphere "
+                             "(no LGS cone, no boiling)")
+        self.atm = atm
+        self.frames = int(frames)
+        if directions is None:
+            self.ox, self.oy = np.zeros(1), np.zeros(1)
+        else:
+            self.ox, self.oy = atm._dirs_tan(directions)
+        self.D = len(self.ox)
+        L = len(atm._layers)
+        self.scale = atm._scale(wavelength)
+        F = self.frames * self.D
+        dev = atm._dev
+        self._disp_host = t.empty((F, L, 2), dtype=t.float64, pin_memory=True)
+        self.disp = t.zeros((F, L, 2), dtype=t.float64, device=dev)
+        self.out = t.empty((F, atm.n, atm.n), dtype=t.float32, device=dev)
+        P = 0 if atm._sh_coeffs is None else int(atm._sh_coeffs.shape[0])
+        self.work = t.empty((ops.frames_work_elems(atm.n_screen, F, L, P),), dtype=t.complex64,
+                            device=dev)
+        self._copied = t.cuda.Event()  # the pinned table is free once this completes
+        self._cap = Captured(self._launch, dev)
+
+    def _launch(self) -> None:
+        a = self.atm
+        ops.spectral_frames(a._spectra, self.disp, a._sh_coeffs, a.pixel_scale,
+                            crop_offset=a.margin_pix, n_out=a.n, scale=self.scale,
+                            out=self.out, work=self.work)
+
+    def __call__(self, times: Sequence[float]) -> Any:
+        a = self.atm
+        times = np.asarray(times, dtype=np.float64).ravel()
+        if times.size != self.frames:
+            raise ValueError(f"expected {self.frames} times, got {times.size}")
+        dx = a._vx[None, None, :] * times[:, None, None] + a._alt * self.ox[None, :, None]
+        dy = a._vy[None, None, :] * times[:, None, None] + a._alt * self.oy[None, :, None]
+        if not a._wrap_warned:
+            a._check_wrap(dx, dy)
+        disp = np.stack([dx, dy], axis=-1).reshape(-1, len(a._layers), 2)
+        self._copied.synchronize()
+        self._disp_host.numpy()[...] = disp
+        self.disp.copy_(self._disp_host, non_blocking=True)
+        self._copied.record()
+        self._cap.replay()
+        out = a._cast(self.out)
+        if self.D > 1:
+            out = out.view(self.frames, self.D, a.n, a.n)
+        return out

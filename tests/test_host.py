@@ -60,7 +60,7 @@ def test_library_loads_and_exports_every_header_symbol():
     for name in syms:
         assert hasattr(so, name), name
         assert name in _hip.SIGNATURES, name
-    assert _hip.lib().pt_abi_version() == 2
+    assert _hip.lib().pt_abi_version() == _hip.ABI_VERSION == 3
     assert "pt_work_elems" in syms
     # host-only sizing: power-of-two fused path vs general path
     # frames: pairs of frames share one transform, plus the folded stack and its
@@ -143,3 +143,38 @@ def test_structure_function_host_estimator():
     r2, d2 = O.structure_function(x, 0.5, max_separation=4)
     np.testing.assert_allclose(r, r2)
     np.testing.assert_allclose(d, d2)
+
+
+def test_tune_state_is_host_side_and_validated():
+    """pt_tune_* select launch shapes per size; arguments are checked on the
+    host and the effective shape clamps to [n/8, n^2/8] threads."""
+    from pyturb_amd import _hip, tune
+
+    lib = _hip.lib()
+    assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_ROWS) == 3
+    assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_COLS) == 6
+    assert lib.pt_tune_variants(7) == -_hip.PT_ERR_ARG
+    assert lib.pt_tune_set(7, 512, 0) == _hip.PT_ERR_ARG
+    assert lib.pt_tune_set(0, 500, 0) == _hip.PT_ERR_ARG      # not a power of two
+    assert lib.pt_tune_set(0, 4096, 0) == _hip.PT_ERR_ARG     # above 2048
+    assert lib.pt_tune_set(0, 512, 3) == _hip.PT_ERR_ARG      # rows has variants 0..2
+    assert lib.pt_tune_get(1, 100) == -_hip.PT_ERR_ARG
+    # shapes: (threads, tile, xcd order)
+    shapes = {(v["threads"], v["tile"], v["xcd"]) for v in tune.variants("gen_cols", 512)}
+    assert shapes == {(1024, 16, 1), (512, 8, 1), (256, 4, 1), (1024, 16, 0), (512, 8, 0),
+                      (256, 4, 0)}
+    assert [(v["threads"], v["tile"]) for v in tune.variants("gen_cols", 2048)
+            if v["xcd"]] == [(1024, 4), (512, 2), (256, 1)]
+    assert [(v["threads"], v["tile"]) for v in tune.variants("gen_rows", 32)] == [(128, 32)]
+    assert [(v["threads"], v["tile"]) for v in tune.variants("gen_rows", 2048)] == [(256, 1),
+                                                                                    (512, 2)]
+    try:
+        tune.set_variant("gen_cols", 256, 4)
+        assert tune.get("gen_cols", 256) == 4 and tune.get("gen_cols", 512) == 0
+    finally:
+        tune.reset()
+    assert tune.get("gen_cols", 256) == 0
+    with pytest.raises(ValueError):
+        tune.set_variant("gen_rows", 256, 9)
+    with pytest.raises(ValueError):
+        tune.variants("fft", 256)
