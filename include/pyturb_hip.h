@@ -35,6 +35,8 @@ extern "C" {
 #define PT_TUNE_GEN_ROWS 0 /* row pass of pt_generate*: threads per workgroup     */
 #define PT_TUNE_GEN_COLS 1 /* column pass of pt_generate* and frame pass B:       */
                            /* threads per workgroup x workgroup order             */
+#define PT_TUNE_FRAMES 2   /* transform order of the fused frame engine:          */
+                           /* 0 column-first, 1 row-first                         */
 
 /* Last error message of the calling thread ("" if none). */
 const char* pt_last_error(void);

@@ -26,6 +26,7 @@ PT_ERR_HIP = 2
 PT_ERR_UNSUPPORTED = 3
 PT_TUNE_GEN_ROWS = 0
 PT_TUNE_GEN_COLS = 1
+PT_TUNE_FRAMES = 2
 ABI_VERSION = 3  # PT_ABI_VERSION of include/pyturb_hip.h
 
 _vp = ctypes.c_void_p

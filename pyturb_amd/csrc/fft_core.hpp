@@ -97,14 +97,20 @@ __device__ __forceinline__ void dftR(float2* v) {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
-template <int N>
+// Stage plan: radix-8 stages plus one radix-4 or radix-2 stage when log2 N is
+// not a multiple of 3.  The small stage is last by default; SF (small
+// first) puts it first, so that the last stage is radix 8 for every N and a
+// thread's eight outputs lie in one sequence (what a per-row epilogue needs).
+template <int N, bool SF = false>
 struct Plan {
   static_assert((N & (N - 1)) == 0 && N >= 32 && N <= 2048, "power-of-two 32..2048");
   static constexpr int LOG = ilog2(N);
   static constexpr int NFULL = LOG / 3;
   static constexpr int REM = LOG % 3;
   static constexpr int NST = NFULL + (REM ? 1 : 0);
-  static constexpr int radix(int s) { return s < NFULL ? 8 : (1 << REM); }
+  static constexpr int radix(int s) {
+    return SF ? ((REM && s == 0) ? (1 << REM) : 8) : (s < NFULL ? 8 : (1 << REM));
+  }
   static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * radix(s - 1); }
 };
 
@@ -121,6 +127,10 @@ __device__ __forceinline__ void init_twiddles(float2* tw, int tid, int nthreads)
 // Element i of a sequence is stored at padded row i + i/8: the radix-8
 // Stockham write pattern (index 8j + r across lanes) then spreads over all
 // LDS banks instead of hitting one bank group 4-16 times (64 banks x 4 B).
+// (An XOR swizzle, i ^ ((i/8) mod 32), and a padded twiddle table remove
+// the remaining read conflicts of row transforms, 76 -> 22 extra cycles per
+// wave, but cost more VALU address work than they save in these VALU-bound
+// kernels: measured slower.)
 template <int N>
 struct Pad {
   static constexpr int ROWS = N + N / 8;
@@ -149,9 +159,9 @@ struct Layout {
 
 // One Stockham stage ST.  For ST == 0 inputs come from `load(m, i)`; for the
 // last stage outputs go to `store(m, i, v)`; otherwise LDS.
-template <int N, int T, int M, bool COLS, int S, int ST, class Load, class Store>
+template <int N, int T, int M, bool COLS, int S, int ST, bool SF, class Load, class Store>
 __device__ __forceinline__ void fft_stage(float2* lds, const float2* tw, int tid, Load& load, Store& store) {
-  using P = Plan<N>;
+  using P = Plan<N, SF>;
   using Lo = Layout<N, T, M, COLS>;
   constexpr int R = P::radix(ST);
   constexpr int NS = P::ns(ST);
@@ -203,13 +213,13 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* tw, int tid
   }
   if constexpr (!LAST) {
     __syncthreads();
-    fft_stage<N, T, M, COLS, S, ST + 1>(lds, tw, tid, load, store);
+    fft_stage<N, T, M, COLS, S, ST + 1, SF>(lds, tw, tid, load, store);
   }
 }
 
-template <int N, int T, int M, bool COLS, int S, class Load, class Store>
+template <int N, int T, int M, bool COLS, int S, bool SF = false, class Load, class Store>
 __device__ __forceinline__ void wg_fft(float2* lds, const float2* tw, int tid, Load& load, Store& store) {
-  fft_stage<N, T, M, COLS, S, 0>(lds, tw, tid, load, store);
+  fft_stage<N, T, M, COLS, S, 0, SF>(lds, tw, tid, load, store);
 }
 
 // ---------------------------------------------------------------------------

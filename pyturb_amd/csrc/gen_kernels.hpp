@@ -142,10 +142,7 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
       if (has1) prev1[c] = o1[(size_t)x * a.n_out];
     }
   }
-  if (P > 0) {
-    sub.template add_low<RL>(0, m, x0, P, v0);
-    if (has1) sub.template add_low<RL>(1, m, x0, P, v1);
-  }
+  if (P > 0) sub.template add_low2<RL>(m, x0, P, v0, v1, has1);
   const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
   const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
 #pragma unroll

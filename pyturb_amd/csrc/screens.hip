@@ -53,13 +53,13 @@ __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_n
 
 __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ disp,
                                                     const double2* __restrict__ coeffs, size_t cstride, int L,
-                                                    int P, int n, double inv_period, SubTables t) {
+                                                    int P, int n, double inv_period, SubTables t, int tr) {
   extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ double s_mean;
   double2* Q = (double2*)dyn;
   double2* SB = Q + 2 * (size_t)P * L;
   const int f = blockIdx.x, tid = threadIdx.x;
-  frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256);
+  frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256, tr != 0);
   sub_fill_set(SB, P, n, f, t, f == 0, &s_mean, tid, 256);
 }
 
@@ -73,12 +73,12 @@ int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int 
 }
 
 int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
-                      SubTables t, hipStream_t st, size_t cstride) {
+                      SubTables t, hipStream_t st, size_t cstride, bool transpose) {
   if (P == 0) return PT_OK;
   if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
   const size_t dyn = (size_t)(2 * P * L + 9 * P) * sizeof(double2);
   if (dyn > 60 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: %d layers x %d levels exceed LDS", L, P);
-  k_frame_prep<<<F, 256, dyn, st>>>(disp, coeffs, cstride, L, P, n, inv_period, t);
+  k_frame_prep<<<F, 256, dyn, st>>>(disp, coeffs, cstride, L, P, n, inv_period, t, transpose ? 1 : 0);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -188,7 +188,9 @@ __host__ __device__ constexpr int pair_tab_stride() { return N / 8 + 9; }
 // (j < N) and the Nyquist column k = (j - N, N/2) (N <= j < 2N, k0 != N/2),
 // with the phase of P taken exactly as one fixed-point sum.  Pass A adds
 // corr to the packed sum of those entries.
-template <int N>
+// TR (column-first engine): the tables split the row factor instead, t8/t1
+// from the row phase U_x, and the per-thread phase entry is U_y.
+template <int N, bool TR = false>
 __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restrict__ disp,
                                                            const float2* __restrict__ side, int L, int F,
                                                            double inv_period, float2* __restrict__ tab,
@@ -230,13 +232,14 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
       ux = cycles_fixed(disp[((size_t)t * L + l) * 2 + 0] * inv_period);
       uy = cycles_fixed(disp[((size_t)t * L + l) * 2 + 1] * inv_period);
     }
+    const uint32_t ut = TR ? ux : uy, uo = TR ? uy : ux;  // table axis, per-thread axis
     float2 v;
     if (e < NB)
-      v = phasor_fixed((uint32_t)(8 * e - N / 2) * uy);
+      v = phasor_fixed((uint32_t)(8 * e - N / 2) * ut);
     else if (e < NB + 8)
-      v = phasor_fixed((uint32_t)(e - NB) * uy);
+      v = phasor_fixed((uint32_t)(e - NB) * ut);
     else
-      v = make_float2(__uint_as_float(ux), 0.f);
+      v = make_float2(__uint_as_float(uo), 0.f);
     tp[q] = v;
   }
 }
@@ -353,6 +356,233 @@ __global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2*
   float2* out = work + (size_t)pr * plane + (size_t)row0 * N;
   auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
   wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
+}
+
+// ----------------------------------------------------------------------------
+// Column-first frame engine
+// ----------------------------------------------------------------------------
+// The same two transforms in the other order: pass A' forms the packed pair
+// sum Z and transforms it along columns in tiles of MT adjacent columns;
+// pass B' transforms rows and runs the epilogue.  Every HBM access of the
+// two passes is then a run of at least MT * 8 bytes (a full 128-byte line
+// for MT = 16, n <= 512) or a whole row, where the row-first order's column
+// pass writes its float32 frames in MT * 4-byte runs (64 B at n = 512).
+template <int N>
+struct PairColsCfg {
+  static constexpr int MT = N <= 512 ? 16 : 8192 / N;  // columns per tile (<= 1024 threads)
+  static constexpr int T = MT * N / 8;
+};
+
+// Folded stack for pass A' (same values as k_frame_fold): tile t = k1 / mt
+// of mt columns; with k0 = j + r * n/8, lane (j, m = k1 % mt) reads its
+// values r = 2h, 2h + 1 as float4 number ((h * (n/mt) + t) * n/8 + j) * mt + m
+// of the layer, so the lanes of one load read one contiguous run.
+__global__ __launch_bounds__(256) void k_frame_fold_tiles(const float2* __restrict__ spectra, int n, int mt,
+                                                          float2* __restrict__ fold, float2* __restrict__ side) {
+  const size_t plane = (size_t)n * n;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= plane) return;
+  const int l = blockIdx.y;
+  const int k0 = (int)(idx / n), k1 = (int)(idx % n);
+  const int m0 = (n - k0) & (n - 1), m1 = (n - k1) & (n - 1);
+  const float2 sv = spectra[(size_t)l * plane + idx];
+  const float2 sm = spectra[(size_t)l * plane + (size_t)m0 * n + m1];
+  const float2 A = make_float2(0.5f * sv.x, 0.5f * sv.y), B = make_float2(0.5f * sm.x, -0.5f * sm.y);
+  const int h = n / 2, nb = n / 8;
+  const int t = k1 / mt, m = k1 % mt, j = k0 % nb, r = k0 / nb;
+  const size_t dst =
+      (size_t)l * plane + ((((size_t)(r >> 1) * (n / mt) + t) * nb + j) * mt + m) * 2 + (r & 1);
+  if (k0 == h || k1 == h) {
+    fold[dst] = A;
+    side[(size_t)l * 2 * n + (k0 == h ? k1 : n + k0)] = B;
+  } else {
+    fold[dst] = make_float2(A.x + B.x, A.y + B.y);
+  }
+}
+
+// Bytes of pass A's dynamic LDS: the transform tile, with the phasor tables
+// (t8, t1 per layer and frame over the row index, the per-column factors
+// and the column phases) aliased at its start; they are dead before the
+// first stage writes the tile.
+template <int N>
+__host__ __device__ constexpr size_t pair_cols_lds(int L) {
+  constexpr int MT = PairColsCfg<N>::MT;
+  const size_t tile = (size_t)Layout<N, PairColsCfg<N>::T, MT, true>::SIZE * 8;
+  const size_t tabs = (size_t)2 * L * (N / 8 + 8 + MT) * 8 + (size_t)2 * L * 4;
+  return tile > tabs ? tile : tabs;
+}
+
+// Pass A' of a frame pair over one column tile: per layer the phasor is
+// (column factor, per thread) x (row factor t8[a] t1[b], (k0 + N/2) mod N
+// = 8a + b); the packed sum in FMA form as in pass A; the Nyquist terms from
+// k_frame_pair_tables; then the column transform, written as MT-column
+// runs of the c64 intermediate.
+template <int N>
+__global__ __launch_bounds__(PairColsCfg<N>::T) void k_frame_pair_cols(const float2* __restrict__ fold,
+                                                                      const float2* __restrict__ corr, int L,
+                                                                      const float2* __restrict__ ptab, int F,
+                                                                      float2* __restrict__ work) {
+  constexpr int T = PairColsCfg<N>::T, MT = PairColsCfg<N>::MT, NB = N / 8;
+  static_assert(T == MT * NB, "one column position per thread in the first stage");
+  extern __shared__ __align__(16) unsigned char dyn[];
+  __shared__ float2 tw[N];
+  float2* lds = (float2*)dyn;             // transform tile (after the tables are consumed)
+  float2* t8 = (float2*)dyn;              // [2][L][NB]
+  float2* t1 = t8 + 2 * L * NB;           // [2][L][8]
+  float2* py = t1 + 2 * L * 8;            // [2][L][MT]
+  uint32_t* U = (uint32_t*)(py + 2 * L * MT);  // [2][L] column phases
+  const int tid = threadIdx.x;
+  const int pr = blockIdx.y;
+  const int tile = blockIdx.x, c0 = tile * MT;
+  const bool two = 2 * pr + 1 < F;
+  constexpr int S = pair_tab_stride<N>();
+  const float2* tp = ptab + (size_t)pr * 2 * L * S;
+  for (int q = tid; q < 2 * L * S; q += T) {
+    const int fl = q / S, e = q % S;
+    const float2 v = tp[q];
+    if (e < NB)
+      t8[fl * NB + e] = v;
+    else if (e < NB + 8)
+      t1[fl * 8 + e - NB] = v;
+    else
+      U[fl] = __float_as_uint(v.x);
+  }
+  init_twiddles<N, 1>(tw, tid, T);
+  __syncthreads();
+  for (int q = tid; q < 2 * L * MT; q += T) py[q] = phasor_fixed((uint32_t)signed_freq(c0 + q % MT, N) * U[q / MT]);
+  __syncthreads();
+  // first-stage positions of this thread: column c0 + m, rows j + r * NB (r < 8)
+  const int m = tid % MT, j = tid / MT;
+  const int k1 = c0 + m;
+  const size_t plane = (size_t)N * N;
+  const float4* fp = (const float4*)fold + ((size_t)tile * NB + j) * MT + m;  // + h * N * NB per chunk
+  float2 z[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) z[r] = make_float2(0.f, 0.f);
+  // For N >= 64 the eight rows share (k0 + N/2) mod 8, so the column factor
+  // times t1 is one product per layer and frame; for N = 32 (row step 4)
+  // it alternates between two values.
+  constexpr bool BSAME = NB % 8 == 0;
+  const int b0 = (j + N / 2) & 7;
+  for (int l = 0; l < L; ++l) {
+    const float2 pa = py[l * MT + m], pb = py[(L + l) * MT + m];
+    const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
+    const float2 cb0 = two ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
+    float2 g[8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * N * NB];
+      g[2 * h] = make_float2(v.x, v.y);
+      g[2 * h + 1] = make_float2(v.z, v.w);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int qq = (j + r * NB + N / 2) & (N - 1);
+      float2 ca = ca0, cb = cb0;
+      if (!BSAME) {
+        ca = cmul(pa, t1[l * 8 + (qq & 7)]);
+        cb = two ? cmul(pb, t1[(L + l) * 8 + (qq & 7)]) : make_float2(0.f, 0.f);
+      }
+      const int a = qq >> 3;
+      const float2 ta = t8[l * NB + a], tb = t8[(L + l) * NB + a];
+      float qx = ca.x * ta.x;
+      qx = fmaf(-ca.y, ta.y, qx);
+      qx = fmaf(-cb.x, tb.y, qx);
+      qx = fmaf(-cb.y, tb.x, qx);
+      float qy = ca.x * ta.y;
+      qy = fmaf(ca.y, ta.x, qy);
+      qy = fmaf(cb.x, tb.x, qy);
+      qy = fmaf(-cb.y, tb.y, qy);
+      z[r].x = fmaf(g[r].x, qx, z[r].x);
+      z[r].x = fmaf(-g[r].y, qy, z[r].x);
+      z[r].y = fmaf(g[r].x, qy, z[r].y);
+      z[r].y = fmaf(g[r].y, qx, z[r].y);
+    }
+  }
+  // Nyquist row / column terms of this pair: corr[k1] on the Nyquist row
+  // (k0 = N/2, every k1), corr[N + k0] on the Nyquist column (k1 = N/2,
+  // k0 != N/2)
+  const float2* cp = corr + (size_t)pr * 2 * N;
+  constexpr int RN = (N / 2) / NB;  // element r holding row N/2 (for j == (N/2) % NB)
+  if (j == (N / 2) % NB) {
+    const float2 c = cp[k1];
+    z[RN].x += c.x;
+    z[RN].y += c.y;
+  }
+  if (k1 == N / 2) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int k0 = j + r * NB;
+      if (k0 == N / 2) continue;
+      const float2 c = cp[N + k0];
+      z[r].x += c.x;
+      z[r].y += c.y;
+    }
+  }
+  __syncthreads();  // every table read precedes the first stage's LDS writes
+  int cnt = 0;
+  auto load = [&](int, int) { return z[cnt++]; };
+  float2* out = work + (size_t)pr * plane + c0;
+  auto store = [&](int mm, int i, float2 v) { out[(size_t)i * N + mm] = v; };
+  wg_fft<N, T, MT, true, 1>(lds, tw, tid, load, store);
+}
+
+// Pass B' of the column-first engine: the row transform of pair b (a plan
+// whose last stage is radix 8, so a thread's eight outputs lie in one row x
+// at columns y0 + d * N/8), then the epilogue: real part to frame 2b,
+// imaginary part to frame 2b + 1, plus the subharmonic field
+//   low(x, y) = K'[s][x] + sum_p Re(U[p][y] H'[s][p][x])
+// from tables of the transposed field (k_frame_prep with tr), i.e. the
+// column epilogue's ColSub with rows and columns exchanged; then crop and
+// scale, whole rows per wave.
+template <int N>
+__global__ __launch_bounds__(RowsCfg<N>::T) void k_frame_rows_epi(GenArgs a) {
+  constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
+  static_assert(Plan<N, true>::radix(Plan<N, true>::NST - 1) == 8, "last stage radix 8");
+  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
+  __shared__ float2 tw[N];
+  using Sub = ColSub<N, M, 2, T>;
+  __shared__ Sub sub;
+  const int tid = threadIdx.x;
+  const int b = blockIdx.y;
+  const int r0 = blockIdx.x * M;
+  if (r0 + M <= a.off || r0 >= a.off + a.n_out) return;  // rows entirely outside the crop
+  const int P = a.levels;
+  const int s0 = b * a.s_mul, s1 = s0 + a.s_add;
+  const bool has1 = s1 < a.count;
+  const int sets[2] = {s0, has1 ? s1 : -1};
+  init_twiddles<N, 1>(tw, tid, T);
+  __syncthreads();
+  typename Sub::Regs pre;
+  if (P > 0) pre = Sub::prefetch(a.tab, sets, r0, P, tid);
+  const size_t plane = (size_t)N * N;
+  const float2* src = a.work + (size_t)b * plane + (size_t)r0 * N;
+  float v0[8], v1[8];
+  int cnt = 0;
+  auto load = [&](int m, int i) { return src[(size_t)m * N + i]; };
+  auto store = [&](int, int, float2 v) {
+    v0[cnt] = v.x;
+    v1[cnt] = v.y;
+    ++cnt;
+  };
+  wg_fft<N, T, M, false, 1, true>(lds, tw, tid, load, store);
+  if (P > 0) sub.commit(pre, P, tid);
+  __syncthreads();
+  const int m = tid / (N / 8), y0 = tid % (N / 8);
+  const int x = r0 + m - a.off;
+  if ((unsigned)x >= (unsigned)a.n_out) return;  // no barrier follows
+  if (P > 0) sub.template add_low2<8>(m, y0, P, v0, v1, has1);
+  const size_t oplane = (size_t)a.n_out * a.n_out;
+  float* o0 = a.out + (size_t)s0 * oplane + (size_t)x * a.n_out;
+  float* o1 = a.out + (size_t)s1 * oplane + (size_t)x * a.n_out;
+  const float f0 = a.scale, f1 = a.scale;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int y = y0 + c * (N / 8) - a.off;
+    if ((unsigned)y >= (unsigned)a.n_out) continue;
+    o0[y] = v0[c] * f0;
+    if (has1) o1[y] = v1[c] * f1;
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -477,6 +707,38 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
 template <int N>
 static int frames_cols_pow2(GenArgs a, hipStream_t st) {
   return gen_cols_pow2<N>(a, st);
+}
+
+// Column-first order: fold into column tiles, transposed pair tables, pass A'.
+template <int N>
+static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
+                               float2* work, hipStream_t st) {
+  constexpr int MT = PairColsCfg<N>::MT;
+  const size_t dyn = pair_cols_lds<N>(L);
+  if (dyn + (size_t)N * 8 > kLdsMax)
+    return fail(PT_ERR_UNSUPPORTED, "frames: %d layers exceed the LDS phasor tables at n=%d", L, N);
+  int rc = set_dyn_lds(k_frame_pair_cols<N>, dyn);
+  if (rc) return rc;
+  const FramesLayout w = frames_layout(N, F, L, 0);
+  float2* fold = work + w.fold;
+  float2* side = work + w.side;
+  float2* ptab = work + w.ptab;
+  float2* corr = work + w.corr;
+  k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, fold, side);
+  PT_LAUNCH_CHECK();
+  k_frame_pair_tables<N, true><<<(F + 1) / 2, 256, 0, st>>>(disp, side, L, F, inv_period, ptab, corr);
+  PT_LAUNCH_CHECK();
+  k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(fold, corr, L, ptab, F, work);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+// Column-first order, pass B' (the tables must come from a transposed prep).
+template <int N>
+static int frames_cf_cols_pow2(const GenArgs& a, hipStream_t st) {
+  k_frame_rows_epi<N><<<dim3(N / RowsCfg<N>::M, a.n_fft), RowsCfg<N>::T, 0, st>>>(a);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
 }
 
 #define PT_POW2_SWITCH(n, CALL) \
@@ -635,6 +897,25 @@ static int frames_check(int layers, int n, double pixel_scale, const double* dis
   return PT_OK;
 }
 
+// Transform order of the fused frame engine for size n: 0 column-first
+// (pass A' + pass B'), 1 row-first (pass A + pass B); PT_TUNE_FRAMES.
+static int frames_order(int n) { return tune_choice(PT_TUNE_FRAMES, n); }
+
+static int spectral_rows_impl(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
+                              int frames, void* work, int order, hipStream_t st) {
+  const double inv_period = 1.0 / ((double)n * pixel_scale);
+  if (order == 0) {
+#define CALL(N) return frames_cf_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
+    PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  } else {
+#define CALL(N) return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
+    PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  }
+  return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: n=%d", n);
+}
+
 extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
                                 int frames, void* work, void* stream) {
   if (!spectra) return fail(PT_ERR_ARG, "pt_spectral_rows: NULL spectra");
@@ -642,24 +923,12 @@ extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double p
   if (rc || frames == 0) return rc;
   if (!is_pow2_fast(n) || layers > FOLD_LMAX)
     return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: power-of-two n 32..2048 and <= %d layers only", FOLD_LMAX);
-  const double inv_period = 1.0 / ((double)n * pixel_scale);
-  hipStream_t st = (hipStream_t)stream;
-#define CALL(N) return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
-  PT_POW2_SWITCH(n, CALL)
-#undef CALL
-  return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: n=%d", n);
+  return spectral_rows_impl(spectra, layers, n, pixel_scale, disp, frames, work, frames_order(n),
+                            (hipStream_t)stream);
 }
-
-extern "C" int pt_spectral_cols(void* work, int layers, int n, double pixel_scale, const double* disp, int frames,
-                                const double* sh_coeffs, int levels, int crop_offset, int n_out, float scale,
-                                float* out, void* stream) {
-  if (!out) return fail(PT_ERR_ARG, "pt_spectral_cols: NULL out");
-  int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
-  if (rc || frames == 0) return rc;
-  if (!frames_fused(n, layers, levels))
-    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: power-of-two n, <= %d layers and <= %d levels only",
-                FOLD_LMAX, PREG);
-  hipStream_t st = (hipStream_t)stream;
+static int spectral_cols_impl(void* work, int layers, int n, double pixel_scale, const double* disp, int frames,
+                              const double* sh_coeffs, int levels, int crop_offset, int n_out, float scale,
+                              float* out, int order, hipStream_t st) {
   GenArgs a;
   a.work = (const float2*)work;
   a.out = out;
@@ -674,13 +943,32 @@ extern "C" int pt_spectral_cols(void* work, int layers, int n, double pixel_scal
   a.s_mul = 2;
   a.s_add = 1;
   a.scale = scale;
-  rc = launch_frame_prep(disp, (const double2*)sh_coeffs, frames, layers, levels, n, 1.0 / ((double)n * pixel_scale),
-                         a.tab, st);
+  int rc = launch_frame_prep(disp, (const double2*)sh_coeffs, frames, layers, levels, n,
+                             1.0 / ((double)n * pixel_scale), a.tab, st, 0, order == 0);
   if (rc) return rc;
-#define CALL(N) return frames_cols_pow2<N>(a, st);
-  PT_POW2_SWITCH(n, CALL)
+  if (order == 0) {
+#define CALL(N) return frames_cf_cols_pow2<N>(a, st);
+    PT_POW2_SWITCH(n, CALL)
 #undef CALL
+  } else {
+#define CALL(N) return frames_cols_pow2<N>(a, st);
+    PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  }
   return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: n=%d", n);
+}
+
+extern "C" int pt_spectral_cols(void* work, int layers, int n, double pixel_scale, const double* disp, int frames,
+                                const double* sh_coeffs, int levels, int crop_offset, int n_out, float scale,
+                                float* out, void* stream) {
+  if (!out) return fail(PT_ERR_ARG, "pt_spectral_cols: NULL out");
+  int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
+  if (rc || frames == 0) return rc;
+  if (!frames_fused(n, layers, levels))
+    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_cols: power-of-two n, <= %d layers and <= %d levels only",
+                FOLD_LMAX, PREG);
+  return spectral_cols_impl(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
+                            scale, out, frames_order(n), (hipStream_t)stream);
 }
 
 extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
@@ -690,10 +978,12 @@ extern "C" int pt_spectral_frames(const void* spectra, int layers, int n, double
   int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
   if (rc || frames == 0) return rc;
   if (frames_fused(n, layers, levels)) {
-    rc = pt_spectral_rows(spectra, layers, n, pixel_scale, disp, frames, work, stream);
+    const int order = frames_order(n);  // read once: both passes use the same order
+    hipStream_t st = (hipStream_t)stream;
+    rc = spectral_rows_impl(spectra, layers, n, pixel_scale, disp, frames, work, order, st);
     if (rc) return rc;
-    return pt_spectral_cols(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, scale,
-                            out, stream);
+    return spectral_cols_impl(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
+                              scale, out, order, st);
   }
   return pt_frames_generic(spectra, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
                            scale, out, work, stream);

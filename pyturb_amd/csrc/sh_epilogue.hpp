@@ -75,9 +75,13 @@ __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTa
 // Shifted 3x3 subharmonic blocks of one frame: for level p,
 //   S_p[a][b] = sum_l c[p][l][a][b] * qx_{p,l}^(a-1) * qy_{p,l}^(b-1),
 //   q_{p,l} = exp(2*pi*i * s_l / (N dx 3^(p+1)))   (FP64, reduced mod 1).
+// With `tr` each block is stored transposed (S_p[b][a]): tables built from
+// it describe the transposed field low(y, x), whose per-column terms are
+// the per-row terms of low(x, y) (the row-pass epilogue).
 // Q needs 2*P*L and SB 9*P double2 of LDS.  Ends with a barrier.
 __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* disp, const double2* coeffs,
-                                       int f, int L, int P, double inv_period, int tid, int nthreads) {
+                                       int f, int L, int P, double inv_period, int tid, int nthreads,
+                                       bool tr = false) {
   for (int q = tid; q < 2 * P * L; q += nthreads) {
     const int axis = q & 1, pl = q >> 1, p = pl / L, l = pl % L;
     const double u = disp[((size_t)f * L + l) * 2 + axis] * inv_period / pow3(p + 1);
@@ -97,7 +101,7 @@ __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* di
       acc.x += t.x;
       acc.y += t.y;
     }
-    SB[q] = acc;
+    SB[tr ? p * 9 + ib * 3 + ia : q] = acc;
   }
   __syncthreads();
 }
@@ -126,13 +130,16 @@ struct ColSub {
     float2 v[R];
   };
 
-  // element q of the packed list [U: P*N/8 (UXL only) | H: S*P*M | K: S*M]
+  // element q of the packed list [U: P*N/8 (UXL only) | H: P*S*M | K: S*M];
+  // H is packed level-major so that every index split divides by a
+  // compile-time constant (a split by the run-time P*M costs a full
+  // integer division per element).
   __device__ __forceinline__ static float2 fetch(const SubTables& t, const int (&set)[S], int c0, int P, int q) {
     const int eu = UXL ? P * (N / 8) : 0, eh = S * P * M;
     if (q < eu) return t.U[(size_t)(q / (N / 8)) * N + q % (N / 8)];
     q -= eu;
     if (q < eh) {
-      const int s = q / (P * M), r = q % (P * M), p = r / M, m = r % M;
+      const int p = q / (S * M), r = q % (S * M), s = r / M, m = r % M;
       return set[s] >= 0 ? t.H[((size_t)set[s] * P + p) * N + c0 + m] : make_float2(0.f, 0.f);
     }
     q -= eh;
@@ -161,8 +168,8 @@ struct ColSub {
       }
       q -= eu;
       if (q < eh) {
-        const int s = q / (P * M), rr = q % (P * M);
-        H[s][rr / M][rr % M] = g.v[r];
+        const int p = q / (S * M), rr = q % (S * M);
+        H[rr / M][p][rr % M] = g.v[r];
         continue;
       }
       q -= eh;
@@ -195,6 +202,34 @@ struct ColSub {
       }
     }
   }
+  // Both sets of a transform pair (S = 2) at once: the step table and the
+  // basis are read once per level for the two outputs.  Same arithmetic
+  // per output as add_low.
+  template <int RL>
+  __device__ __forceinline__ void add_low2(int m, int x0, int P, float (&v0)[8], float (&v1)[8],
+                                           bool has1) const {
+    static_assert(S == 2, "two coefficient sets");
+    if (!has1) {
+      add_low<RL>(0, m, x0, P, v0);
+      return;
+    }
+    const float k0 = K[0][m], k1 = K[1][m];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      v0[c] += k0;
+      v1[c] += k1;
+    }
+    for (int p = 0; p < P; ++p) {
+      const float2 e = UXL ? Ux[p][UXL ? x0 : 0] : sh_basis_e(p, x0, N);  // e_p(x0)
+      const float2 a0 = cmul(e, H[0][p][m]), a1 = cmul(e, H[1][p][m]);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float2 f = F[p][row_step<RL>(c)];
+        v0[c] += f.x * a0.x - f.y * a0.y;
+        v1[c] += f.x * a1.x - f.y * a1.y;
+      }
+    }
+  }
   // c-th output of a thread in the last Stockham stage (radix RL): stage
   // block q = c / RL, butterfly output r = c % RL, row x0 + (q + r * 8/RL) * N/8.
   template <int RL>
@@ -216,8 +251,10 @@ __device__ __forceinline__ float sub_eval(const SubTables& t, int s, int x, int 
 int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
                     hipStream_t st, int groups = 1);
 // cstride: coefficient sets per frame apart (0: one set for every frame).
+// transpose: tables of the transposed field (for the row-pass epilogue of
+// the column-first frame engine).
 int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, int P, int n, double inv_period,
-                      SubTables t, hipStream_t st, size_t cstride = 0);
+                      SubTables t, hipStream_t st, size_t cstride = 0, bool transpose = false);
 // Generic-size epilogue of frame-pair transforms: frame 2b = Re, 2b+1 = Im
 // of pair b, plus the subharmonic field, cropped and scaled.
 int launch_pair_epi(const float2* z, float* out, SubTables t, int n, int frames, int levels, int off, int n_out,

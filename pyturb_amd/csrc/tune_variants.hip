@@ -7,8 +7,12 @@
 // pass B): threads per workgroup x workgroup order
 //   variant 0: 1024 XCD-contiguous (default), 1: 512 XCD, 2: 256 XCD,
 //           3: 1024 flat, 4: 512 flat, 5: 256 flat.
+// Knob PT_TUNE_FRAMES (fused frame engine): transform order
+//   variant 0: column-first (pass A' column tiles, pass B' rows + epilogue),
+//           1: row-first (pass A rows, pass B column tiles + epilogue).
 // Thread counts are clamped to [N/8, N*N/8]; pt_tune_info reports the
-// effective shape, so variants that coincide at a size can be skipped.
+// effective shape (for PT_TUNE_FRAMES: pass A's threads and rows or
+// columns per workgroup), so variants that coincide at a size can be skipped.
 //
 // The selection is process-wide host state, read when a call launches: a
 // change applies to later calls, not to work already enqueued, and is not
@@ -21,12 +25,12 @@ namespace pt {
 
 namespace {
 
-constexpr int kKnobs = 2;
+constexpr int kKnobs = 3;
 constexpr int kLogMin = 5, kLogMax = 11;  // N = 32 .. 2048
 constexpr int kRowsThreads[] = {256, 128, 512};
 constexpr int kColsThreads[] = {1024, 512, 256, 1024, 512, 256};
 constexpr bool kColsXcd[] = {true, true, true, false, false, false};
-constexpr int kVariants[kKnobs] = {3, 6};
+constexpr int kVariants[kKnobs] = {3, 6, 2};
 
 int g_choice[kKnobs][kLogMax + 1] = {};
 
@@ -118,7 +122,11 @@ extern "C" int pt_tune_info(int knob, int n, int variant, int* threads, int* til
   if (log2_pow2(n) < 0) return fail(PT_ERR_ARG, "pt_tune_info: n=%d is not a power of two in 32..2048", n);
   if (variant < 0 || variant >= kVariants[knob])
     return fail(PT_ERR_ARG, "pt_tune_info: knob %d has no variant %d", knob, variant);
-  const int t = clamp_threads(n, knob == PT_TUNE_GEN_ROWS ? kRowsThreads[variant] : kColsThreads[variant]);
+  int t;
+  if (knob == PT_TUNE_FRAMES)  // pass A: 16-column tiles up to n = 512, else 1024 threads; 256-thread rows
+    t = variant == 0 ? (n <= 512 ? 2 * n : 1024) : clamp_threads(n, 256);
+  else
+    t = clamp_threads(n, knob == PT_TUNE_GEN_ROWS ? kRowsThreads[variant] : kColsThreads[variant]);
   if (threads) *threads = t;
   if (tile) *tile = 8 * t / n;
   if (xcd_order) *xcd_order = knob == PT_TUNE_GEN_COLS ? (kColsXcd[variant] ? 1 : 0) : 0;

@@ -153,6 +153,11 @@ def test_tune_state_is_host_side_and_validated():
     lib = _hip.lib()
     assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_ROWS) == 3
     assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_COLS) == 6
+    assert lib.pt_tune_variants(_hip.PT_TUNE_FRAMES) == 2
+    assert [(v["threads"], v["tile"]) for v in tune.variants("frames_order", 512)] == [(1024, 16),
+                                                                                      (256, 4)]
+    assert [(v["threads"], v["tile"]) for v in tune.variants("frames_order", 1024)] == [(1024, 8),
+                                                                                       (256, 2)]
     assert lib.pt_tune_variants(7) == -_hip.PT_ERR_ARG
     assert lib.pt_tune_set(7, 512, 0) == _hip.PT_ERR_ARG
     assert lib.pt_tune_set(0, 500, 0) == _hip.PT_ERR_ARG      # not a power of two
