@@ -145,6 +145,17 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
   if (P > 0) sub.template add_low2<RL>(m, x0, P, v0, v1, has1);
   const float f0 = a.scale * (a.screen_scale ? a.screen_scale[s0] : 1.0f);
   const float f1 = has1 ? a.scale * (a.screen_scale ? a.screen_scale[s1] : 1.0f) : 0.0f;
+  if (!ACC && a.off == 0 && a.n_out == N) {  // whole grid (uniform): no crop tests
+    float* p0 = o0 + (size_t)x0 * N;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) p0[(size_t)Sub::template row_step<RL>(c) * (N / 8) * N] = v0[c] * f0;
+    if (has1) {
+      float* p1 = o1 + (size_t)x0 * N;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) p1[(size_t)Sub::template row_step<RL>(c) * (N / 8) * N] = v1[c] * f1;
+    }
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int x = x0 + Sub::template row_step<RL>(c) * (N / 8) - a.off;

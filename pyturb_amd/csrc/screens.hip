@@ -575,13 +575,24 @@ __global__ __launch_bounds__(RowsCfg<N>::T) void k_frame_rows_epi(GenArgs a) {
   const size_t oplane = (size_t)a.n_out * a.n_out;
   float* o0 = a.out + (size_t)s0 * oplane + (size_t)x * a.n_out;
   float* o1 = a.out + (size_t)s1 * oplane + (size_t)x * a.n_out;
-  const float f0 = a.scale, f1 = a.scale;
+  const float f = a.scale;
+  if (a.off == 0 && a.n_out == N) {  // whole grid (uniform): constant offsets, no tests
+    float* p0 = o0 + y0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) p0[c * (N / 8)] = v0[c] * f;
+    if (has1) {
+      float* p1 = o1 + y0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) p1[c * (N / 8)] = v1[c] * f;
+    }
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int y = y0 + c * (N / 8) - a.off;
     if ((unsigned)y >= (unsigned)a.n_out) continue;
-    o0[y] = v0[c] * f0;
-    if (has1) o1[y] = v1[c] * f1;
+    o0[y] = v0[c] * f;
+    if (has1) o1[y] = v1[c] * f;
   }
 }
 
