@@ -114,13 +114,69 @@ struct Plan {
   static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * radix(s - 1); }
 };
 
-// Twiddle table exp(S*2*pi*i*t/N), t < N, written cooperatively into LDS.
+// Twiddles exp(2*pi*i*t/2048), t < 2048, evaluated at compile time: the
+// angle is reduced to an octant (|x| <= pi/4, from the exact 1/8-turn
+// symmetries), then sine and cosine are double-precision Taylor sums
+// (error below 1e-17) rounded once to float.  Every power-of-two N <= 2048
+// reads entry t * 2048/N, the same angle exactly.  Filling the LDS table
+// from this read-only array costs one load per entry where evaluating
+// sincospif in every workgroup cost about 40 VALU instructions per entry.
+struct TwEntry {
+  float c, s;
+};
+struct TwTable {
+  TwEntry e[2048];
+};
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double tw_sin_series(double x) {  // |x| <= pi/4
+  double x2 = x * x, term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double tw_cos_series(double x) {  // |x| <= pi/4
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+constexpr TwTable make_tw_table() {
+  TwTable t{};
+  for (int i = 0; i < 2048; ++i) {
+    // i/2048 turn = octant o (256 entries each) + remainder r, |r| <= 128
+    const int o = (i + 128) / 256, r = i - 256 * o;
+    const double x = kTwoPi * r / 2048.0;
+    const double sr = tw_sin_series(x), cr = tw_cos_series(x);
+    // rotate (cos r, sin r) by o eighths of a turn
+    const double h = 0.70710678118654752440084436210485;
+    double c = 0.0, s = 0.0;
+    switch (o & 7) {
+      case 0: c = cr; s = sr; break;
+      case 1: c = h * (cr - sr); s = h * (cr + sr); break;
+      case 2: c = -sr; s = cr; break;
+      case 3: c = -h * (cr + sr); s = h * (cr - sr); break;
+      case 4: c = -cr; s = -sr; break;
+      case 5: c = -h * (cr - sr); s = -h * (cr + sr); break;
+      case 6: c = sr; s = -cr; break;
+      default: c = h * (cr + sr); s = -h * (cr - sr); break;
+    }
+    t.e[i].c = (float)c;
+    t.e[i].s = (float)s;
+  }
+  return t;
+}
+__device__ constexpr TwTable kTw = make_tw_table();
+
+// Twiddle table exp(S*2*pi*i*t/N), t < N, copied cooperatively into LDS.
 template <int N, int S>
 __device__ __forceinline__ void init_twiddles(float2* tw, int tid, int nthreads) {
   for (int t = tid; t < N; t += nthreads) {
-    float s, c;
-    sincospif(2.0f * (float)t / (float)N, &s, &c);  // exact argument for power-of-two N
-    tw[t] = make_float2(c, S * s);
+    const TwEntry e = kTw.e[t * (2048 / N)];
+    tw[t] = make_float2(e.c, S * e.s);
   }
 }
 

@@ -198,7 +198,7 @@ struct ColSub {
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const float2 f = F[p][row_step<RL>(c)];
-        v[c] += f.x * a.x - f.y * a.y;
+        v[c] = fmaf(-f.y, a.y, fmaf(f.x, a.x, v[c]));  // two FMAs (not mul, fma, add)
       }
     }
   }
@@ -225,8 +225,8 @@ struct ColSub {
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const float2 f = F[p][row_step<RL>(c)];
-        v0[c] += f.x * a0.x - f.y * a0.y;
-        v1[c] += f.x * a1.x - f.y * a1.y;
+        v0[c] = fmaf(-f.y, a0.y, fmaf(f.x, a0.x, v0[c]));
+        v1[c] = fmaf(-f.y, a1.y, fmaf(f.x, a1.x, v1[c]));
       }
     }
   }
