@@ -13,8 +13,9 @@ Also reported on the same line:
     per-screen r0 ~ U(0.08, 0.25) m (seed 4), white noise resident in HBM;
   * the plain GPU form of both (torch ops, one kernel chain per layer, FP32
     everywhere -- the reference's array-module algorithm on ROCm);
-  * roofline of the dominant kernel (pass A of the frame engine) from HIP
-    events on the launch stream;
+  * roofline of the dominant kernel (pass A of the frame engine, in the
+    transform order the autotuner picked) from HIP events on the launch
+    stream;
   * the CPU baseline: the FP64 NumPy oracle timed on this host (1 thread);
   * a closed-loop call pattern, one frame per call, eager against a
     HIP-graph replay (pyturb_amd.graph.CapturedFrames), and the ensemble
@@ -307,12 +308,14 @@ def run_extrude(ext, thx, thy, mags, out, steps, t0=0):
         ext.integrate_dirs(thx, thy, scale=scale, mags=mags, out=out)
 
 
-def measured_traffic():
-    """HBM bytes per pass-A launch from the committed counter summary, or None."""
+def measured_traffic(order):
+    """HBM bytes per pass-A launch of the given transform order from the
+    committed counter summary, or None."""
+    key = "traffic_bytes_per_launch_" + ("column_first:" if order == 0 else "row_first:")
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return int(line.split(":", 1)[1].strip().split()[0])
     except (OSError, ValueError, IndexError):
         pass
@@ -574,7 +577,10 @@ def main(argv=None):
         extra["extrude_width"] = ext.width
         del ext, eout
         torch.cuda.empty_cache()
-    traffic = measured_traffic()
+    from pyturb_amd import tune as _tune
+
+    order = _tune.get("frames_order", N)
+    traffic = measured_traffic(order)
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, 1 thread)", file=sys.stderr, flush=True)
@@ -601,7 +607,8 @@ def main(argv=None):
                      "unit": "GB/s", "frac": round(ach_a / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_GBs": round(traffic / t_a / 1e9, 1) if traffic else None,
-                     "kernel": "k_frame_pair_rows<512> (pass A)",
+                     "kernel": ("k_frame_pair_cols<512> (pass A, column-first order)" if order == 0
+                                else "k_frame_pair_rows<512> (pass A, row-first order)"),
                      "pass_a_ms": round(t_a * 1e3, 4), "pass_b_ms": round(t_b * 1e3, 4),
                      "frame_algorithmic_GBs": round(frame_bytes * fps / 1e9, 1),
                      "pass_b_GBs": round(bytes_b / t_b / 1e9, 1)},
