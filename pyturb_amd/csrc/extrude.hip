@@ -389,8 +389,18 @@ __global__ __launch_bounds__(256) void k_readout(ReadoutArgs a) {
   }
 }
 
-constexpr int TB = 32;    // tile edge
-constexpr int BOXE = 60;  // box edge: 31 * sqrt(2) + taps + 2 spare
+constexpr int TB = 32;  // tile edge
+// Box edge per tap count: for magnification <= 1 (every NGS direction and
+// LGS cone) a tile's footprint spans at most floor(31 sqrt 2) = 43 rows and
+// columns between its extreme pixels, plus the taps and a spare on each
+// side (box_geom): 48 / 50 / 52.  A larger footprint takes the direct
+// gather.  Smaller boxes leave LDS for more workgroups per CU (bilinear:
+// 18.8 KiB, 8 workgroups per CU, against 5 with 60-wide boxes).
+template <int TAPS>
+struct BoxCfg {
+  static constexpr int E = 46 + TAPS;
+  static constexpr int ROWS = (E + 3) / 4;  // box rows per wave
+};
 
 // Footprint box of one (layer, direction) for a tile: local ring rows rb0..
 // and buffer columns cb0.. (before clamping), extents nr x nc.
@@ -421,7 +431,7 @@ __device__ __forceinline__ BoxGeom box_geom(const ReadoutArgs& a, int d, int l, 
   b.cb0 = (int)floorf(cmin) + OFF - 1;
   b.nr = (int)floorf(rmax) + OFF + TAPS + 1 - b.rb0;
   b.nc = (int)floorf(cmax) + OFF + TAPS + 1 - b.cb0;
-  b.fits = b.nr <= BOXE && b.nc <= BOXE;
+  b.fits = b.nr <= BoxCfg<TAPS>::E && b.nc <= BoxCfg<TAPS>::E;
   return b;
 }
 
@@ -432,13 +442,12 @@ __device__ __forceinline__ BoxGeom box_geom(const ReadoutArgs& a, int d, int l, 
 // [0, W-1] exactly as the per-tap clamp does; the taps are then read from
 // LDS.  The box of layer l + 1 is loaded into registers while layer l is
 // interpolated (two LDS buffers), so the gather latency overlaps the
-// arithmetic.  A footprint larger than the box (magnification > ~1.1) falls
+// arithmetic.  A footprint larger than the box (magnification > 1) falls
 // back to the direct gather for that layer.
-constexpr int BOX_ROWS = (BOXE + 3) / 4;  // box rows per wave
-
 template <int TAPS>
 __global__ __launch_bounds__(256) void k_readout_box(ReadoutArgs a) {
   constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
+  constexpr int BOXE = BoxCfg<TAPS>::E, BOX_ROWS = BoxCfg<TAPS>::ROWS;
   __shared__ float box[2][BOXE][BOXE + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int i0 = blockIdx.y * TB, j0 = blockIdx.x * TB;
