@@ -45,7 +45,9 @@ __all__ = ["Atmosphere", "PeriodicWrapWarning", "ExtrudeBoilingPerformanceWarnin
 
 _ARCSEC_TO_RAD = np.pi / (180.0 * 3600.0)
 _FRAME_CHUNK = 64
-_LGS_CHUNK = 64  # frames per per-layer transform batch of the LGS cone
+# per-layer screens of one LGS-cone batch (L x frames x ns^2 float32): a
+# few GiB of the 288 GB HBM, so each batch amortises its per-layer launches
+_LGS_BUDGET = 4 << 30
 
 
 def _lgs_zoom_taps(n: int, n_screen: int, mag, interp: str):
@@ -396,7 +398,7 @@ class Atmosphere:
         t = _hip.torch()
         F, L, ns = disp.shape[0], len(self._layers), self.n_screen
         out = t.empty((F, self.n, self.n), dtype=t.float32, device=self._dev)
-        k = min(F, _LGS_CHUNK)
+        k = max(1, min(F, 65535, _LGS_BUDGET // (4 * L * ns * ns)))
         screens = t.empty((L, k, ns, ns), dtype=t.float32, device=self._dev)
         for f0 in range(0, F, k):
             m = min(k, F - f0)

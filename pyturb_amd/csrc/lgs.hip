@@ -12,9 +12,13 @@
 // k_lgs_zoom: a workgroup owns a 32 x 32 output tile of one frame.  For
 // each layer it stages the tile's source window (rows idx[0][i0] ..
 // idx[T-1][i1] by the same columns; at most 31 mag + T + 1 <= 38 on a side
-// for mag <= 1) into LDS with coalesced row reads, interpolates along rows
+// for mag <= 1) and the tile's row taps in LDS, interpolates along rows
 // into a 32 x window-width LDS block, then along columns into registers.
-// Windows wider than the LDS box (mag > 1) read the screen directly.
+// The window and row taps of layer l + 1 are loaded into registers while
+// layer l is interpolated and committed to the second LDS buffer during
+// its column pass: two barriers per layer, and the loads' latency overlaps
+// the arithmetic.  Windows wider than the LDS box (mag > 1) read the screen
+// directly.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -23,62 +27,114 @@ namespace pt {
 
 constexpr int LT = 32;     // output tile edge
 constexpr int LBOX = 40;   // LDS window edge
+constexpr int LPW = (LBOX * LBOX + 255) / 256;  // window values per thread
+
+struct ZoomWin {
+  int r_lo, c_lo, nr, nc;
+  bool fits;
+};
 
 template <int T>
 __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ screens, int L, int F, int ns,
                                                   const int* __restrict__ idx, const float* __restrict__ w, int n,
                                                   float scale, float* __restrict__ out) {
-  __shared__ float box[LBOX][LBOX + 1];
+  static_assert(T * LT <= 256, "one row tap per thread");
+  __shared__ float box[2][LBOX][LBOX + 1];
   __shared__ float rows[LT][LBOX + 1];
+  __shared__ float rw[2][T][LT];  // row tap weights of the tile's rows
+  __shared__ int rix[2][T][LT];   // row tap indices, relative to the window's first row
   const int tid = threadIdx.x;
   const int i0 = blockIdx.y * LT, j0 = blockIdx.x * LT, f = blockIdx.z;
   const int i1 = min(i0 + LT, n) - 1, j1 = min(j0 + LT, n) - 1;
   const int jl = tid & 31, il = tid >> 5;  // pixel (i0 + il + 8k, j0 + jl), k < 4
   const size_t plane = (size_t)ns * ns;
+  const int ta = tid / LT, ti = tid % LT;  // this thread's row tap (tid < T * LT)
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  float pre[LPW];
+  float prw = 0.f;
+  int pri = 0;
+  // window of layer l: idx is non-decreasing in the pixel index and in the tap
+  auto window = [&](int l) {
+    const int* ix = idx + (size_t)l * T * n;
+    ZoomWin g;
+    g.r_lo = ix[i0];
+    g.c_lo = ix[j0];
+    g.nr = ix[(size_t)(T - 1) * n + i1] - g.r_lo + 1;
+    g.nc = ix[(size_t)(T - 1) * n + j1] - g.c_lo + 1;
+    g.fits = g.nr <= LBOX && g.nc <= LBOX;
+    return g;
+  };
+  auto fetch = [&](int l, const ZoomWin& g) {
+    if (!g.fits) return;
+    const float* S = screens + ((size_t)l * F + f) * plane;
+#pragma unroll
+    for (int k = 0; k < LPW; ++k) {
+      const int q = tid + 256 * k, r = q / LBOX, c = q % LBOX;
+      if (r < g.nr && c < g.nc) pre[k] = S[(size_t)(g.r_lo + r) * ns + g.c_lo + c];
+    }
+    if (tid < T * LT) {
+      const size_t e = ((size_t)l * T + ta) * n + min(i0 + ti, n - 1);
+      prw = w[e];
+      pri = idx[e] - g.r_lo;
+    }
+  };
+  auto commit = [&](int sel, const ZoomWin& g) {
+    if (!g.fits) return;
+#pragma unroll
+    for (int k = 0; k < LPW; ++k) {
+      const int q = tid + 256 * k, r = q / LBOX, c = q % LBOX;
+      if (r < g.nr && c < g.nc) box[sel][r][c] = pre[k];
+    }
+    if (tid < T * LT) {
+      rw[sel][ta][ti] = prw;
+      rix[sel][ta][ti] = pri;
+    }
+  };
+  ZoomWin cur = window(0);
+  fetch(0, cur);
+  commit(0, cur);
+  __syncthreads();
   for (int l = 0; l < L; ++l) {
+    const int sel = l & 1;
+    ZoomWin nxt;
+    nxt.fits = false;
+    if (l + 1 < L) {
+      nxt = window(l + 1);
+      fetch(l + 1, nxt);  // in flight while layer l is interpolated
+    }
     const float* S = screens + ((size_t)l * F + f) * plane;
     const int* ix = idx + (size_t)l * T * n;
     const float* wl = w + (size_t)l * T * n;
-    // window: idx is non-decreasing in the pixel index and in the tap
-    const int r_lo = ix[i0], r_hi = ix[(size_t)(T - 1) * n + i1];
-    const int c_lo = ix[j0], c_hi = ix[(size_t)(T - 1) * n + j1];
-    const int nr = r_hi - r_lo + 1, nc = c_hi - c_lo + 1;
-    if (nr <= LBOX && nc <= LBOX) {
-      for (int q = tid; q < nr * LBOX; q += 256) {
-        const int r = q / LBOX, c = q % LBOX;
-        if (c < nc) box[r][c] = S[(size_t)(r_lo + r) * ns + c_lo + c];
-      }
-      __syncthreads();
+    const int j = min(j0 + jl, n - 1);
+    float wb[T];
+    int cb[T];
+#pragma unroll
+    for (int b = 0; b < T; ++b) {
+      wb[b] = wl[(size_t)b * n + j];
+      cb[b] = ix[(size_t)b * n + j];
+    }
+    if (cur.fits) {
       // rows: rows[i][c] = sum_a w_a(i) box[idx_a(i) - r_lo][c]
+      const int nc = cur.nc;
       for (int q = tid; q < LT * nc; q += 256) {
         const int i = q / nc, c = q % nc;
-        const int gi = min(i0 + i, n - 1);
         float v = 0.f;
 #pragma unroll
-        for (int a = 0; a < T; ++a) v = fmaf(wl[(size_t)a * n + gi], box[ix[(size_t)a * n + gi] - r_lo][c], v);
+        for (int a = 0; a < T; ++a) v = fmaf(rw[sel][a][i], box[sel][rix[sel][a][i]][c], v);
         rows[i][c] = v;
       }
-      __syncthreads();
-      const int j = min(j0 + jl, n - 1);
-      float wb[T];
-      int cb[T];
-#pragma unroll
-      for (int b = 0; b < T; ++b) {
-        wb[b] = wl[(size_t)b * n + j];
-        cb[b] = ix[(size_t)b * n + j] - c_lo;
-      }
+    }
+    __syncthreads();  // rows complete
+    if (cur.fits) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         float v = 0.f;
 #pragma unroll
-        for (int b = 0; b < T; ++b) v = fmaf(wb[b], rows[il + 8 * k][cb[b]], v);
+        for (int b = 0; b < T; ++b) v = fmaf(wb[b], rows[il + 8 * k][cb[b] - cur.c_lo], v);
         acc[k] += v;
       }
-      __syncthreads();  // box and rows are refilled for the next layer
-    } else {
-      const int j = min(j0 + jl, n - 1);
-#pragma unroll
+    } else {  // magnification > 1: not unrolled, so it does not set the register budget
+#pragma unroll 1
       for (int k = 0; k < 4; ++k) {
         const int i = min(i0 + il + 8 * k, n - 1);
         float v = 0.f;
@@ -87,12 +143,18 @@ __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ scre
           const float* row = S + (size_t)ix[(size_t)a * n + i] * ns;
           float r = 0.f;
 #pragma unroll
-          for (int b = 0; b < T; ++b) r = fmaf(wl[(size_t)b * n + j], row[ix[(size_t)b * n + j]], r);
+          for (int b = 0; b < T; ++b) r = fmaf(wb[b], row[cb[b]], r);
           v = fmaf(wl[(size_t)a * n + i], r, v);
         }
         acc[k] += v;
       }
     }
+    if (l + 1 < L) {
+      // box[sel ^ 1] and its taps were last read in layer l - 1's row pass
+      commit(sel ^ 1, nxt);
+      cur = nxt;
+    }
+    __syncthreads();  // rows free for the next row pass; the committed window visible
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {

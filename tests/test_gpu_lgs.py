@@ -4,7 +4,8 @@
   1000-1090) for linear, cubic and Lanczos taps, on- and off-axis, on a
   field-of-view grid that is not a power of two;
 * the 9-layer 512^2 atmosphere with a 90 km beacon against the FP64 oracle;
-* many frames in one call (several transform chunks) against the oracle;
+* many frames in one call (several transform batches, the last one short)
+  against the oracle;
 * the cone with boiling, reference draws injected.
 """
 
@@ -59,10 +60,12 @@ def test_lgs_config2_512_matches_oracle(hip):
     assert err < 1e-5
 
 
-def test_lgs_many_frames_match_oracle(hip):
+def test_lgs_many_frames_match_oracle(hip, monkeypatch):
     import pyturb_amd as pa
+    import pyturb_amd.atmosphere as atmosphere_mod
 
-    print("LGS 150 frames n=64", flush=True)
+    print("LGS 150 frames n=64 in batches of 40", flush=True)
+    monkeypatch.setattr(atmosphere_mod, "_LGS_BUDGET", 4 * 9 * 64 * 64 * 40)
     kw = dict(diameter=8.0, n=64, L0=25.0, subharmonics=4, seed=3)
     atm = pa.Atmosphere([pa.Layer(*ly) for ly in config2_layer_tuples()], r0=0.15,
                         lgs_altitude=90e3, interp="linear", device="mi355x", **kw)
