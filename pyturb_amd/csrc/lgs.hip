@@ -114,14 +114,18 @@ __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ scre
       cb[b] = ix[(size_t)b * n + j];
     }
     if (cur.fits) {
-      // rows: rows[i][c] = sum_a w_a(i) box[idx_a(i) - r_lo][c]
+      // rows: rows[i][c] = sum_a w_a(i) box[idx_a(i) - r_lo][c], over the
+      // fixed LBOX-wide rows (a constant divisor; columns >= nc skipped)
       const int nc = cur.nc;
-      for (int q = tid; q < LT * nc; q += 256) {
-        const int i = q / nc, c = q % nc;
-        float v = 0.f;
+#pragma unroll 1
+      for (int q = tid; q < LT * LBOX; q += 256) {
+        const int i = q / LBOX, c = q % LBOX;
+        if (c < nc) {
+          float v = 0.f;
 #pragma unroll
-        for (int a = 0; a < T; ++a) v = fmaf(rw[sel][a][i], box[sel][rix[sel][a][i]][c], v);
-        rows[i][c] = v;
+          for (int a = 0; a < T; ++a) v = fmaf(rw[sel][a][i], box[sel][rix[sel][a][i]][c], v);
+          rows[i][c] = v;
+        }
       }
     }
     __syncthreads();  // rows complete
