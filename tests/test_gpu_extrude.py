@@ -26,7 +26,10 @@ def _np(t):
 def test_readout_matches_oracle_on_random_ring(hip, interp, n, W, cap):
     """Pure-function check of pt_readout: ring wrap, row/column clamping,
     per-direction magnification; n=16 runs the direct-gather kernel, n=80
-    the LDS box kernel with partial tiles."""
+    the LDS box kernel with partial tiles.  The last direction magnifies
+    above 1, so its layers' footprints exceed the box (sized for
+    magnification <= 1) and take the box kernel's direct-gather fallback,
+    one layer fitting and two not."""
     import torch
     from pyturb_amd.infinite import readout
     from pyturb_amd.ring import Ring
@@ -44,7 +47,8 @@ def test_readout_matches_oracle_on_random_ring(hip, interp, n, W, cap):
     c = 0.4 * n  # centre the footprints in the buffer; some taps still clamp
     geom = np.array([[[c + 0.3, 0.7, 1.0], [c + 2.9, -1.3, 1.0], [c - 0.5, 0.2, 1.0]],
                      [[c + 1.1, 2.2, 0.8], [c + 4.4, 0.0, 0.6], [c - 2.0, -3.0, 0.9]],
-                     [[2.5, -0.4, 1.0], [float(cap), 1.5, 1.0], [1.0, float(W), 0.7]]])
+                     [[2.5, -0.4, 1.0], [float(cap), 1.5, 1.0], [1.0, float(W), 0.7]],
+                     [[c + 0.6, -0.9, 1.3], [c + 1.7, 0.4, 1.0], [c - 1.0, 1.1, 1.5]]])
     dev = torch.device("cuda")
     got = _np(readout(torch.from_numpy(buf.astype(np.float32)).to(dev), cap, W, rings, cs,
                       geom, n, interp, scale=0.5))

@@ -95,3 +95,33 @@ def test_lgs_with_boiling_matches_oracle(hip):
     err = O.rel_rms(got, np.stack(ref))
     print("rel rms", err)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("interp", ["linear", "cubic"])
+def test_lgs_zoom_wide_windows_match_numpy(hip, interp):
+    """pt_lgs_zoom with magnification 1.4 on two of three layers: their
+    windows exceed the LDS box and take the direct path, the third (0.9)
+    is staged; against the separable sum evaluated in NumPy."""
+    import torch
+
+    from pyturb_amd import ops
+    from pyturb_amd.atmosphere import _lgs_zoom_taps
+
+    print("LGS zoom wide windows", interp, flush=True)
+    n, ns, F = 48, 96, 3
+    mags = [1.4, 0.9, 1.4]
+    idx, w = _lgs_zoom_taps(n, ns, mags, interp)
+    rng = np.random.default_rng(12)
+    scr = rng.standard_normal((len(mags), F, ns, ns)).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    got = ops.lgs_zoom(torch.tensor(scr, device=dev), torch.tensor(idx, device=dev),
+                       torch.tensor(w, device=dev), n, scale=0.7).double().cpu().numpy()
+    ref = np.zeros((F, n, n))
+    s64, w64 = scr.astype(np.float64), w.astype(np.float64)
+    for l in range(len(mags)):
+        for f in range(F):
+            # rows then columns: sum_a sum_b w_a(i) w_b(j) S[idx_a(i), idx_b(j)]
+            rows = np.einsum("ai,aic->ic", w64[l], s64[l, f][idx[l]])
+            ref[f] += np.einsum("bj,ibj->ij", w64[l], rows[:, idx[l]])
+    ref *= 0.7
+    assert O.rel_rms(got, ref) < 1e-6
