@@ -149,3 +149,22 @@ def test_frame_orders_agree_on_cropped_odd_calls(hip, tuned_reset, n, off, n_out
     print("rel rms", err)
     assert got[0].shape == (frames, n_out, n_out)
     assert err < 1e-6
+
+
+def test_tune_cache_applied_when_the_library_loads(hip, tuned_reset, tmp_path, monkeypatch):
+    """PYTURB_AMD_TUNE_CACHE names a cache file whose choices for this
+    device are applied when the library is (re)loaded; other devices'
+    entries are ignored."""
+    from pyturb_amd import _hip
+
+    tune = tuned_reset
+    path = tmp_path / "cache.json"
+    with open(path, "w") as fh:
+        json.dump({tune.device_key(): {"128": {"gen_rows": 2, "gen_cols": 4, "frames_order": 1}},
+                   "gfx000:1CU": {"256": {"gen_cols": 5}}}, fh)
+    monkeypatch.setenv("PYTURB_AMD_TUNE_CACHE", str(path))
+    monkeypatch.setattr(_hip, "_lib", None)
+    _hip.lib()  # the same shared object again: its process-wide selection is updated
+    assert (tune.get("gen_rows", 128), tune.get("gen_cols", 128), tune.get("frames_order", 128)) \
+        == (2, 4, 1)
+    assert tune.get("gen_cols", 256) == 0
