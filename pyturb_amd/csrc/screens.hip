@@ -60,8 +60,13 @@ __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ d
   double2* SB = Q + 2 * (size_t)P * L;
   const int f = blockIdx.x, tid = threadIdx.x;
   frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256, tr != 0);
-  sub_fill_set(SB, P, n, f, t, f == 0, &s_mean, tid, 256);
+  sub_fill_set(SB, P, n, f, t, f == 0, &s_mean, tid, 256, blockIdx.y, gridDim.y);
 }
+
+// Workgroups per frame (or pair) of the small per-call table kernels: one
+// each for large calls, up to 8 so that a call of a few frames does not run
+// on a handful of CUs.
+static inline int table_split(int units) { return units >= 256 ? 1 : (256 / units < 8 ? 256 / units : 8); }
 
 int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
                     hipStream_t st, int groups) {
@@ -78,7 +83,8 @@ int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, i
   if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
   const size_t dyn = (size_t)(2 * P * L + 9 * P) * sizeof(double2);
   if (dyn > 60 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: %d layers x %d levels exceed LDS", L, P);
-  k_frame_prep<<<F, 256, dyn, st>>>(disp, coeffs, cstride, L, P, n, inv_period, t, transpose ? 1 : 0);
+  k_frame_prep<<<dim3(F, table_split(F)), 256, dyn, st>>>(disp, coeffs, cstride, L, P, n, inv_period, t,
+                                                          transpose ? 1 : 0);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -197,8 +203,9 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
                                                            float2* __restrict__ corr) {
   constexpr int NB = N / 8, S = pair_tab_stride<N>();
   const int pr = blockIdx.x;
+  const int first = blockIdx.y * 256 + threadIdx.x, stride = gridDim.y * 256;  // entries split over blocks
   float2* tp = tab + (size_t)pr * 2 * L * S;
-  for (int q = threadIdx.x; q < 2 * N; q += 256) {
+  for (int q = first; q < 2 * N; q += stride) {
     const int k0 = q < N ? N / 2 : q - N, k1 = q < N ? q : N / 2;
     float2 acc = make_float2(0.f, 0.f);
     if (q < N || k0 != N / 2) {
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
     }
     corr[(size_t)pr * 2 * N + q] = acc;
   }
-  for (int q = threadIdx.x; q < 2 * L * S; q += 256) {
+  for (int q = first; q < 2 * L * S; q += stride) {
     const int fl = q / S, e = q % S;
     const int t = 2 * pr + fl / L, l = fl % L;
     uint32_t ux = 0u, uy = 0u;
@@ -708,7 +715,8 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   float2* corr = work + w.corr;
   k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
   PT_LAUNCH_CHECK();
-  k_frame_pair_tables<N><<<(F + 1) / 2, 256, 0, st>>>(disp, side, L, F, inv_period, ptab, corr);
+  k_frame_pair_tables<N><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(disp, side, L, F,
+                                                                                     inv_period, ptab, corr);
   PT_LAUNCH_CHECK();
   k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), PairCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, work);
   PT_LAUNCH_CHECK();
@@ -737,7 +745,8 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
   float2* corr = work + w.corr;
   k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, fold, side);
   PT_LAUNCH_CHECK();
-  k_frame_pair_tables<N, true><<<(F + 1) / 2, 256, 0, st>>>(disp, side, L, F, inv_period, ptab, corr);
+  k_frame_pair_tables<N, true><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
+      disp, side, L, F, inv_period, ptab, corr);
   PT_LAUNCH_CHECK();
   k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(fold, corr, L, ptab, F, work);
   PT_LAUNCH_CHECK();

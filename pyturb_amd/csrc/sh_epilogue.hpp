@@ -48,27 +48,48 @@ __host__ __device__ inline SubTables sub_tables(float2* base, int P, int n, int 
 
 // Fill the tables of one coefficient set (C: P blocks of 9, in LDS) for all
 // columns; `writeU` also writes the shared basis table.  256-thread blocks.
+// With `parts` > 1, block `part` of the set fills one contiguous share of
+// the columns (and a strided share of the basis entries).  The grid-mean
+// terms are computed one level per thread, and the columns in chunks of
+// SUB_CC with one thread per (level, column): the FP64 work is spread over
+// the block instead of running as per-thread level loops.  K sums its level
+// terms in ascending order, as a level loop would.
+constexpr int SUB_CC = 32;  // columns per chunk
+
 __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTables t, bool writeU,
-                                    double* s_mean, int tid, int nthreads) {
+                                    double* s_mean, int tid, int nthreads, int part = 0, int parts = 1) {
+  __shared__ double terms[PMAX];
+  __shared__ double g1s[PMAX * SUB_CC];
+  for (int p = tid; p < P; p += nthreads) terms[p] = sh_block_mean(C + p * 9, sh_mean_e(p, n));
+  if (writeU)
+    for (int idx = part * nthreads + tid; idx < P * n; idx += parts * nthreads)
+      t.U[idx] = sh_basis_e(idx / n, idx % n, n);
+  __syncthreads();
   if (tid == 0) {
     double m = 0.0;
-    for (int p = 0; p < P; ++p) m += sh_block_mean(C + p * 9, sh_mean_e(p, n));
+    for (int p = 0; p < P; ++p) m += terms[p];
     *s_mean = m;
   }
-  if (writeU)
-    for (int idx = tid; idx < P * n; idx += nthreads) t.U[idx] = sh_basis_e(idx / n, idx % n, n);
   __syncthreads();
   const double mean = *s_mean;
-  for (int y = tid; y < n; y += nthreads) {
-    double kacc = -mean;
-    for (int p = 0; p < P; ++p) {
+  const int cpp = (n + parts - 1) / parts, y0 = part * cpp, y1 = min(n, y0 + cpp);
+  for (int c0 = y0; c0 < y1; c0 += SUB_CC) {
+    const int nc = min(SUB_CC, y1 - c0);
+    for (int w = tid; w < P * nc; w += nthreads) {
+      const int p = w / nc, j = w % nc, y = c0 + j;
       double2 Hd;
       double g1;
       sh_column(C + p * 9, sh_basis_e(p, y, n), Hd, g1);
       t.H[((size_t)s * P + p) * n + y] = make_float2((float)Hd.x, (float)Hd.y);
-      kacc += g1;
+      g1s[p * SUB_CC + j] = g1;
     }
-    t.K[(size_t)s * n + y] = (float)kacc;
+    __syncthreads();
+    for (int j = tid; j < nc; j += nthreads) {
+      double kacc = -mean;
+      for (int p = 0; p < P; ++p) kacc += g1s[p * SUB_CC + j];
+      t.K[(size_t)s * n + c0 + j] = (float)kacc;
+    }
+    __syncthreads();  // g1s reused by the next chunk
   }
 }
 
