@@ -467,6 +467,14 @@ def main(argv=None):
         extra["loop_eager_frames_per_s"] = round(ncall / te, 1)
         extra["loop_graph_frames_per_s"] = round(ncall / tg, 1)
         del cap
+    # the public generator (chunked device calls, one yield per frame)
+    def frames_api():
+        for _ in atm.frames(DT, N_FRAMES):
+            pass
+
+    atm.reset()
+    tapi = timed(frames_api, 1, 1)
+    extra["frames_api_frames_per_s"] = round(N_FRAMES / tapi, 1)
     if not args.no_plain:
         ctx = plain_context(atm)
         nplain = 200

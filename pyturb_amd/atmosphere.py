@@ -44,7 +44,8 @@ from .utils import r0_at_wavelength, seeing_from_r0
 __all__ = ["Atmosphere", "PeriodicWrapWarning", "ExtrudeBoilingPerformanceWarning"]
 
 _ARCSEC_TO_RAD = np.pi / (180.0 * 3600.0)
-_FRAME_CHUNK = 64
+_FRAME_CHUNK = 64  # first frozen-flow chunk of frames(); later chunks double
+_FRAME_BUDGET = 1 << 30  # bytes of output frames per chunk at most (and 512 frames)
 # per-layer screens of one LGS-cone batch (L x frames x ns^2 float32): a
 # few GiB of the 288 GB HBM, so each batch amortises its per-layer launches
 _LGS_BUDGET = 4 << 30
@@ -472,8 +473,14 @@ class Atmosphere:
                 self._ext.boil_step(dt)  # no-op unless tau_boil is finite
             return
         left = int(steps)
+        # chunks of 64, 128, 256, ... frames up to a memory-sized cap: the
+        # first frame comes quickly, long runs amortise each call's fixed
+        # work (fold, tables, launches) over up to 512 frames
+        cap = max(_FRAME_CHUNK, min(512, _FRAME_BUDGET // (4 * self.n * self.n)))
+        size = _FRAME_CHUNK
         while left > 0:
-            k = min(left, _FRAME_CHUNK)
+            k = min(left, size)
+            size = min(2 * size, cap)
             times = []
             t = self._t
             for _ in range(k):
