@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 3
+#define PT_ABI_VERSION 4
 
 #define PT_OK 0
 #define PT_ERR_ARG 1         /* invalid argument (shape, count, NULL)      */
@@ -165,6 +165,23 @@ int pt_spectral_cols(void* work, int layers, int n, double pixel_scale,
                      const double* disp, int frames, const double* sh_coeffs,
                      int levels, int crop_offset, int n_out, float scale,
                      float* out, void* stream);
+
+/*
+ * The fold of pt_spectral_rows on its own, and pt_spectral_rows without it.
+ * The fold (the Hermitian-folded layer stack and its Nyquist side table,
+ * in `work`) depends only on the spectra, the layer count, n, the frame
+ * count (which places it in the workspace) and the transform order
+ * (PT_TUNE_FRAMES).  A caller whose spectra do not change between calls of
+ * the same frame count, such as a loop replaying a captured HIP graph,
+ * folds once with pt_spectral_fold and then calls pt_spectral_rows_folded
+ * + pt_spectral_cols per step.  No counterpart in the reference (its
+ * per-frame layer sum is atmosphere.py:910-917).  Power-of-two n only.
+ */
+int pt_spectral_fold(const void* spectra, int layers, int n, int frames,
+                     void* work, void* stream);
+int pt_spectral_rows_folded(int layers, int n, double pixel_scale,
+                            const double* disp, int frames, void* work,
+                            void* stream);
 
 /*
  * Ring-buffer and matrix rows below have a pitch `ld` >= width, a multiple

@@ -27,7 +27,7 @@ PT_ERR_UNSUPPORTED = 3
 PT_TUNE_GEN_ROWS = 0
 PT_TUNE_GEN_COLS = 1
 PT_TUNE_FRAMES = 2
-ABI_VERSION = 3  # PT_ABI_VERSION of include/pyturb_hip.h
+ABI_VERSION = 4  # PT_ABI_VERSION of include/pyturb_hip.h
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -47,6 +47,8 @@ SIGNATURES = {
     "pt_spectral_frames": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp]),
     "pt_spectral_rows": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _vp]),
     "pt_spectral_cols": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp]),
+    "pt_spectral_fold": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "pt_spectral_rows_folded": (_i, [_i, _i, _d, _vp, _i, _vp, _vp]),
     "pt_innovation_gemm": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "pt_extrude_rows": (_i, [_vp, _i, _i, _i, _vp, _vp, ctypes.c_longlong, _i, _vp, _vp, _vp,
                              _vp, _vp, _vp]),

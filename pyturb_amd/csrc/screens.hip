@@ -699,9 +699,13 @@ static FramesLayout frames_layout(int n, int F, int L, int P) {
   return w;
 }
 
+// Stages of pass A's call: the fold of the stack (FOLD), the pair tables and
+// pass A (PAIRS), or both.
+enum { STAGE_FOLD = 1, STAGE_PAIRS = 2, STAGE_ALL = 3 };
+
 template <int N>
 static int frames_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
-                            float2* work, hipStream_t st) {
+                            float2* work, hipStream_t st, int stages = STAGE_ALL) {
   constexpr int MR = PairCfg<N>::M;
   const size_t dyn_r = pair_rows_lds<N>(L);
   if (dyn_r + (size_t)Layout<N, PairCfg<N>::T, MR, false>::SIZE * 8 + (size_t)N * 8 > kLdsMax)
@@ -713,8 +717,11 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   float2* side = work + w.side;
   float2* ptab = work + w.ptab;
   float2* corr = work + w.corr;
-  k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
-  PT_LAUNCH_CHECK();
+  if (stages & STAGE_FOLD) {
+    k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
+    PT_LAUNCH_CHECK();
+  }
+  if (!(stages & STAGE_PAIRS)) return PT_OK;
   k_frame_pair_tables<N><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(disp, side, L, F,
                                                                                      inv_period, ptab, corr);
   PT_LAUNCH_CHECK();
@@ -731,7 +738,7 @@ static int frames_cols_pow2(GenArgs a, hipStream_t st) {
 // Column-first order: fold into column tiles, transposed pair tables, pass A'.
 template <int N>
 static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
-                               float2* work, hipStream_t st) {
+                               float2* work, hipStream_t st, int stages = STAGE_ALL) {
   constexpr int MT = PairColsCfg<N>::MT;
   const size_t dyn = pair_cols_lds<N>(L);
   if (dyn + (size_t)N * 8 > kLdsMax)
@@ -743,8 +750,11 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
   float2* side = work + w.side;
   float2* ptab = work + w.ptab;
   float2* corr = work + w.corr;
-  k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, fold, side);
-  PT_LAUNCH_CHECK();
+  if (stages & STAGE_FOLD) {
+    k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, fold, side);
+    PT_LAUNCH_CHECK();
+  }
+  if (!(stages & STAGE_PAIRS)) return PT_OK;
   k_frame_pair_tables<N, true><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
       disp, side, L, F, inv_period, ptab, corr);
   PT_LAUNCH_CHECK();
@@ -922,18 +932,43 @@ static int frames_check(int layers, int n, double pixel_scale, const double* dis
 static int frames_order(int n) { return tune_choice(PT_TUNE_FRAMES, n); }
 
 static int spectral_rows_impl(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
-                              int frames, void* work, int order, hipStream_t st) {
+                              int frames, void* work, int order, hipStream_t st, int stages = STAGE_ALL) {
   const double inv_period = 1.0 / ((double)n * pixel_scale);
   if (order == 0) {
-#define CALL(N) return frames_cf_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
+#define CALL(N)                                                                                              \
+  return frames_cf_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st, \
+                                stages);
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   } else {
-#define CALL(N) return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st);
+#define CALL(N)                                                                                                \
+  return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st, stages);
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   }
   return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows: n=%d", n);
+}
+
+extern "C" int pt_spectral_fold(const void* spectra, int layers, int n, int frames, void* work, void* stream) {
+  if (!spectra || !work) return fail(PT_ERR_ARG, "pt_spectral_fold: NULL array");
+  if (layers < 1 || frames < 0) return fail(PT_ERR_ARG, "pt_spectral_fold: bad layers/frames");
+  if (frames > 65535) return fail(PT_ERR_ARG, "spectral frames: frames > 65535 per call");
+  if (!is_pow2_fast(n) || layers > FOLD_LMAX)
+    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_fold: power-of-two n 32..2048 and <= %d layers only", FOLD_LMAX);
+  if (frames == 0) return PT_OK;
+  return spectral_rows_impl(spectra, layers, n, 1.0, nullptr, frames, work, frames_order(n), (hipStream_t)stream,
+                            STAGE_FOLD);
+}
+
+extern "C" int pt_spectral_rows_folded(int layers, int n, double pixel_scale, const double* disp, int frames,
+                                       void* work, void* stream) {
+  int rc = frames_check(layers, n, pixel_scale, disp, frames, nullptr, 0, 0, n, work);
+  if (rc || frames == 0) return rc;
+  if (!is_pow2_fast(n) || layers > FOLD_LMAX)
+    return fail(PT_ERR_UNSUPPORTED, "pt_spectral_rows_folded: power-of-two n 32..2048 and <= %d layers only",
+                FOLD_LMAX);
+  return spectral_rows_impl(nullptr, layers, n, pixel_scale, disp, frames, work, frames_order(n), (hipStream_t)stream,
+                            STAGE_PAIRS);
 }
 
 extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double pixel_scale, const double* disp,

@@ -10,10 +10,11 @@ one launch from the host.
   noise that the caller refreshes in place between replays).
 * ``CapturedFrames(atm, frames, directions)`` is ``atm.opd_batch`` for a
   fixed number of times.  A closed-loop simulation asks for a few frames at
-  a time, over and over, and each eager call is then dominated by launch and
-  host overhead: five launches through ctypes plus a displacement upload.
-  Each call refreshes only the displacement table, with one host-to-device
-  copy, and replays the graph.
+  a time, over and over, and each eager call is then dominated by fixed
+  costs: five launches through ctypes, a displacement upload and the fold
+  of the unchanged layer stack.  The stack is folded once at construction
+  (``ops.spectral_fold``); each call refreshes only the displacement table,
+  with one host-to-device copy, and replays the four per-call kernels.
 """
 
 from __future__ import annotations
@@ -80,13 +81,19 @@ class CapturedFrames:
         self.work = t.empty((ops.frames_work_elems(atm.n_screen, F, L, P),), dtype=t.complex64,
                             device=dev)
         self._copied = t.cuda.Event()  # the pinned table is free once this completes
+        ns = atm.n_screen
+        # the fused engine's fold depends only on the (fixed) spectra: fold
+        # once here and record only the per-call kernels
+        self.folded = (ns & (ns - 1)) == 0 and 32 <= ns <= 2048 and L <= 16 and P <= 12
+        if self.folded:
+            ops.spectral_fold(atm._spectra, F, self.work)
         self._cap = Captured(self._launch, dev)
 
     def _launch(self) -> None:
         a = self.atm
         ops.spectral_frames(a._spectra, self.disp, a._sh_coeffs, a.pixel_scale,
                             crop_offset=a.margin_pix, n_out=a.n, scale=self.scale,
-                            out=self.out, work=self.work)
+                            out=self.out, work=self.work, folded=self.folded)
 
     def __call__(self, times: Sequence[float]) -> Any:
         a = self.atm

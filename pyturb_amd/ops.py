@@ -152,12 +152,27 @@ def generate_groups(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offs
     return out
 
 
+def spectral_fold(spectra, frames: int, work):
+    """Fold the layer stack into ``work`` for calls of ``frames`` frames
+    (pt_spectral_fold); then ``spectral_frames(..., folded=True)`` skips it."""
+    t = _t()
+    L, n = int(spectra.shape[0]), int(spectra.shape[-1])
+    _require(spectra, t.complex64, (L, n, n), "spectra")
+    rc = _hip.lib().pt_spectral_fold(_hip.ptr(spectra), L, n, int(frames), _hip.ptr(work),
+                                     _hip.stream_ptr())
+    _hip.check(rc, "pt_spectral_fold")
+
+
 def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset: int = 0,
-                    n_out: Optional[int] = None, scale: float = 1.0, out=None, work=None):
+                    n_out: Optional[int] = None, scale: float = 1.0, out=None, work=None,
+                    folded: bool = False):
     """Frozen-flow frames (see pt_spectral_frames).
 
     ``spectra`` (L, n, n) complex64; ``disp`` (F, L, 2) float64 metres;
     ``sh_coeffs`` (P, L, 3, 3) complex128 or None.  Returns (F, n_out, n_out).
+    With ``folded``, ``work`` already holds the fold of these spectra for
+    this frame count (``spectral_fold``): pt_spectral_rows_folded +
+    pt_spectral_cols (power-of-two n, <= 16 layers, <= 12 levels).
     """
     t = _t()
     L, n = int(spectra.shape[0]), int(spectra.shape[-1])
@@ -172,6 +187,18 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
         out = t.empty((F, n_out, n_out), dtype=t.float32, device=spectra.device)
     _require(out, t.float32, (F, n_out, n_out), "out")
     need = frames_work_elems(n, F, L, P)
+    if folded:
+        if work is None or work.numel() < need:
+            raise ValueError("folded: pass the workspace that spectral_fold filled")
+        lib = _hip.lib()
+        rc = lib.pt_spectral_rows_folded(L, n, float(pixel_scale), _hip.ptr(disp), F, _hip.ptr(work),
+                                         _hip.stream_ptr())
+        _hip.check(rc, "pt_spectral_rows_folded")
+        rc = lib.pt_spectral_cols(_hip.ptr(work), L, n, float(pixel_scale), _hip.ptr(disp), F,
+                                  _hip.ptr(sh_coeffs) if P else None, P, int(crop_offset), n_out,
+                                  float(scale), _hip.ptr(out), _hip.stream_ptr())
+        _hip.check(rc, "pt_spectral_cols")
+        return out
     if work is None or work.numel() < need:
         work = t.empty((need,), dtype=t.complex64, device=spectra.device)
     rc = _hip.lib().pt_spectral_frames(_hip.ptr(spectra), L, n, float(pixel_scale),

@@ -86,3 +86,33 @@ def test_captured_mixed_ensemble_replays_match_eager(hip):
         for g, c in zip(got, classes):
             assert torch.isfinite(g).all()
             assert torch.equal(g, c["out"]), (k, c["n"])
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_folded_frames_equal_unfolded(hip, order):
+    """spectral_fold once, then spectral_frames(folded=True) for two calls
+    with different displacements: bit-identical to the unfolded call, for
+    both transform orders."""
+    import torch
+
+    from pyturb_amd import ops, tune
+
+    print(f"folded frames order {order}", flush=True)
+    n, L, F, P = 128, 5, 3, 4
+    rng = np.random.default_rng(40 + order)
+    dev = torch.device("cuda", 0)
+    spec = (rng.standard_normal((L, n, n)) + 1j * rng.standard_normal((L, n, n))) / n
+    spectra = torch.tensor(spec.astype(np.complex64), device=dev)
+    cf = torch.tensor(rng.standard_normal((P, L, 3, 3)) * 0.1 + 0j, dtype=torch.complex128,
+                      device=dev)
+    work = torch.empty((ops.frames_work_elems(n, F, L, P),), dtype=torch.complex64, device=dev)
+    try:
+        tune.set_variant("frames_order", n, order)
+        ops.spectral_fold(spectra, F, work)
+        for k in range(2):
+            disp = torch.tensor(rng.uniform(-2.0, 2.0, (F, L, 2)), dtype=torch.float64, device=dev)
+            got = ops.spectral_frames(spectra, disp, cf, 0.05, work=work, folded=True).clone()
+            ref = ops.spectral_frames(spectra, disp, cf, 0.05)
+            assert torch.equal(got, ref), k
+    finally:
+        tune.reset()
