@@ -196,7 +196,12 @@ def autotune(sizes: Iterable[int] = (256, 512, 1024), *, screens: Optional[int] 
         times: Dict[int, float] = {}
         for var in variants(knob, n):
             set_variant(knob, n, var["variant"])
-            ms = _time_ms(run, repeats)
+            try:
+                ms = _time_ms(run, repeats)
+            except (ValueError, NotImplementedError, _hip.HipError) as exc:
+                warnings.warn(f"autotune: {knob} variant {var} at n={n} failed ({exc}); skipped",
+                              RuntimeWarning, stacklevel=3)
+                continue
             if exact:
                 ok = bool(t.equal(out, ref))
             else:
@@ -208,7 +213,7 @@ def autotune(sizes: Iterable[int] = (256, 512, 1024), *, screens: Optional[int] 
             times[var["variant"]] = ms
             if verbose:
                 print(f"autotune n={n} {knob} {var}: {ms:.3f} ms")
-        best = min(times, key=times.get)
+        best = min(times, key=times.get) if times else 0
         set_variant(knob, n, best)
         entry[knob] = best
         entry["ms"][knob] = {str(v): round(ms, 4) for v, ms in times.items()}
