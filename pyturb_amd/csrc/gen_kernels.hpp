@@ -68,8 +68,10 @@ __global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* _
       const float* rg = re + (size_t)g * gstride;
       const float* ig = rg + im_off;
       const float a = am[(size_t)g * plane + o];
-      acc.x = fmaf(rg[o], a, acc.x);
-      acc.y = fmaf(ig[o], a, acc.y);
+      // the noise is read once: non-temporal, so it does not evict the
+      // amplitude rows that every screen pair re-reads from L2
+      acc.x = fmaf(__builtin_nontemporal_load(rg + o), a, acc.x);
+      acc.y = fmaf(__builtin_nontemporal_load(ig + o), a, acc.y);
     }
     return acc;
   };

@@ -566,7 +566,12 @@ __global__ __launch_bounds__(RowsCfg<N>::T) void k_frame_rows_epi(GenArgs a) {
   const float2* src = a.work + (size_t)b * plane + (size_t)r0 * N;
   float v0[8], v1[8];
   int cnt = 0;
-  auto load = [&](int m, int i) { return src[(size_t)m * N + i]; };
+  // the intermediate is read once: non-temporal, so the tables stay in L2
+  auto load = [&](int m, int i) {
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    const v2f v = __builtin_nontemporal_load((const v2f*)(src + (size_t)m * N + i));
+    return make_float2(v.x, v.y);
+  };
   auto store = [&](int, int, float2 v) {
     v0[cnt] = v.x;
     v1[cnt] = v.y;
