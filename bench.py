@@ -308,10 +308,11 @@ def run_extrude(ext, thx, thy, mags, out, steps, t0=0):
         ext.integrate_dirs(thx, thy, scale=scale, mags=mags, out=out)
 
 
-def measured_traffic(order):
-    """HBM bytes per pass-A launch of the given transform order from the
+def measured_traffic(order, what="traffic_bytes_per_launch"):
+    """HBM bytes (or, with ``what="valu_insts_per_launch"``, wave64 VALU
+    instructions) per pass-A launch of the given transform order from the
     committed counter summary, or None."""
-    key = "traffic_bytes_per_launch_" + ("column_first:" if order == 0 else "row_first:")
+    key = what + "_" + ("column_first:" if order == 0 else "row_first:")
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:
@@ -589,6 +590,10 @@ def main(argv=None):
 
     order = _tune.get("frames_order", N)
     traffic = measured_traffic(order)
+    valu = measured_traffic(order, "valu_insts_per_launch")
+    # VALU issue share of pass A at the 2.4 GHz peak clock: 1024 SIMDs, 4 cycles per wave64
+    # FP32 instruction (packed FP32 is not used)
+    valu_frac = round(valu * 4 / 1024 / 2.4e9 / t_a, 3) if valu else None
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, 1 thread)", file=sys.stderr, flush=True)
@@ -618,6 +623,7 @@ def main(argv=None):
                      "kernel": ("k_frame_pair_cols<512> (pass A, column-first order)" if order == 0
                                 else "k_frame_pair_rows<512> (pass A, row-first order)"),
                      "pass_a_ms": round(t_a * 1e3, 4), "pass_b_ms": round(t_b * 1e3, 4),
+                     "pass_a_valu_issue_frac": valu_frac,
                      "frame_algorithmic_GBs": round(frame_bytes * fps / 1e9, 1),
                      "pass_b_GBs": round(bytes_b / t_b / 1e9, 1)},
         "cpu_baseline": cpu,
