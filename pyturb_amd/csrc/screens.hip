@@ -15,6 +15,8 @@
 // is the only extra HBM traffic.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "fft_core.hpp"
 #include "boil.hpp"
@@ -148,7 +150,75 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __rest
 // each of a thread's four 16-byte loads is a fully coalesced wave load
 // (with the eight values contiguous per thread, lanes 64 bytes apart, every
 // load touched 32 cache lines and the texture-data path saturated).
+//
+// Uniform step.  The phase of frame t, layer l at k is the fixed-point sum
+// s(k0) u_x + s(k1) u_y (mod 2^32, s the signed frequency).  When the second
+// frame of every pair is its first moved by one and the same step per layer,
+// d_l = u(2p + 1) - u(2p), then P_l,2p+1 = P_l,2p D_l with
+// D_l(k) = exp(2 pi i 2^-32 (s(k0) d_x + s(k1) d_y)), and
+//   (A_l + B_l)(P_1 + i P_2) = ((A_l + B_l)(1 + i D_l)) P_1,
+// so the fold stores (A_l + B_l)(1 + i D_l) (A_l (1 + i D_l) on the Nyquist
+// lines) and pass A forms one phasor product per layer and element instead of
+// two.  k_frame_step decides per call: the step of pair 0 is d, and every pair
+// must match it to within one unit of 2^-32 cycles per axis (host-rounded
+// displacements of equally spaced times differ by that much).  Frame 2p + 1
+// is then built with phase u(2p) + d, at most N/2 units = N/2 * 2^-32 cycles
+// from its own.  Otherwise the fold is plain and pass A forms both products.
+//
+// The check runs in STEP_BLOCKS workgroups, each writing whether its share of
+// the pairs matches: step[b] = 1 (b < STEP_BLOCKS); step[STEP_BLOCKS + 2l] =
+// d_x, step[STEP_BLOCKS + 1 + 2l] = d_y.  The step is uniform when every
+// step[b] is 1.
+constexpr int FOLD_LMAX = 16;  // layers of the fused (paired) frame path
+constexpr int STEP_BLOCKS = 16;
+constexpr int STEP_WORDS = STEP_BLOCKS + 2 * FOLD_LMAX;
+
+__global__ __launch_bounds__(1024) void k_frame_step(const double* __restrict__ disp, int L, int F,
+                                                     double inv_period, uint32_t* __restrict__ step) {
+  __shared__ uint32_t d0[2 * FOLD_LMAX];
+  const int tid = threadIdx.x;
+  const int E = 2 * L;
+  if (F < 2) {
+    if (tid == 0) step[blockIdx.x] = 0u;
+    return;
+  }
+  if (tid < E) d0[tid] = cycles_fixed(disp[E + tid] * inv_period) - cycles_fixed(disp[tid] * inv_period);
+  __syncthreads();
+  int bad = 0;
+  const int pairs = F / 2;  // pairs with two frames
+  for (int q = blockIdx.x * 1024 + tid; q < (pairs - 1) * E; q += STEP_BLOCKS * 1024) {
+    const int p = 1 + q / E, e = q % E;
+    const uint32_t u0 = cycles_fixed(disp[(size_t)(2 * p) * E + e] * inv_period);
+    const uint32_t u1 = cycles_fixed(disp[(size_t)(2 * p + 1) * E + e] * inv_period);
+    const int32_t dd = (int32_t)(u1 - u0 - d0[e]);
+    if (dd > 1 || dd < -1) bad = 1;
+  }
+  bad = __syncthreads_or(bad);
+  if (tid == 0) step[blockIdx.x] = bad ? 0u : 1u;
+  if (blockIdx.x == 0 && tid < E) step[STEP_BLOCKS + tid] = d0[tid];
+}
+
+__device__ __forceinline__ bool step_uniform(const uint32_t* __restrict__ step) {
+  if (!step) return false;
+  uint32_t all = 1u;
+#pragma unroll
+  for (int b = 0; b < STEP_BLOCKS; ++b) all &= step[b];
+  return all != 0u;
+}
+
+// v (1 + i D) with D the uniform step's phasor at (k0, k1), or v itself
+// for no uniform step.
+__device__ __forceinline__ float2 fold_step(float2 v, const uint32_t* __restrict__ step, int l, int k0, int k1,
+                                            int n) {
+  if (!step_uniform(step)) return v;
+  const uint32_t* d = step + STEP_BLOCKS + 2 * l;
+  const uint32_t ph = (uint32_t)signed_freq(k0, n) * d[0] + (uint32_t)signed_freq(k1, n) * d[1];
+  const float2 vd = cmul(v, phasor_fixed(ph));
+  return make_float2(v.x - vd.y, v.y + vd.x);
+}
+
 __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ spectra, int n,
+                                                    const uint32_t* __restrict__ step,
                                                     float2* __restrict__ fold, float2* __restrict__ side) {
   const size_t plane = (size_t)n * n;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -163,10 +233,10 @@ __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ s
   const int j = k1 % nb, r = k1 / nb;
   const size_t dst = (size_t)l * plane + (size_t)k0 * n + (size_t)(r >> 1) * 2 * nb + 2 * j + (r & 1);
   if (k0 == h || k1 == h) {
-    fold[dst] = A;
+    fold[dst] = fold_step(A, step, l, k0, k1, n);
     side[(size_t)l * 2 * n + (k0 == h ? k1 : n + k0)] = B;
   } else {
-    fold[dst] = make_float2(A.x + B.x, A.y + B.y);
+    fold[dst] = fold_step(make_float2(A.x + B.x, A.y + B.y), step, l, k0, k1, n);
   }
 }
 
@@ -196,14 +266,19 @@ __host__ __device__ constexpr int pair_tab_stride() { return N / 8 + 9; }
 // corr to the packed sum of those entries.
 // TR (column-first engine): the tables split the row factor instead, t8/t1
 // from the row phase U_x, and the per-thread phase entry is U_y.
+// With a uniform step (step_uniform) the main term builds the second frame of
+// each pair at phase u(2p) + d, so the Nyquist sums use that phase too, and
+// the pair stays Hermitian in both parts.  The last pair of an odd frame
+// count includes that second frame as well (it is not written).
 template <int N, bool TR = false>
 __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restrict__ disp,
                                                            const float2* __restrict__ side, int L, int F,
-                                                           double inv_period, float2* __restrict__ tab,
-                                                           float2* __restrict__ corr) {
+                                                           double inv_period, const uint32_t* __restrict__ step,
+                                                           float2* __restrict__ tab, float2* __restrict__ corr) {
   constexpr int NB = N / 8, S = pair_tab_stride<N>();
   const int pr = blockIdx.x;
   const int first = blockIdx.y * 256 + threadIdx.x, stride = gridDim.y * 256;  // entries split over blocks
+  const bool uni = step_uniform(step);
   float2* tp = tab + (size_t)pr * 2 * L * S;
   for (int q = first; q < 2 * N; q += stride) {
     const int k0 = q < N ? N / 2 : q - N, k1 = q < N ? q : N / 2;
@@ -215,9 +290,15 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
         for (int f = 0; f < 2; ++f) {
           const int t = 2 * pr + f;
           w[f] = make_float2(0.f, 0.f);
-          if (t >= F) continue;
-          const uint32_t ux = cycles_fixed(disp[((size_t)t * L + l) * 2 + 0] * inv_period);
-          const uint32_t uy = cycles_fixed(disp[((size_t)t * L + l) * 2 + 1] * inv_period);
+          if (t >= F && !uni) continue;
+          const bool virt = uni && f == 1;  // second frame of a uniform step: u(2p) + d
+          const int ts = virt ? t - 1 : t;
+          uint32_t ux = cycles_fixed(disp[((size_t)ts * L + l) * 2 + 0] * inv_period);
+          uint32_t uy = cycles_fixed(disp[((size_t)ts * L + l) * 2 + 1] * inv_period);
+          if (virt) {
+            ux += step[STEP_BLOCKS + 2 * l];
+            uy += step[STEP_BLOCKS + 1 + 2 * l];
+          }
           uint32_t ph = (uint32_t)signed_freq(k0, N) * ux + (uint32_t)signed_freq(k1, N) * uy;
           if (k0 == N / 2) ph += (uint32_t)N * ux;  // E: exp(2 pi i s_0 / dx)
           if (k1 == N / 2) ph += (uint32_t)N * uy;  // E: exp(2 pi i s_1 / dx)
@@ -264,6 +345,7 @@ template <int N>
 __global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2* __restrict__ fold,
                                                                    const float2* __restrict__ corr, int L,
                                                                    const float2* __restrict__ ptab, int F,
+                                                                   const uint32_t* __restrict__ step,
                                                                    float2* __restrict__ work) {
   constexpr int T = PairCfg<N>::T, M = PairCfg<N>::M, NB = N / 8;
   static_assert(T == M * NB, "one row position per thread in the first stage");
@@ -301,7 +383,8 @@ __global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2*
   const float4* fp = (const float4*)(fold + (size_t)k0 * N) + jq;  // chunk h at float4 h * NB + jq
   // z[r] accumulates the packed sum Herm X_t1 + i Herm X_t2 directly: per layer
   // and element, Q = P_1 + i P_2 (two phasor products folded into 8 FMAs),
-  // then one complex multiply-accumulate z += g Q (4 FMAs).
+  // then one complex multiply-accumulate z += g Q (4 FMAs).  With a uniform
+  // step the fold carries (1 + i D), and Q = P_1 (4 FMAs).
   float2 z[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) z[r] = make_float2(0.f, 0.f);
@@ -309,40 +392,51 @@ __global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2*
   // times t1 is one product per layer and frame.
   constexpr bool BSAME = NB % 8 == 0;
   const int b0 = (jq + N / 2) & 7;
-  for (int l = 0; l < L; ++l) {
-    const float2 pa = px[l * M + m], pb = px[(L + l) * M + m];
-    const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
-    const float2 cb0 = two ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
-    float2 g[8];
+  auto layer_sum = [&](auto uni_tag) {
+    constexpr bool UNI = decltype(uni_tag)::value;
+    const bool two2 = !UNI && two;
+    for (int l = 0; l < L; ++l) {
+      const float2 pa = px[l * M + m], pb = px[(L + l) * M + m];
+      const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
+      const float2 cb0 = two2 ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
+      float2 g[8];
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * NB];
-      g[2 * h] = make_float2(v.x, v.y);
-      g[2 * h + 1] = make_float2(v.z, v.w);
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int qq = (jq + r * NB + N / 2) & (N - 1);
-      float2 ca = ca0, cb = cb0;
-      if (!BSAME) {
-        ca = cmul(pa, t1[l * 8 + (qq & 7)]);
-        cb = two ? cmul(pb, t1[(L + l) * 8 + (qq & 7)]) : make_float2(0.f, 0.f);
+      for (int h = 0; h < 4; ++h) {
+        const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * NB];
+        g[2 * h] = make_float2(v.x, v.y);
+        g[2 * h + 1] = make_float2(v.z, v.w);
       }
-      const float2 ta = t8[l * NB + (qq >> 3)], tb = t8[(L + l) * NB + (qq >> 3)];
-      float qx = ca.x * ta.x;
-      qx = fmaf(-ca.y, ta.y, qx);
-      qx = fmaf(-cb.x, tb.y, qx);
-      qx = fmaf(-cb.y, tb.x, qx);
-      float qy = ca.x * ta.y;
-      qy = fmaf(ca.y, ta.x, qy);
-      qy = fmaf(cb.x, tb.x, qy);
-      qy = fmaf(-cb.y, tb.y, qy);
-      z[r].x = fmaf(g[r].x, qx, z[r].x);
-      z[r].x = fmaf(-g[r].y, qy, z[r].x);
-      z[r].y = fmaf(g[r].x, qy, z[r].y);
-      z[r].y = fmaf(g[r].y, qx, z[r].y);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int qq = (jq + r * NB + N / 2) & (N - 1);
+        float2 ca = ca0, cb = cb0;
+        if (!BSAME) {
+          ca = cmul(pa, t1[l * 8 + (qq & 7)]);
+          cb = two2 ? cmul(pb, t1[(L + l) * 8 + (qq & 7)]) : make_float2(0.f, 0.f);
+        }
+        const float2 ta = t8[l * NB + (qq >> 3)];
+        float qx = ca.x * ta.x;
+        qx = fmaf(-ca.y, ta.y, qx);
+        float qy = ca.x * ta.y;
+        qy = fmaf(ca.y, ta.x, qy);
+        if (!UNI) {
+          const float2 tb = t8[(L + l) * NB + (qq >> 3)];
+          qx = fmaf(-cb.x, tb.y, qx);
+          qx = fmaf(-cb.y, tb.x, qx);
+          qy = fmaf(cb.x, tb.x, qy);
+          qy = fmaf(-cb.y, tb.y, qy);
+        }
+        z[r].x = fmaf(g[r].x, qx, z[r].x);
+        z[r].x = fmaf(-g[r].y, qy, z[r].x);
+        z[r].y = fmaf(g[r].x, qy, z[r].y);
+        z[r].y = fmaf(g[r].y, qx, z[r].y);
+      }
     }
-  }
+  };
+  if (step_uniform(step))
+    layer_sum(std::true_type{});
+  else
+    layer_sum(std::false_type{});
   // Nyquist row / column terms of this pair (see k_frame_pair_tables)
   const float2* cp = corr + (size_t)pr * 2 * N;
   if (k0 == N / 2) {
@@ -378,6 +472,9 @@ template <int N>
 struct PairColsCfg {
   static constexpr int MT = N <= 512 ? 16 : 8192 / N;  // columns per tile (<= 1024 threads)
   static constexpr int T = MT * N / 8;
+  // waves per SIMD the register budget must allow: two 1024-thread tiles
+  // per CU (64 VGPRs); n = 32 (one wave, 8 VGPR-heavy rows) is left free
+  static constexpr int W = N >= 64 ? 8 : 1;
 };
 
 // Folded stack for pass A' (same values as k_frame_fold): tile t = k1 / mt
@@ -385,6 +482,7 @@ struct PairColsCfg {
 // values r = 2h, 2h + 1 as float4 number ((h * (n/mt) + t) * n/8 + j) * mt + m
 // of the layer, so the lanes of one load read one contiguous run.
 __global__ __launch_bounds__(256) void k_frame_fold_tiles(const float2* __restrict__ spectra, int n, int mt,
+                                                          const uint32_t* __restrict__ step,
                                                           float2* __restrict__ fold, float2* __restrict__ side) {
   const size_t plane = (size_t)n * n;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -400,10 +498,10 @@ __global__ __launch_bounds__(256) void k_frame_fold_tiles(const float2* __restri
   const size_t dst =
       (size_t)l * plane + ((((size_t)(r >> 1) * (n / mt) + t) * nb + j) * mt + m) * 2 + (r & 1);
   if (k0 == h || k1 == h) {
-    fold[dst] = A;
+    fold[dst] = fold_step(A, step, l, k0, k1, n);
     side[(size_t)l * 2 * n + (k0 == h ? k1 : n + k0)] = B;
   } else {
-    fold[dst] = make_float2(A.x + B.x, A.y + B.y);
+    fold[dst] = fold_step(make_float2(A.x + B.x, A.y + B.y), step, l, k0, k1, n);
   }
 }
 
@@ -425,9 +523,10 @@ __host__ __device__ constexpr size_t pair_cols_lds(int L) {
 // k_frame_pair_tables; then the column transform, written as MT-column
 // runs of the c64 intermediate.
 template <int N>
-__global__ __launch_bounds__(PairColsCfg<N>::T) void k_frame_pair_cols(const float2* __restrict__ fold,
+__global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(PairColsCfg<N>::W))) void k_frame_pair_cols(const float2* __restrict__ fold,
                                                                       const float2* __restrict__ corr, int L,
                                                                       const float2* __restrict__ ptab, int F,
+                                                                      const uint32_t* __restrict__ step,
                                                                       float2* __restrict__ work) {
   constexpr int T = PairColsCfg<N>::T, MT = PairColsCfg<N>::MT, NB = N / 8;
   static_assert(T == MT * NB, "one column position per thread in the first stage");
@@ -471,41 +570,53 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) void k_frame_pair_cols(const flo
   // it alternates between two values.
   constexpr bool BSAME = NB % 8 == 0;
   const int b0 = (j + N / 2) & 7;
-  for (int l = 0; l < L; ++l) {
-    const float2 pa = py[l * MT + m], pb = py[(L + l) * MT + m];
-    const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
-    const float2 cb0 = two ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
-    float2 g[8];
+  // UNI: uniform step, the fold carries (1 + i D) and Q = P_1
+  auto layer_sum = [&](auto uni_tag) {
+    constexpr bool UNI = decltype(uni_tag)::value;
+    const bool two2 = !UNI && two;
+    for (int l = 0; l < L; ++l) {
+      const float2 pa = py[l * MT + m], pb = py[(L + l) * MT + m];
+      const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
+      const float2 cb0 = two2 ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
+      float2 g[8];
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * N * NB];
-      g[2 * h] = make_float2(v.x, v.y);
-      g[2 * h + 1] = make_float2(v.z, v.w);
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int qq = (j + r * NB + N / 2) & (N - 1);
-      float2 ca = ca0, cb = cb0;
-      if (!BSAME) {
-        ca = cmul(pa, t1[l * 8 + (qq & 7)]);
-        cb = two ? cmul(pb, t1[(L + l) * 8 + (qq & 7)]) : make_float2(0.f, 0.f);
+      for (int h = 0; h < 4; ++h) {
+        const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * N * NB];
+        g[2 * h] = make_float2(v.x, v.y);
+        g[2 * h + 1] = make_float2(v.z, v.w);
       }
-      const int a = qq >> 3;
-      const float2 ta = t8[l * NB + a], tb = t8[(L + l) * NB + a];
-      float qx = ca.x * ta.x;
-      qx = fmaf(-ca.y, ta.y, qx);
-      qx = fmaf(-cb.x, tb.y, qx);
-      qx = fmaf(-cb.y, tb.x, qx);
-      float qy = ca.x * ta.y;
-      qy = fmaf(ca.y, ta.x, qy);
-      qy = fmaf(cb.x, tb.x, qy);
-      qy = fmaf(-cb.y, tb.y, qy);
-      z[r].x = fmaf(g[r].x, qx, z[r].x);
-      z[r].x = fmaf(-g[r].y, qy, z[r].x);
-      z[r].y = fmaf(g[r].x, qy, z[r].y);
-      z[r].y = fmaf(g[r].y, qx, z[r].y);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int qq = (j + r * NB + N / 2) & (N - 1);
+        float2 ca = ca0, cb = cb0;
+        if (!BSAME) {
+          ca = cmul(pa, t1[l * 8 + (qq & 7)]);
+          cb = two2 ? cmul(pb, t1[(L + l) * 8 + (qq & 7)]) : make_float2(0.f, 0.f);
+        }
+        const int a = qq >> 3;
+        const float2 ta = t8[l * NB + a];
+        float qx = ca.x * ta.x;
+        qx = fmaf(-ca.y, ta.y, qx);
+        float qy = ca.x * ta.y;
+        qy = fmaf(ca.y, ta.x, qy);
+        if (!UNI) {
+          const float2 tb = t8[(L + l) * NB + a];
+          qx = fmaf(-cb.x, tb.y, qx);
+          qx = fmaf(-cb.y, tb.x, qx);
+          qy = fmaf(cb.x, tb.x, qy);
+          qy = fmaf(-cb.y, tb.y, qy);
+        }
+        z[r].x = fmaf(g[r].x, qx, z[r].x);
+        z[r].x = fmaf(-g[r].y, qy, z[r].x);
+        z[r].y = fmaf(g[r].x, qy, z[r].y);
+        z[r].y = fmaf(g[r].y, qx, z[r].y);
+      }
     }
-  }
+  };
+  if (step_uniform(step))
+    layer_sum(std::true_type{});
+  else
+    layer_sum(std::false_type{});
   // Nyquist row / column terms of this pair: corr[k1] on the Nyquist row
   // (k0 = N/2, every k1), corr[N + k0] on the Nyquist column (k1 = N/2,
   // k0 != N/2)
@@ -684,12 +795,11 @@ static int generate_pow2(const float* nre, ptrdiff_t im_off, size_t gstride, int
   return gen_cols_pow2<N>(a, st);
 }
 
-constexpr int FOLD_LMAX = 16;  // layers of the fused (paired) frame path
 
 // Workspace of the fused frame path (c64 elements): the paired intermediate,
 // the folded stack, its Nyquist side table, then the subharmonic tables.
 struct FramesLayout {
-  size_t fold, side, ptab, corr, tab, total;
+  size_t fold, side, ptab, corr, step, tab, total;
 };
 static inline size_t r32(size_t v) { return (v + 31) & ~(size_t)31; }
 static FramesLayout frames_layout(int n, int F, int L, int P) {
@@ -699,7 +809,8 @@ static FramesLayout frames_layout(int n, int F, int L, int P) {
   w.side = w.fold + r32((size_t)L * plane);
   w.ptab = w.side + r32((size_t)L * 2 * n);
   w.corr = w.ptab + r32((size_t)((F + 1) / 2) * 2 * L * (n / 8 + 9));
-  w.tab = w.corr + r32((size_t)((F + 1) / 2) * 2 * n);
+  w.step = w.corr + r32((size_t)((F + 1) / 2) * 2 * n);
+  w.tab = w.step + r32((size_t)(STEP_WORDS + 1) / 2);  // uint32 words of k_frame_step
   w.total = w.tab + sub_table_c64(P, n, F);
   return w;
 }
@@ -722,15 +833,23 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   float2* side = work + w.side;
   float2* ptab = work + w.ptab;
   float2* corr = work + w.corr;
+  // uniform-step fold only where one call folds and pairs (the step is read
+  // from this call's displacements)
+  uint32_t* step = stages == STAGE_ALL ? (uint32_t*)(work + w.step) : nullptr;
+  if (step) {
+    k_frame_step<<<STEP_BLOCKS, 1024, 0, st>>>(disp, L, F, inv_period, step);
+    PT_LAUNCH_CHECK();
+  }
   if (stages & STAGE_FOLD) {
-    k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, fold, side);
+    k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, step, fold, side);
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
   k_frame_pair_tables<N><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(disp, side, L, F,
-                                                                                     inv_period, ptab, corr);
+                                                                                     inv_period, step, ptab, corr);
   PT_LAUNCH_CHECK();
-  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), PairCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, work);
+  k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), PairCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, step,
+                                                                                 work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -755,15 +874,21 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
   float2* side = work + w.side;
   float2* ptab = work + w.ptab;
   float2* corr = work + w.corr;
+  uint32_t* step = stages == STAGE_ALL ? (uint32_t*)(work + w.step) : nullptr;  // as in frames_rows_pow2
+  if (step) {
+    k_frame_step<<<STEP_BLOCKS, 1024, 0, st>>>(disp, L, F, inv_period, step);
+    PT_LAUNCH_CHECK();
+  }
   if (stages & STAGE_FOLD) {
-    k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, fold, side);
+    k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, step, fold, side);
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
   k_frame_pair_tables<N, true><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
-      disp, side, L, F, inv_period, ptab, corr);
+      disp, side, L, F, inv_period, step, ptab, corr);
   PT_LAUNCH_CHECK();
-  k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(fold, corr, L, ptab, F, work);
+  k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(fold, corr, L, ptab, F, step,
+                                                                                  work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }

@@ -92,13 +92,15 @@ def test_captured_mixed_ensemble_replays_match_eager(hip):
 def test_folded_frames_equal_unfolded(hip, order):
     """spectral_fold once, then spectral_frames(folded=True) for two calls
     with different displacements: bit-identical to the unfolded call, for
-    both transform orders."""
+    both transform orders.  Random displacements keep the unfolded call on
+    the plain fold; with equally spaced displacements the unfolded call folds
+    the uniform step in and the two agree to 1e-6."""
     import torch
 
     from pyturb_amd import ops, tune
 
     print(f"folded frames order {order}", flush=True)
-    n, L, F, P = 128, 5, 3, 4
+    n, L, F, P = 128, 5, 5, 4
     rng = np.random.default_rng(40 + order)
     dev = torch.device("cuda", 0)
     spec = (rng.standard_normal((L, n, n)) + 1j * rng.standard_normal((L, n, n))) / n
@@ -114,5 +116,11 @@ def test_folded_frames_equal_unfolded(hip, order):
             got = ops.spectral_frames(spectra, disp, cf, 0.05, work=work, folded=True).clone()
             ref = ops.spectral_frames(spectra, disp, cf, 0.05)
             assert torch.equal(got, ref), k
+        ramp = torch.arange(F, dtype=torch.float64, device=dev)[:, None, None]
+        disp = disp[:1] + 0.013 * ramp * torch.tensor(rng.uniform(-1.0, 1.0, (1, L, 2)), device=dev)
+        got = ops.spectral_frames(spectra, disp, cf, 0.05, work=work, folded=True).clone()
+        ref = ops.spectral_frames(spectra, disp, cf, 0.05)
+        err = ((got - ref).square().mean() / ref.square().mean()).sqrt().item()
+        assert err < 1e-6, err
     finally:
         tune.reset()

@@ -169,6 +169,47 @@ def test_spectral_frames_white_spectra_all_modes(hip, n, frames):
         assert O.rel_rms(got[t], ref) < REL, (t, O.rel_rms(got[t], ref))
 
 
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("n,frames", [(64, 9), (256, 8)])
+def test_uniform_step_frames_match_oracle(hip, order, n, frames):
+    """Equally spaced times move every frame pair by one step per layer: the
+    fold then carries the step's phasor and pass A forms one phasor product
+    per layer and element.  The same times with one frame moved off the grid
+    take the two-product path.  Both against the oracle frame by frame, on
+    flat spectra (full weight on the Nyquist lines), for both transform
+    orders, and the two paths agree to 1e-6 on the frames they share."""
+    import torch
+    from pyturb_amd import ops, tune
+
+    print(f"uniform step n={n} frames={frames} order={order}", flush=True)
+    rng = np.random.default_rng(7 * n + frames)
+    L, dx, dt = 3, 0.05, 1e-3
+    spectra = (rng.standard_normal((L, n, n)) + 1j * rng.standard_normal((L, n, n))).astype(np.complex64)
+    wind = rng.uniform(-40.0, 40.0, (L, 2))
+    base = rng.uniform(-5.0, 5.0, (L, 2))
+    times = 3.7 + dt * np.arange(frames)
+    disp = base[None] + times[:, None, None] * wind[None]
+    off = disp.copy()
+    off[frames - 3] += 0.37 * dx  # breaks the uniform step of one pair
+    f = np.fft.fftfreq(n, d=dx)
+    sp64 = spectra.astype(np.complex128)
+    dev = torch.device("cuda")
+    sp = torch.from_numpy(spectra).to(dev)
+    try:
+        tune.set_variant("frames_order", n, order)
+        got = {k: _np(ops.spectral_frames(sp, torch.from_numpy(d).to(dev), None, dx))
+               for k, d in (("uniform", disp), ("off", off))}
+    finally:
+        tune.reset()
+    for k, d in (("uniform", disp), ("off", off)):
+        for t in range(frames):
+            ref = O.integrate(sp64, np.zeros((0, L, 3, 3), complex), f, None, None, d[t, :, 0], d[t, :, 1])
+            assert O.rel_rms(got[k][t], ref) < REL, (k, t, O.rel_rms(got[k][t], ref))
+    for t in range(frames):
+        if t != frames - 3:
+            assert O.rel_rms(got["uniform"][t], got["off"][t]) < 1e-6, t
+
+
 def test_spectral_frame_matches_reference_golden(hip):
     g = load_golden("atmosphere_n64_seed2.npz")
     atm = _frames_case(64, None)

@@ -63,11 +63,12 @@ def test_library_loads_and_exports_every_header_symbol():
     assert _hip.lib().pt_abi_version() == _hip.ABI_VERSION == 4
     assert "pt_work_elems" in syms
     # host-only sizing: power-of-two fused path vs general path
-    # frames: pairs of frames share one transform, plus the folded stack and its
-    # Nyquist side table (each section rounded to 32 elements)
+    # frames: pairs of frames share one transform, plus the folded stack, its
+    # Nyquist side table, the pair tables and Nyquist sums, and the uniform-step
+    # words (1 + 2 * 16 uint32; each section rounded to 32 elements)
     r32 = lambda v: (v + 31) // 32 * 32
     assert _hip.lib().pt_frames_work_elems(512, 10, 9, 0) == (
-        5 * 512 * 512 + 9 * 512 * 512 + 9 * 1024 + r32(5 * 2 * 9 * (512 // 8 + 9)) + 5 * 1024)
+        5 * 512 * 512 + 9 * 512 * 512 + 9 * 1024 + r32(5 * 2 * 9 * (512 // 8 + 9)) + 5 * 1024 + 32)
     assert _hip.lib().pt_frames_work_elems(100, 10, 9, 0) == 2 * 10 * 100 * 100
     assert _hip.lib().pt_frames_work_elems(512, 10, 17, 0) == 2 * 10 * 512 * 512
     for L in (1, 9, 16, 40):
