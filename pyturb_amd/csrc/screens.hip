@@ -657,10 +657,15 @@ template <int N>
 __global__ __launch_bounds__(RowsCfg<N>::T) void k_frame_rows_epi(GenArgs a) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
   static_assert(Plan<N, true>::radix(Plan<N, true>::NST - 1) == 8, "last stage radix 8");
-  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
-  __shared__ float2 tw[N];
   using Sub = ColSub<N, M, 2, T>;
-  __shared__ Sub sub;
+  // the subharmonic staging block reuses the transform tile once the last
+  // stage has read it: a smaller footprint per workgroup (6 instead of 5 per
+  // CU at n = 512) for one more barrier
+  constexpr size_t TILE_B = (size_t)Layout<N, T, M, false>::SIZE * sizeof(float2);
+  __shared__ __align__(16) unsigned char smem[TILE_B > sizeof(Sub) ? TILE_B : sizeof(Sub)];
+  __shared__ float2 tw[N];
+  float2* lds = reinterpret_cast<float2*>(smem);
+  Sub& sub = *reinterpret_cast<Sub*>(smem);
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
   const int r0 = blockIdx.x * M;
@@ -689,7 +694,10 @@ __global__ __launch_bounds__(RowsCfg<N>::T) void k_frame_rows_epi(GenArgs a) {
     ++cnt;
   };
   wg_fft<N, T, M, false, 1, true>(lds, tw, tid, load, store);
-  if (P > 0) sub.commit(pre, P, tid);
+  if (P > 0) {
+    __syncthreads();  // every lane has read its last-stage inputs from the tile
+    sub.commit(pre, P, tid);
+  }
   __syncthreads();
   const int m = tid / (N / 8), y0 = tid % (N / 8);
   const int x = r0 + m - a.off;
