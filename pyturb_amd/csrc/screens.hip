@@ -654,7 +654,9 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
 // column epilogue's ColSub with rows and columns exchanged; then crop and
 // scale, whole rows per wave.
 template <int N>
-__global__ __launch_bounds__(RowsCfg<N>::T) void k_frame_rows_epi(GenArgs a) {
+// Seven 256-thread workgroups per CU at n = 512: 22 KiB of LDS each, and at
+// most 72 VGPRs (73 uncapped, no spills at 70).
+__global__ __launch_bounds__(RowsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(7))) void k_frame_rows_epi(GenArgs a) {
   constexpr int T = RowsCfg<N>::T, M = RowsCfg<N>::M;
   static_assert(Plan<N, true>::radix(Plan<N, true>::NST - 1) == 8, "last stage radix 8");
   using Sub = ColSub<N, M, 2, T>;
