@@ -63,19 +63,33 @@ __device__ __forceinline__ double2 dcmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 dconj(double2 a) { return make_double2(a.x, -a.y); }
 
-// H_p(y) and Re(g_1(y)) for one coefficient block and one basis value eb.
-__device__ __forceinline__ void sh_column(const double2 C[9], float2 eb, double2& H, double& g1re) {
-  double2 e = make_double2(eb.x, eb.y);
-  double2 ec = dconj(e);
-  double2 g[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double2 t0 = dcmul(C[a * 3 + 0], ec);
-    double2 t2 = dcmul(C[a * 3 + 2], e);
-    g[a] = make_double2(t0.x + C[a * 3 + 1].x + t2.x, t0.y + C[a * 3 + 1].y + t2.y);
-  }
-  H = make_double2(g[2].x + g[0].x, g[2].y - g[0].y);
-  g1re = g[1].x;
+// H_p(y) = g_2 + conj(g_0) and Re(g_1(y)) are affine in the basis value
+// e = (c, s): with alpha = C20 + conj(C02), beta = C21 + conj(C01) and
+// gamma = C22 + conj(C00),
+//   H = beta + alpha conj(e) + gamma e,   Re g_1 = Re C11 + Re((conj(C10) + C12) e).
+// sh_column_coeffs folds one block into the nine coefficients
+//   H.x = h0 + h2 c + h4 s,  H.y = h1 + h3 c + h5 s,  Re g_1 = h6 + h7 c + h8 s,
+// once per block, so each column costs six FP64 FMAs.
+__device__ __forceinline__ void sh_column_coeffs(const double2 C[9], double h[9]) {
+  const double2 al = make_double2(C[6].x + C[2].x, C[6].y - C[2].y);
+  const double2 be = make_double2(C[7].x + C[1].x, C[7].y - C[1].y);
+  const double2 ga = make_double2(C[8].x + C[0].x, C[8].y - C[0].y);
+  h[0] = be.x;
+  h[1] = be.y;
+  h[2] = al.x + ga.x;
+  h[3] = al.y + ga.y;
+  h[4] = al.y - ga.y;
+  h[5] = ga.x - al.x;
+  h[6] = C[4].x;
+  h[7] = C[3].x + C[5].x;
+  h[8] = C[3].y - C[5].y;
+}
+
+// H_p(y) and Re(g_1(y)) from a block's coefficients and one basis value eb.
+__device__ __forceinline__ void sh_column(const double* h, float2 eb, double2& H, double& g1re) {
+  const double c = eb.x, s = eb.y;
+  H = make_double2(fma(h[4], s, fma(h[2], c, h[0])), fma(h[5], s, fma(h[3], c, h[1])));
+  g1re = fma(h[8], s, fma(h[7], c, h[6]));
 }
 
 // Re(Mb^T C Mb): the block's contribution to the grid mean.

@@ -279,6 +279,20 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
   const int pr = blockIdx.x;
   const int first = blockIdx.y * 256 + threadIdx.x, stride = gridDim.y * 256;  // entries split over blocks
   const bool uni = step_uniform(step);
+  // fixed-point phases of the pair's frames, once per workgroup:
+  // u[f][l][axis]; frame 2p + 1 of a uniform step is u(2p) + d, and a frame
+  // past the end (odd F, no uniform step) is absent
+  __shared__ uint32_t u[2][FOLD_LMAX][2];
+  if (threadIdx.x < 4 * L) {
+    const int f = threadIdx.x / (2 * L), e = threadIdx.x % (2 * L);
+    const bool virt = uni && f == 1;
+    const int t = 2 * pr + (virt ? 0 : f);
+    uint32_t v = t < F ? cycles_fixed(disp[(size_t)t * 2 * L + e] * inv_period) : 0u;
+    if (virt) v += step[STEP_BLOCKS + e];
+    u[f][e >> 1][e & 1] = v;
+  }
+  __syncthreads();
+  const bool has_b = uni || 2 * pr + 1 < F;
   float2* tp = tab + (size_t)pr * 2 * L * S;
   for (int q = first; q < 2 * N; q += stride) {
     const int k0 = q < N ? N / 2 : q - N, k1 = q < N ? q : N / 2;
@@ -288,17 +302,9 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
         float2 w[2];
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-          const int t = 2 * pr + f;
           w[f] = make_float2(0.f, 0.f);
-          if (t >= F && !uni) continue;
-          const bool virt = uni && f == 1;  // second frame of a uniform step: u(2p) + d
-          const int ts = virt ? t - 1 : t;
-          uint32_t ux = cycles_fixed(disp[((size_t)ts * L + l) * 2 + 0] * inv_period);
-          uint32_t uy = cycles_fixed(disp[((size_t)ts * L + l) * 2 + 1] * inv_period);
-          if (virt) {
-            ux += step[STEP_BLOCKS + 2 * l];
-            uy += step[STEP_BLOCKS + 1 + 2 * l];
-          }
+          if (f == 1 && !has_b) continue;
+          const uint32_t ux = u[f][l][0], uy = u[f][l][1];
           uint32_t ph = (uint32_t)signed_freq(k0, N) * ux + (uint32_t)signed_freq(k1, N) * uy;
           if (k0 == N / 2) ph += (uint32_t)N * ux;  // E: exp(2 pi i s_0 / dx)
           if (k1 == N / 2) ph += (uint32_t)N * uy;  // E: exp(2 pi i s_1 / dx)
@@ -314,12 +320,8 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
   }
   for (int q = first; q < 2 * L * S; q += stride) {
     const int fl = q / S, e = q % S;
-    const int t = 2 * pr + fl / L, l = fl % L;
-    uint32_t ux = 0u, uy = 0u;
-    if (t < F) {
-      ux = cycles_fixed(disp[((size_t)t * L + l) * 2 + 0] * inv_period);
-      uy = cycles_fixed(disp[((size_t)t * L + l) * 2 + 1] * inv_period);
-    }
+    const int fb = fl / L, l = fl % L;
+    const uint32_t ux = u[fb][l][0], uy = u[fb][l][1];
     const uint32_t ut = TR ? ux : uy, uo = TR ? uy : ux;  // table axis, per-thread axis
     float2 v;
     if (e < NB)

@@ -60,7 +60,11 @@ __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTa
                                     double* s_mean, int tid, int nthreads, int part = 0, int parts = 1) {
   __shared__ double terms[PMAX];
   __shared__ double g1s[PMAX * SUB_CC];
-  for (int p = tid; p < P; p += nthreads) terms[p] = sh_block_mean(C + p * 9, sh_mean_e(p, n));
+  __shared__ double hc[PMAX * 9];  // sh_column_coeffs per level
+  for (int p = tid; p < P; p += nthreads) {
+    terms[p] = sh_block_mean(C + p * 9, sh_mean_e(p, n));
+    sh_column_coeffs(C + p * 9, hc + p * 9);
+  }
   if (writeU)
     for (int idx = part * nthreads + tid; idx < P * n; idx += parts * nthreads)
       t.U[idx] = sh_basis_e(idx / n, idx % n, n);
@@ -79,7 +83,7 @@ __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTa
       const int p = w / nc, j = w % nc, y = c0 + j;
       double2 Hd;
       double g1;
-      sh_column(C + p * 9, sh_basis_e(p, y, n), Hd, g1);
+      sh_column(hc + p * 9, sh_basis_e(p, y, n), Hd, g1);
       t.H[((size_t)s * P + p) * n + y] = make_float2((float)Hd.x, (float)Hd.y);
       g1s[p * SUB_CC + j] = g1;
     }
