@@ -27,13 +27,24 @@ namespace pt {
 
 constexpr int LT = 32;     // output tile edge
 constexpr int LBOX = 40;   // LDS window edge
-constexpr int LPW = (LBOX * LBOX + 255) / 256;  // window values per thread
+constexpr int LNG = 256 / LBOX;                  // row groups of the window lanes (6; lanes 240.. idle)
+constexpr int LWR = (LBOX + LNG - 1) / LNG;      // window rows per lane (7)
+constexpr int LRR = (LT + LNG - 1) / LNG;        // row-pass rows per lane (6)
 
 struct ZoomWin {
   int r_lo, c_lo, nr, nc;
   bool fits;
 };
 
+struct alignas(8) RowTap {
+  float w;  // weight
+  int off;  // float offset of the source row in the window box
+};
+
+// Lanes of the window load, the commit and the row pass own one window
+// column cw = tid % LBOX and the rows gw + LNG k (gw = tid / LBOX), so no
+// per-value index division remains; a row tap is stored as (weight, LDS
+// offset of its source row) and read as one 8-byte value.
 template <int T>
 __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ screens, int L, int F, int ns,
                                                   const int* __restrict__ idx, const float* __restrict__ w, int n,
@@ -41,16 +52,17 @@ __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ scre
   static_assert(T * LT <= 256, "one row tap per thread");
   __shared__ float box[2][LBOX][LBOX + 1];
   __shared__ float rows[LT][LBOX + 1];
-  __shared__ float rw[2][T][LT];  // row tap weights of the tile's rows
-  __shared__ int rix[2][T][LT];   // row tap indices, relative to the window's first row
+  __shared__ RowTap rt[2][T][LT];  // row taps of the tile's rows
   const int tid = threadIdx.x;
   const int i0 = blockIdx.y * LT, j0 = blockIdx.x * LT, f = blockIdx.z;
   const int i1 = min(i0 + LT, n) - 1, j1 = min(j0 + LT, n) - 1;
   const int jl = tid & 31, il = tid >> 5;  // pixel (i0 + il + 8k, j0 + jl), k < 4
+  const int cw = tid % LBOX, gw = tid / LBOX;
+  const bool wl_lane = gw < LNG;
   const size_t plane = (size_t)ns * ns;
   const int ta = tid / LT, ti = tid % LT;  // this thread's row tap (tid < T * LT)
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  float pre[LPW];
+  float pre[LWR];
   float prw = 0.f;
   int pri = 0;
   // window of layer l: idx is non-decreasing in the pixel index and in the tap
@@ -66,29 +78,30 @@ __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ scre
   };
   auto fetch = [&](int l, const ZoomWin& g) {
     if (!g.fits) return;
-    const float* S = screens + ((size_t)l * F + f) * plane;
+    if (wl_lane && cw < g.nc) {
+      const float* S = screens + ((size_t)l * F + f) * plane + (size_t)g.r_lo * ns + g.c_lo + cw;
 #pragma unroll
-    for (int k = 0; k < LPW; ++k) {
-      const int q = tid + 256 * k, r = q / LBOX, c = q % LBOX;
-      if (r < g.nr && c < g.nc) pre[k] = S[(size_t)(g.r_lo + r) * ns + g.c_lo + c];
+      for (int k = 0; k < LWR; ++k) {
+        const int r = gw + LNG * k;
+        if (r < g.nr) pre[k] = S[(size_t)r * ns];
+      }
     }
     if (tid < T * LT) {
       const size_t e = ((size_t)l * T + ta) * n + min(i0 + ti, n - 1);
       prw = w[e];
-      pri = idx[e] - g.r_lo;
+      pri = (idx[e] - g.r_lo) * (LBOX + 1);
     }
   };
   auto commit = [&](int sel, const ZoomWin& g) {
     if (!g.fits) return;
+    if (wl_lane && cw < g.nc) {
 #pragma unroll
-    for (int k = 0; k < LPW; ++k) {
-      const int q = tid + 256 * k, r = q / LBOX, c = q % LBOX;
-      if (r < g.nr && c < g.nc) box[sel][r][c] = pre[k];
+      for (int k = 0; k < LWR; ++k) {
+        const int r = gw + LNG * k;
+        if (r < g.nr) box[sel][r][cw] = pre[k];
+      }
     }
-    if (tid < T * LT) {
-      rw[sel][ta][ti] = prw;
-      rix[sel][ta][ti] = pri;
-    }
+    if (tid < T * LT) rt[sel][ta][ti] = RowTap{prw, pri};
   };
   ZoomWin cur = window(0);
   fetch(0, cur);
@@ -113,28 +126,32 @@ __global__ __launch_bounds__(256) void k_lgs_zoom(const float* __restrict__ scre
       wb[b] = wl[(size_t)b * n + j];
       cb[b] = ix[(size_t)b * n + j];
     }
-    if (cur.fits) {
-      // rows: rows[i][c] = sum_a w_a(i) box[idx_a(i) - r_lo][c], over the
-      // fixed LBOX-wide rows (a constant divisor; columns >= nc skipped)
-      const int nc = cur.nc;
-#pragma unroll 1
-      for (int q = tid; q < LT * LBOX; q += 256) {
-        const int i = q / LBOX, c = q % LBOX;
-        if (c < nc) {
+    if (cur.fits && wl_lane && cw < cur.nc) {
+      // rows: rows[i][c] = sum_a w_a(i) box[idx_a(i) - r_lo][c]
+      const float* bx = &box[sel][0][0] + cw;
+#pragma unroll
+      for (int k = 0; k < LRR; ++k) {
+        const int i = gw + LNG * k;
+        if (i < LT) {
           float v = 0.f;
 #pragma unroll
-          for (int a = 0; a < T; ++a) v = fmaf(rw[sel][a][i], box[sel][rix[sel][a][i]][c], v);
-          rows[i][c] = v;
+          for (int a = 0; a < T; ++a) {
+            const RowTap tp = rt[sel][a][i];
+            v = fmaf(tp.w, bx[tp.off], v);
+          }
+          rows[i][cw] = v;
         }
       }
     }
     __syncthreads();  // rows complete
     if (cur.fits) {
 #pragma unroll
+      for (int b = 0; b < T; ++b) cb[b] -= cur.c_lo;
+#pragma unroll
       for (int k = 0; k < 4; ++k) {
         float v = 0.f;
 #pragma unroll
-        for (int b = 0; b < T; ++b) v = fmaf(wb[b], rows[il + 8 * k][cb[b] - cur.c_lo], v);
+        for (int b = 0; b < T; ++b) v = fmaf(wb[b], rows[il + 8 * k][cb[b]], v);
         acc[k] += v;
       }
     } else {  // magnification > 1: not unrolled, so it does not set the register budget
