@@ -50,16 +50,14 @@ __host__ __device__ inline SubTables sub_tables(float2* base, int P, int n, int 
 // columns; `writeU` also writes the shared basis table.  256-thread blocks.
 // With `parts` > 1, block `part` of the set fills one contiguous share of
 // the columns (and a strided share of the basis entries).  The grid-mean
-// terms are computed one level per thread, and the columns in chunks of
-// SUB_CC with one thread per (level, column): the FP64 work is spread over
-// the block instead of running as per-thread level loops.  K sums its level
-// terms in ascending order, as a level loop would.
-constexpr int SUB_CC = 32;  // columns per chunk
-
+// terms and the per-level column coefficients are computed one level per
+// thread; then each thread owns whole columns and runs the level loop, so
+// no barrier follows the coefficients (a chunked (level, column) split needed
+// two per 32 columns, and a 2000-frame call was bound by that serial chain).
+// K sums its level terms in ascending order.
 __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTables t, bool writeU,
                                     double* s_mean, int tid, int nthreads, int part = 0, int parts = 1) {
   __shared__ double terms[PMAX];
-  __shared__ double g1s[PMAX * SUB_CC];
   __shared__ double hc[PMAX * 9];  // sh_column_coeffs per level
   for (int p = tid; p < P; p += nthreads) {
     terms[p] = sh_block_mean(C + p * 9, sh_mean_e(p, n));
@@ -77,23 +75,17 @@ __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTa
   __syncthreads();
   const double mean = *s_mean;
   const int cpp = (n + parts - 1) / parts, y0 = part * cpp, y1 = min(n, y0 + cpp);
-  for (int c0 = y0; c0 < y1; c0 += SUB_CC) {
-    const int nc = min(SUB_CC, y1 - c0);
-    for (int w = tid; w < P * nc; w += nthreads) {
-      const int p = w / nc, j = w % nc, y = c0 + j;
+  float2* H = t.H + (size_t)s * P * n;
+  for (int y = y0 + tid; y < y1; y += nthreads) {
+    double kacc = -mean;
+    for (int p = 0; p < P; ++p) {
       double2 Hd;
       double g1;
       sh_column(hc + p * 9, sh_basis_e(p, y, n), Hd, g1);
-      t.H[((size_t)s * P + p) * n + y] = make_float2((float)Hd.x, (float)Hd.y);
-      g1s[p * SUB_CC + j] = g1;
+      H[(size_t)p * n + y] = make_float2((float)Hd.x, (float)Hd.y);
+      kacc += g1;
     }
-    __syncthreads();
-    for (int j = tid; j < nc; j += nthreads) {
-      double kacc = -mean;
-      for (int p = 0; p < P; ++p) kacc += g1s[p * SUB_CC + j];
-      t.K[(size_t)s * n + c0 + j] = (float)kacc;
-    }
-    __syncthreads();  // g1s reused by the next chunk
+    t.K[(size_t)s * n + y] = (float)kacc;
   }
 }
 
