@@ -74,7 +74,13 @@ class CapturedFrames:
         self.scale = atm._scale(wavelength)
         F = self.frames * self.D
         dev = atm._dev
+        # disp[t, d, l] = v_l * t + h_l * theta_d, in the order opd_batch uses:
+        # the per-call host work is one multiply and one add into the pinned table
+        self._vel = np.stack([atm._vx, atm._vy], axis=-1)  # (L, 2)
+        self._base = np.stack([atm._alt[None, :] * self.ox[:, None],
+                               atm._alt[None, :] * self.oy[:, None]], axis=-1)  # (D, L, 2)
         self._disp_host = t.empty((F, L, 2), dtype=t.float64, pin_memory=True)
+        self._disp_view = self._disp_host.numpy().reshape(self.frames, self.D, L, 2)
         self.disp = t.zeros((F, L, 2), dtype=t.float64, device=dev)
         self.out = t.empty((F, atm.n, atm.n), dtype=t.float32, device=dev)
         P = 0 if atm._sh_coeffs is None else int(atm._sh_coeffs.shape[0])
@@ -100,13 +106,12 @@ class CapturedFrames:
         times = np.asarray(times, dtype=np.float64).ravel()
         if times.size != self.frames:
             raise ValueError(f"expected {self.frames} times, got {times.size}")
-        dx = a._vx[None, None, :] * times[:, None, None] + a._alt * self.ox[None, :, None]
-        dy = a._vy[None, None, :] * times[:, None, None] + a._alt * self.oy[None, :, None]
-        if not a._wrap_warned:
-            a._check_wrap(dx, dy)
-        disp = np.stack([dx, dy], axis=-1).reshape(-1, len(a._layers), 2)
         self._copied.synchronize()
-        self._disp_host.numpy()[...] = disp
+        v = self._disp_view
+        np.multiply(times[:, None, None, None], self._vel[None, None], out=v)
+        np.add(v, self._base[None], out=v)
+        if not a._wrap_warned:
+            a._check_wrap(v[..., 0], v[..., 1])
         self.disp.copy_(self._disp_host, non_blocking=True)
         self._copied.record()
         self._cap.replay()
