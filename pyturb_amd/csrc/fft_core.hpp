@@ -34,6 +34,18 @@ namespace pt {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Streaming stores for data that no kernel of the same pass reads again (a
+// pass's output, re-read by the next pass from HBM anyway): non-temporal, so
+// they do not evict the lines the pass itself re-reads from L2.
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  v2f w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, (v2f*)p);
+}
+__device__ __forceinline__ void store_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

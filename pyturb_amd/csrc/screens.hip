@@ -457,7 +457,7 @@ __global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2*
   int cnt = 0;
   auto load = [&](int, int) { return z[cnt++]; };
   float2* out = work + (size_t)pr * plane + (size_t)row0 * N;
-  auto store = [&](int mm, int i, float2 v) { out[(size_t)mm * N + i] = v; };
+  auto store = [&](int mm, int i, float2 v) { store_nt(out + (size_t)mm * N + i, v); };
   wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
 }
 
@@ -643,7 +643,7 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
   int cnt = 0;
   auto load = [&](int, int) { return z[cnt++]; };
   float2* out = work + (size_t)pr * plane + c0;
-  auto store = [&](int mm, int i, float2 v) { out[(size_t)i * N + mm] = v; };
+  auto store = [&](int mm, int i, float2 v) { store_nt(out + (size_t)i * N + mm, v); };
   wg_fft<N, T, MT, true, 1>(lds, tw, tid, load, store);
 }
 
@@ -714,11 +714,11 @@ __global__ __launch_bounds__(RowsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(7
   if (a.off == 0 && a.n_out == N) {  // whole grid (uniform): constant offsets, no tests
     float* p0 = o0 + y0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) p0[c * (N / 8)] = v0[c] * f;
+    for (int c = 0; c < 8; ++c) store_nt(p0 + c * (N / 8), v0[c] * f);
     if (has1) {
       float* p1 = o1 + y0;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) p1[c * (N / 8)] = v1[c] * f;
+      for (int c = 0; c < 8; ++c) store_nt(p1 + c * (N / 8), v1[c] * f);
     }
     return;
   }
@@ -726,8 +726,8 @@ __global__ __launch_bounds__(RowsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(7
   for (int c = 0; c < 8; ++c) {
     const int y = y0 + c * (N / 8) - a.off;
     if ((unsigned)y >= (unsigned)a.n_out) continue;
-    o0[y] = v0[c] * f;
-    if (has1) o1[y] = v1[c] * f;
+    store_nt(o0 + y, v0[c] * f);
+    if (has1) store_nt(o1 + y, v1[c] * f);
   }
 }
 
