@@ -75,7 +75,7 @@ __global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* _
     }
     return acc;
   };
-  auto store = [&](int m, int i, float2 v) { out[(size_t)m * N + i] = v; };
+  auto store = [&](int m, int i, float2 v) { store_nt(out + (size_t)m * N + i, v); };
   wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
 }
 
