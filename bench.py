@@ -376,8 +376,9 @@ def parity(atm, out, frames=(0, 1000, 1999)):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    # the engine clock settles over the first ~10 calls of 2000 frames
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte-Carlo ensemble")
