@@ -65,7 +65,7 @@ def test_library_loads_and_exports_every_header_symbol():
     # host-only sizing: power-of-two fused path vs general path
     # frames: pairs of frames share one transform, plus the folded stack, its
     # Nyquist side table, the pair tables and Nyquist sums, and the uniform-step
-    # words (1 + 2 * 16 uint32; each section rounded to 32 elements)
+    # words (16 check flags + 2 * 16 steps, uint32; each section rounded to 32 elements)
     r32 = lambda v: (v + 31) // 32 * 32
     assert _hip.lib().pt_frames_work_elems(512, 10, 9, 0) == (
         5 * 512 * 512 + 9 * 512 * 512 + 9 * 1024 + r32(5 * 2 * 9 * (512 // 8 + 9)) + 5 * 1024 + 32)
