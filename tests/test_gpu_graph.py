@@ -124,3 +124,43 @@ def test_folded_frames_equal_unfolded(hip, order):
         assert err < 1e-6, err
     finally:
         tune.reset()
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_uniform_step_decision_boundary(hip, order):
+    """The uniform-step path is taken when every frame pair's fixed-point step
+    matches pair 0's to within one unit of 2^-32 cycles per axis, and not
+    otherwise.  With n = 64 and a 1/64 m pixel the period is 1 m, so
+    displacements of m / 2^32 metres are exact phase units.  The plain path
+    is recognised by being bit-identical to the folded call (which always
+    takes it); the uniform path differs from it by rounding, within 1e-6."""
+    import torch
+
+    from pyturb_amd import ops, tune
+
+    print(f"uniform-step decision order {order}", flush=True)
+    n, L, F = 64, 2, 4
+    rng = np.random.default_rng(70 + order)
+    dev = torch.device("cuda", 0)
+    spec = (rng.standard_normal((L, n, n)) + 1j * rng.standard_normal((L, n, n))) / n
+    spectra = torch.tensor(spec.astype(np.complex64), device=dev)
+    base = rng.integers(0, 1 << 31, (L, 2)).astype(np.float64)
+    step = rng.integers(1 << 20, 1 << 24, (L, 2)).astype(np.float64)
+    work = torch.empty((ops.frames_work_elems(n, F, L, 0),), dtype=torch.complex64, device=dev)
+    try:
+        tune.set_variant("frames_order", n, order)
+        ops.spectral_fold(spectra, F, work)
+        for extra, uniform in ((0, True), (1, True), (-1, True), (2, False), (-2, False)):
+            units = base[None] + np.arange(F)[:, None, None] * step[None]
+            units[3, 0, 0] += extra  # the second frame of pair 1
+            disp = torch.tensor(units / 2.0**32, dtype=torch.float64, device=dev)
+            got = ops.spectral_frames(spectra, disp, None, 1.0 / n)
+            plain = ops.spectral_frames(spectra, disp, None, 1.0 / n, work=work, folded=True)
+            if uniform:
+                assert not torch.equal(got, plain), extra
+                err = ((got - plain).square().mean() / plain.square().mean()).sqrt().item()
+                assert err < 1e-6, (extra, err)
+            else:
+                assert torch.equal(got, plain), extra
+    finally:
+        tune.reset()
