@@ -294,29 +294,53 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
   __syncthreads();
   const bool has_b = uni || 2 * pr + 1 < F;
   float2* tp = tab + (size_t)pr * 2 * L * S;
-  for (int q = first; q < 2 * N; q += stride) {
-    const int k0 = q < N ? N / 2 : q - N, k1 = q < N ? q : N / 2;
-    float2 acc = make_float2(0.f, 0.f);
-    if (q < N || k0 != N / 2) {
-      for (int l = 0; l < L; ++l) {
-        float2 w[2];
+  // Nyquist sums.  With s'(k) the signed frequency that maps the Nyquist index
+  // to +N/2 (the E factor folds into it), a row entry (q < N: k0 = N/2,
+  // k1 = q) has phase (N/2) u_x + s'(k1) u_y and a column entry (q = N + k0,
+  // k1 = N/2) s'(k0) u_x + (N/2) u_y.  Per layer the phasors along each axis
+  // come from two small tables, exp(2 pi i 2^-32 s' u) = T8[a] T1[b] with
+  // s' + N/2 - 1 = 8a + b, the other axis' N/2 term folded into T8: one
+  // complex product per entry instead of a sine and cosine.
+  constexpr int QMAX = (2 * N + 255) / 256;
+  __shared__ float2 t8s[2][2][NB], t1s[2][2][8];  // [frame][varying axis]
+  float2 acc[QMAX];
 #pragma unroll
-        for (int f = 0; f < 2; ++f) {
-          w[f] = make_float2(0.f, 0.f);
-          if (f == 1 && !has_b) continue;
-          const uint32_t ux = u[f][l][0], uy = u[f][l][1];
-          uint32_t ph = (uint32_t)signed_freq(k0, N) * ux + (uint32_t)signed_freq(k1, N) * uy;
-          if (k0 == N / 2) ph += (uint32_t)N * ux;  // E: exp(2 pi i s_0 / dx)
-          if (k1 == N / 2) ph += (uint32_t)N * uy;  // E: exp(2 pi i s_1 / dx)
-          w[f] = phasor_fixed(ph);
-        }
-        const float2 wv = make_float2(w[0].x - w[1].y, w[0].y + w[1].x);
-        const float2 u = cmul(side[(size_t)l * 2 * N + q], wv);
-        acc.x += u.x;
-        acc.y += u.y;
+  for (int i = 0; i < QMAX; ++i) acc[i] = make_float2(0.f, 0.f);
+  for (int l = 0; l < L; ++l) {
+    __syncthreads();  // the previous layer's tables are no longer read
+    for (int e = threadIdx.x; e < 4 * (NB + 8); e += 256) {
+      const int fa = e / (NB + 8), j = e % (NB + 8);
+      const int f = fa >> 1, ax = fa & 1;
+      float2 v = make_float2(0.f, 0.f);
+      if (f == 0 || has_b) {
+        const uint32_t uv = u[f][l][ax], uo = u[f][l][ax ^ 1];
+        v = j < NB ? phasor_fixed((uint32_t)(8 * j - N / 2 + 1) * uv + (uint32_t)(N / 2) * uo)
+                   : phasor_fixed((uint32_t)(j - NB) * uv);
       }
+      if (j < NB)
+        t8s[f][ax][j] = v;
+      else
+        t1s[f][ax][j - NB] = v;
     }
-    corr[(size_t)pr * 2 * N + q] = acc;
+    __syncthreads();
+    const float2* sl = side + (size_t)l * 2 * N;
+#pragma unroll
+    for (int i = 0; i < QMAX; ++i) {
+      const int q = first + stride * i;
+      if (q >= 2 * N || (q >= N && q - N == N / 2)) continue;
+      const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
+      const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
+      const float2 wa = cmul(t8s[0][ax][id >> 3], t1s[0][ax][id & 7]);
+      const float2 wb = cmul(t8s[1][ax][id >> 3], t1s[1][ax][id & 7]);
+      const float2 t = cmul(sl[q], make_float2(wa.x - wb.y, wa.y + wb.x));
+      acc[i].x += t.x;
+      acc[i].y += t.y;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < QMAX; ++i) {
+    const int q = first + stride * i;
+    if (q < 2 * N) corr[(size_t)pr * 2 * N + q] = acc[i];
   }
   for (int q = first; q < 2 * L * S; q += stride) {
     const int fl = q / S, e = q % S;
