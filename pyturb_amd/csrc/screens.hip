@@ -32,9 +32,16 @@ namespace pt {
 // of the groups are summed (the low-frequency field is linear in them).
 // Group g's noise is sh_noise + g * P * 2 * count * 9, its amplitudes
 // sh_amp + g * P * 9.
+// The basis table U[p][x] = e_p(x), for calls whose sets read it instead of
+// each evaluating it (k_gen_prep / k_frame_prep with `ubasis`).
+__global__ __launch_bounds__(256) void k_sub_basis(float2* __restrict__ U, int P, int n) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx < P * n) U[idx] = sh_basis_e(idx / n, idx % n, n);
+}
+
 __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_noise,
                                                   const double* __restrict__ sh_amp, int count, int P, int n,
-                                                  int G, SubTables t) {
+                                                  int G, SubTables t, int ubasis) {
   __shared__ double2 C[PMAX * 9];
   __shared__ double s_mean;
   const int s = blockIdx.x, tid = threadIdx.x;
@@ -50,19 +57,21 @@ __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_n
     C[q] = c;
   }
   __syncthreads();
-  sub_fill_set(C, P, n, s, t, s == 0, &s_mean, tid, 256);
+  sub_fill_set(C, P, n, s, t, !ubasis && s == 0, &s_mean, tid, 256, 0, 1, ubasis ? t.U : nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ disp,
                                                     const double2* __restrict__ coeffs, size_t cstride, int L,
-                                                    int P, int n, double inv_period, SubTables t, int tr) {
+                                                    int P, int n, double inv_period, SubTables t, int tr,
+                                                    int ubasis) {
   extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ double s_mean;
   double2* Q = (double2*)dyn;
   double2* SB = Q + 2 * (size_t)P * L;
   const int f = blockIdx.x, tid = threadIdx.x;
   frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256, tr != 0);
-  sub_fill_set(SB, P, n, f, t, f == 0, &s_mean, tid, 256, blockIdx.y, gridDim.y);
+  sub_fill_set(SB, P, n, f, t, !ubasis && f == 0, &s_mean, tid, 256, blockIdx.y, gridDim.y,
+               ubasis ? t.U : nullptr);
 }
 
 // Workgroups per frame (or pair) of the small per-call table kernels: one
@@ -70,11 +79,26 @@ __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ d
 // on a handful of CUs.
 static inline int table_split(int units) { return units >= 256 ? 1 : (256 / units < 8 ? 256 / units : 8); }
 
+// Calls of at least this many sets write the basis table first and let every
+// set read it (one small launch instead of P * n evaluations per set).
+constexpr int UBASIS_MIN = 16;
+
+static int launch_sub_basis(SubTables t, int P, int n, hipStream_t st) {
+  k_sub_basis<<<(P * n + 255) / 256, 256, 0, st>>>(t.U, P, n);
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
 int launch_gen_prep(const float* sh_noise, const double* sh_amp, int count, int P, int n, SubTables t,
                     hipStream_t st, int groups) {
   if (P == 0) return PT_OK;
   if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
-  k_gen_prep<<<count, 256, 0, st>>>(sh_noise, sh_amp, count, P, n, groups, t);
+  const int ub = count >= UBASIS_MIN;
+  if (ub) {
+    const int rc = launch_sub_basis(t, P, n, st);
+    if (rc) return rc;
+  }
+  k_gen_prep<<<count, 256, 0, st>>>(sh_noise, sh_amp, count, P, n, groups, t, ub);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -85,8 +109,13 @@ int launch_frame_prep(const double* disp, const double2* coeffs, int F, int L, i
   if (P > PMAX) return fail(PT_ERR_UNSUPPORTED, "more than %d subharmonic levels", PMAX);
   const size_t dyn = (size_t)(2 * P * L + 9 * P) * sizeof(double2);
   if (dyn > 60 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: %d layers x %d levels exceed LDS", L, P);
+  const int ub = F >= UBASIS_MIN;
+  if (ub) {
+    const int rc = launch_sub_basis(t, P, n, st);
+    if (rc) return rc;
+  }
   k_frame_prep<<<dim3(F, table_split(F)), 256, dyn, st>>>(disp, coeffs, cstride, L, P, n, inv_period, t,
-                                                          transpose ? 1 : 0);
+                                                          transpose ? 1 : 0, ub);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }

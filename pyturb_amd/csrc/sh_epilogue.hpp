@@ -55,8 +55,11 @@ __host__ __device__ inline SubTables sub_tables(float2* base, int P, int n, int 
 // no barrier follows the coefficients (a chunked (level, column) split needed
 // two per 32 columns, and a 2000-frame call was bound by that serial chain).
 // K sums its level terms in ascending order.
+// `basis` (may be NULL) is the finished basis table U: its entries are
+// read instead of evaluated per column.
 __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTables t, bool writeU,
-                                    double* s_mean, int tid, int nthreads, int part = 0, int parts = 1) {
+                                    double* s_mean, int tid, int nthreads, int part = 0, int parts = 1,
+                                    const float2* basis = nullptr) {
   __shared__ double terms[PMAX];
   __shared__ double hc[PMAX * 9];  // sh_column_coeffs per level
   for (int p = tid; p < P; p += nthreads) {
@@ -81,7 +84,7 @@ __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTa
     for (int p = 0; p < P; ++p) {
       double2 Hd;
       double g1;
-      sh_column(hc + p * 9, sh_basis_e(p, y, n), Hd, g1);
+      sh_column(hc + p * 9, basis ? basis[(size_t)p * n + y] : sh_basis_e(p, y, n), Hd, g1);
       H[(size_t)p * n + y] = make_float2((float)Hd.x, (float)Hd.y);
       kacc += g1;
     }
