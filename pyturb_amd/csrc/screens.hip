@@ -330,16 +330,20 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
   // come from two small tables, exp(2 pi i 2^-32 s' u) = T8[a] T1[b] with
   // s' + N/2 - 1 = 8a + b, the other axis' N/2 term folded into T8: one
   // complex product per entry instead of a sine and cosine.
+  // Tables of LB layers are built per pair of barriers (at most 34 KiB).
   constexpr int QMAX = (2 * N + 255) / 256;
-  __shared__ float2 t8s[2][2][NB], t1s[2][2][8];  // [frame][varying axis]
+  constexpr int LB = 4;
+  __shared__ float2 t8s[LB][2][2][NB], t1s[LB][2][2][8];  // [layer][frame][varying axis]
   float2 acc[QMAX];
 #pragma unroll
   for (int i = 0; i < QMAX; ++i) acc[i] = make_float2(0.f, 0.f);
-  for (int l = 0; l < L; ++l) {
-    __syncthreads();  // the previous layer's tables are no longer read
-    for (int e = threadIdx.x; e < 4 * (NB + 8); e += 256) {
-      const int fa = e / (NB + 8), j = e % (NB + 8);
-      const int f = fa >> 1, ax = fa & 1;
+  for (int l0 = 0; l0 < L; l0 += LB) {
+    const int nl = min(LB, L - l0);
+    __syncthreads();  // the previous layers' tables are no longer read
+    for (int e = threadIdx.x; e < nl * 4 * (NB + 8); e += 256) {
+      const int lb = e / (4 * (NB + 8)), r = e % (4 * (NB + 8));
+      const int fa = r / (NB + 8), j = r % (NB + 8);
+      const int f = fa >> 1, ax = fa & 1, l = l0 + lb;
       float2 v = make_float2(0.f, 0.f);
       if (f == 0 || has_b) {
         const uint32_t uv = u[f][l][ax], uo = u[f][l][ax ^ 1];
@@ -347,23 +351,25 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
                    : phasor_fixed((uint32_t)(j - NB) * uv);
       }
       if (j < NB)
-        t8s[f][ax][j] = v;
+        t8s[lb][f][ax][j] = v;
       else
-        t1s[f][ax][j - NB] = v;
+        t1s[lb][f][ax][j - NB] = v;
     }
     __syncthreads();
-    const float2* sl = side + (size_t)l * 2 * N;
+    for (int lb = 0; lb < nl; ++lb) {
+      const float2* sl = side + (size_t)(l0 + lb) * 2 * N;
 #pragma unroll
-    for (int i = 0; i < QMAX; ++i) {
-      const int q = first + stride * i;
-      if (q >= 2 * N || (q >= N && q - N == N / 2)) continue;
-      const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
-      const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
-      const float2 wa = cmul(t8s[0][ax][id >> 3], t1s[0][ax][id & 7]);
-      const float2 wb = cmul(t8s[1][ax][id >> 3], t1s[1][ax][id & 7]);
-      const float2 t = cmul(sl[q], make_float2(wa.x - wb.y, wa.y + wb.x));
-      acc[i].x += t.x;
-      acc[i].y += t.y;
+      for (int i = 0; i < QMAX; ++i) {
+        const int q = first + stride * i;
+        if (q >= 2 * N || (q >= N && q - N == N / 2)) continue;
+        const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
+        const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
+        const float2 wa = cmul(t8s[lb][0][ax][id >> 3], t1s[lb][0][ax][id & 7]);
+        const float2 wb = cmul(t8s[lb][1][ax][id >> 3], t1s[lb][1][ax][id & 7]);
+        const float2 t = cmul(sl[q], make_float2(wa.x - wb.y, wa.y + wb.x));
+        acc[i].x += t.x;
+        acc[i].y += t.y;
+      }
     }
   }
 #pragma unroll
