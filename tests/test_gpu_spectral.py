@@ -210,6 +210,26 @@ def test_uniform_step_frames_match_oracle(hip, order, n, frames):
             assert O.rel_rms(got["uniform"][t], got["off"][t]) < 1e-6, t
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_spectral_frames_many_frames_match_oracle(hip, order):
+    """20 equally spaced frames with 8 subharmonic levels: a call of 16 or
+    more frames reads the subharmonic basis from its table instead of
+    evaluating it per frame, and the frame pairs share one uniform step."""
+    from pyturb_amd import tune
+
+    print(f"20 frames order={order}", flush=True)
+    atm = _frames_case(64, None)
+    times = 0.25 + 1e-3 * np.arange(20)
+    disp = np.stack([np.stack(atm.displacements(t), axis=-1) for t in times])
+    try:
+        tune.set_variant("frames_order", atm.ns, order)
+        got = _run_frames(atm, disp)
+    finally:
+        tune.reset()
+    for k, t in enumerate(times):
+        assert O.rel_rms(got[k], atm.phase(t)) < REL, (t, O.rel_rms(got[k], atm.phase(t)))
+
+
 def test_spectral_frame_matches_reference_golden(hip):
     g = load_golden("atmosphere_n64_seed2.npz")
     atm = _frames_case(64, None)
