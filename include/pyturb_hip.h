@@ -135,7 +135,12 @@ int pt_generate_groups(const float* noise, const float* amplitude,
  *   work      : pt_frames_work_elems(n, frames, layers, levels) c64 (or the
  *               layer-independent bound pt_work_elems(1, n, frames, levels))
  * Shift-theorem phases are reduced exactly (32-bit fixed-point cycles), so
- * large travel keeps full FP32 phase accuracy.
+ * large travel keeps full FP32 phase accuracy.  When every frame pair moves
+ * by the same fixed-point step per layer (to within one unit of 2^-32
+ * cycles; equally spaced times), the call folds that step into the stack and
+ * builds each pair's second frame at the first frame's phase plus the step:
+ * half the phasor work, results within 1e-6 of the general path.  The call
+ * decides this on the device; the decision lives in `work`.
  */
 int pt_spectral_frames(const void* spectra, int layers, int n,
                        double pixel_scale, const double* disp, int frames,
