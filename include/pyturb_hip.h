@@ -173,14 +173,20 @@ int pt_spectral_cols(void* work, int layers, int n, double pixel_scale,
 
 /*
  * The fold of pt_spectral_rows on its own, and pt_spectral_rows without it.
- * The fold (the Hermitian-folded layer stack and its Nyquist side table,
- * in `work`) depends only on the spectra, the layer count, n, the frame
- * count (which places it in the workspace) and the transform order
- * (PT_TUNE_FRAMES).  A caller whose spectra do not change between calls of
- * the same frame count, such as a loop replaying a captured HIP graph,
- * folds once with pt_spectral_fold and then calls pt_spectral_rows_folded
- * + pt_spectral_cols per step.  No counterpart in the reference (its
- * per-frame layer sum is atmosphere.py:910-917).  Power-of-two n only.
+ * The fold written by pt_spectral_fold (the Hermitian-folded layer stack
+ * and its Nyquist side table, in `work`) depends only on the spectra, the
+ * layer count, n, the frame count (which places it in the workspace) and
+ * the transform order (PT_TUNE_FRAMES).  A caller whose spectra do not
+ * change between calls of the same frame count, such as a loop replaying a
+ * captured HIP graph, folds once with pt_spectral_fold and then calls
+ * pt_spectral_rows_folded + pt_spectral_cols per step.
+ * Only pt_spectral_fold's output may be reused this way: the fold that
+ * pt_spectral_rows or pt_spectral_frames leave in `work` carries that
+ * call's uniform step when it had one.  The workspace records which fold
+ * it holds, and pt_spectral_rows_folded on any other fold (a stepped one,
+ * or a workspace never folded) produces NaN frames.
+ * No counterpart in the reference (its per-frame layer sum is
+ * atmosphere.py:910-917).  Power-of-two n only.
  */
 int pt_spectral_fold(const void* spectra, int layers, int n, int frames,
                      void* work, void* stream);

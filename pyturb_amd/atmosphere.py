@@ -338,20 +338,33 @@ class Atmosphere:
     def time(self) -> float:
         return self._t
 
-    def _check_wrap(self, disp_x, disp_y) -> None:
+    def _check_wrap(self, disp_x, disp_y, stacklevel: int = 3) -> None:
+        """Warn once per run the first time a layer's travel passes the
+        screen period (atmosphere.py:732-751); reset() re-arms it."""
         disp = np.hypot(np.asarray(disp_x), np.asarray(disp_y))
         wrapped = disp > self._screen_period_m
         if not np.any(wrapped):
             return
         self._wrap_warned = True
-        idx = int(np.argmax(wrapped.reshape(-1, len(self.layers)).any(axis=0)))
+        per_layer = disp.reshape(-1, len(self.layers))
+        idx = int(np.argmax((per_layer > self._screen_period_m).any(axis=0)))
         layer = self.layers[idx]
+        travel = float(per_layer[:, idx].max())
         warnings.warn(
             f"engine='spectral' layer {idx} (altitude={layer.altitude:.0f} m, "
             f"wind_speed={layer.wind_speed:.1f} m/s) has wrapped: its cumulative travel "
-            f"exceeds the screen's period ({self._screen_period_m:.2f} m). Use "
-            "engine='extrude' for non-periodic frozen flow, or keep run duration below "
-            "`atm.time_to_wrap` seconds.", PeriodicWrapWarning, stacklevel=3)
+            f"({travel:.2f} m) exceeds the screen's period ({self._screen_period_m:.2f} m). "
+            "Its 'new' turbulence is a repeat of previous turbulence, biasing temporal "
+            "statistics over this run. Use engine='extrude' for non-periodic frozen flow, "
+            "or keep run duration below `atm.time_to_wrap` seconds.",
+            PeriodicWrapWarning, stacklevel=stacklevel)
+
+    def _wrap_check_at(self, t: float) -> None:
+        """The on-axis wrap check of one frame at time ``t``, made when that
+        frame is handed out (atmosphere.py:1146-1149 checks inside
+        ``_phase``, frame by frame)."""
+        if not self._wrap_warned and self.engine == "spectral":
+            self._check_wrap(self._vx * t, self._vy * t, stacklevel=4)
 
     # ------------------------------------------------------------------ output
     def _scale(self, wavelength: Optional[float]) -> float:
@@ -377,13 +390,16 @@ class Atmosphere:
             oys.append(np.tan(thy * _ARCSEC_TO_RAD))
         return np.array(oxs), np.array(oys)
 
-    def _spectral(self, times, oxs, oys, scale):
-        """(T*D) frames for every (time, direction) pair, one device call."""
+    def _spectral(self, times, oxs, oys, scale, check_wrap=True):
+        """(T*D) frames for every (time, direction) pair, one device call.
+
+        ``check_wrap=False`` leaves the wrap warning to the caller, which
+        checks each frame as it hands it out (frames())."""
         times = np.atleast_1d(np.asarray(times, dtype=np.float64))
         dx = self._vx[None, None, :] * times[:, None, None] + self._alt * oxs[None, :, None]
         dy = self._vy[None, None, :] * times[:, None, None] + self._alt * oys[None, :, None]
-        if not self._wrap_warned:
-            self._check_wrap(dx, dy)
+        if check_wrap and not self._wrap_warned:
+            self._check_wrap(dx, dy, stacklevel=4)
         disp = np.stack([dx, dy], axis=-1).reshape(-1, len(self._layers), 2)
         if self._lgs_mag is not None:
             return self._lgs_frames(disp, scale)
@@ -486,10 +502,12 @@ class Atmosphere:
             for _ in range(k):
                 times.append(t)
                 t += dt
-            chunk = self._cast(self._spectral(times, np.zeros(1), np.zeros(1), scale))
+            chunk = self._cast(self._spectral(times, np.zeros(1), np.zeros(1), scale,
+                                              check_wrap=False))
             for i in range(k):
                 if self._t != times[i]:  # clock moved under us (reset): recompute
                     break
+                self._wrap_check_at(times[i])
                 yield times[i], chunk[i]
                 self._t += dt
                 left -= 1
@@ -516,8 +534,6 @@ class Atmosphere:
             times = np.asarray(times, dtype=np.float64)
             dx = self._vx[None, :] * times[:, None]
             dy = self._vy[None, :] * times[:, None]
-            if not self._wrap_warned:
-                self._check_wrap(dx, dy)
             disp = _hip.to_device(np.stack([dx, dy], axis=-1), np.float64, self._dev)
         noise = sh_noise = None
         if self._boil_noise is not None:
@@ -572,6 +588,7 @@ class Atmosphere:
             used = 0
             try:
                 for i in range(k):
+                    self._wrap_check_at(times[i])
                     yield times[i], chunk[i]
                     if self._epoch != epoch:  # reset while the frame was out
                         break

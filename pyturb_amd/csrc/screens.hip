@@ -201,6 +201,14 @@ __global__ __launch_bounds__(ColsCfg<N>::T) void k_fft_cols(const float2* __rest
 constexpr int FOLD_LMAX = 16;  // layers of the fused (paired) frame path
 constexpr int STEP_BLOCKS = 16;
 constexpr int STEP_WORDS = STEP_BLOCKS + 2 * FOLD_LMAX;
+// Word STEP_WORDS of the step region records what the fold in the workspace
+// holds: FOLD_PLAIN for the plain Hermitian fold (written by every fold
+// without a uniform step, pt_spectral_fold's included), anything else for a
+// fold that carries a step factor (1 + i D) and is valid only for the call
+// that made it.  The pair tables of pt_spectral_rows_folded check it and
+// write NaN tables for a fold that is not plain, so a stepped fold reused by
+// mistake gives NaN frames instead of silently wrong ones.
+constexpr uint32_t FOLD_PLAIN = 0x504C4E46u;
 
 __global__ __launch_bounds__(1024) void k_frame_step(const double* __restrict__ disp, int L, int F,
                                                      double inv_period, uint32_t* __restrict__ step) {
@@ -246,9 +254,15 @@ __device__ __forceinline__ float2 fold_step(float2 v, const uint32_t* __restrict
   return make_float2(v.x - vd.y, v.y + vd.x);
 }
 
+__device__ __forceinline__ void fold_mark(uint32_t* mark, const uint32_t* step) {
+  if (mark && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *mark = step_uniform(step) ? 0u : FOLD_PLAIN;
+}
+
 __global__ __launch_bounds__(256) void k_frame_fold(const float2* __restrict__ spectra, int n,
                                                     const uint32_t* __restrict__ step,
-                                                    float2* __restrict__ fold, float2* __restrict__ side) {
+                                                    float2* __restrict__ fold, float2* __restrict__ side,
+                                                    uint32_t* __restrict__ mark) {
+  fold_mark(mark, step);
   const size_t plane = (size_t)n * n;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= plane) return;
@@ -303,9 +317,13 @@ template <int N, bool TR = false>
 __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restrict__ disp,
                                                            const float2* __restrict__ side, int L, int F,
                                                            double inv_period, const uint32_t* __restrict__ step,
-                                                           float2* __restrict__ tab, float2* __restrict__ corr) {
+                                                           float2* __restrict__ tab, float2* __restrict__ corr,
+                                                           const uint32_t* __restrict__ guard) {
   constexpr int NB = N / 8, S = pair_tab_stride<N>();
   const int pr = blockIdx.x;
+  // a reused fold (guard set) must be the plain one, else NaN tables
+  const bool poison = guard && *guard != FOLD_PLAIN;
+  const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
   const int first = blockIdx.y * 256 + threadIdx.x, stride = gridDim.y * 256;  // entries split over blocks
   const bool uni = step_uniform(step);
   // fixed-point phases of the pair's frames, once per workgroup:
@@ -375,7 +393,7 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
 #pragma unroll
   for (int i = 0; i < QMAX; ++i) {
     const int q = first + stride * i;
-    if (q < 2 * N) corr[(size_t)pr * 2 * N + q] = acc[i];
+    if (q < 2 * N) corr[(size_t)pr * 2 * N + q] = poison ? nan2 : acc[i];
   }
   for (int q = first; q < 2 * L * S; q += stride) {
     const int fl = q / S, e = q % S;
@@ -389,7 +407,7 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
       v = phasor_fixed((uint32_t)(e - NB) * ut);
     else
       v = make_float2(__uint_as_float(uo), 0.f);
-    tp[q] = v;
+    tp[q] = poison && e < NB + 8 ? nan2 : v;
   }
 }
 
@@ -544,7 +562,9 @@ struct PairColsCfg {
 // of the layer, so the lanes of one load read one contiguous run.
 __global__ __launch_bounds__(256) void k_frame_fold_tiles(const float2* __restrict__ spectra, int n, int mt,
                                                           const uint32_t* __restrict__ step,
-                                                          float2* __restrict__ fold, float2* __restrict__ side) {
+                                                          float2* __restrict__ fold, float2* __restrict__ side,
+                                                          uint32_t* __restrict__ mark) {
+  fold_mark(mark, step);
   const size_t plane = (size_t)n * n;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= plane) return;
@@ -881,7 +901,7 @@ static FramesLayout frames_layout(int n, int F, int L, int P) {
   w.ptab = w.side + r32((size_t)L * 2 * n);
   w.corr = w.ptab + r32((size_t)((F + 1) / 2) * 2 * L * (n / 8 + 9));
   w.step = w.corr + r32((size_t)((F + 1) / 2) * 2 * n);
-  w.tab = w.step + r32((size_t)(STEP_WORDS + 1) / 2);  // uint32 words of k_frame_step
+  w.tab = w.step + r32((size_t)(STEP_WORDS + 2) / 2);  // uint32 words of k_frame_step, then the fold marker
   w.total = w.tab + sub_table_c64(P, n, F);
   return w;
 }
@@ -907,17 +927,18 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
   // uniform-step fold only where one call folds and pairs (the step is read
   // from this call's displacements)
   uint32_t* step = stages == STAGE_ALL ? (uint32_t*)(work + w.step) : nullptr;
+  uint32_t* mark = (uint32_t*)(work + w.step) + STEP_WORDS;
   if (step) {
     k_frame_step<<<STEP_BLOCKS, 1024, 0, st>>>(disp, L, F, inv_period, step);
     PT_LAUNCH_CHECK();
   }
   if (stages & STAGE_FOLD) {
-    k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, step, fold, side);
+    k_frame_fold<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, step, fold, side, mark);
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
-  k_frame_pair_tables<N><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(disp, side, L, F,
-                                                                                     inv_period, step, ptab, corr);
+  k_frame_pair_tables<N><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
+      disp, side, L, F, inv_period, step, ptab, corr, stages == STAGE_PAIRS ? mark : nullptr);
   PT_LAUNCH_CHECK();
   k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), PairCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, step,
                                                                                  work);
@@ -946,17 +967,18 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
   float2* ptab = work + w.ptab;
   float2* corr = work + w.corr;
   uint32_t* step = stages == STAGE_ALL ? (uint32_t*)(work + w.step) : nullptr;  // as in frames_rows_pow2
+  uint32_t* mark = (uint32_t*)(work + w.step) + STEP_WORDS;
   if (step) {
     k_frame_step<<<STEP_BLOCKS, 1024, 0, st>>>(disp, L, F, inv_period, step);
     PT_LAUNCH_CHECK();
   }
   if (stages & STAGE_FOLD) {
-    k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, step, fold, side);
+    k_frame_fold_tiles<<<dim3((N * N + 255) / 256, L), 256, 0, st>>>(spectra, N, MT, step, fold, side, mark);
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
   k_frame_pair_tables<N, true><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
-      disp, side, L, F, inv_period, step, ptab, corr);
+      disp, side, L, F, inv_period, step, ptab, corr, stages == STAGE_PAIRS ? mark : nullptr);
   PT_LAUNCH_CHECK();
   k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(fold, corr, L, ptab, F, step,
                                                                                   work);
