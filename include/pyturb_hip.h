@@ -149,6 +149,26 @@ int pt_spectral_frames(const void* spectra, int layers, int n,
                        void* stream);
 
 /*
+ * pt_spectral_frames with no workspace argument, for callers that cannot
+ * allocate device memory (code under HIP-graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a static device array of PT_NOALLOC_SCRATCH_BYTES per GPU,
+ * held by the library until the process ends; calls whose
+ * pt_frames_work_elems(n, frames, layers, levels) * 8 bytes exceed it return
+ * PT_ERR_ARG (512^2 with 9 layers and 8 levels: up to 44 frames per call).
+ * The call allocates nothing and runs the kernels of pt_spectral_frames, so
+ * its frames are bit-identical to pt_spectral_frames'.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams
+ * or threads) race and must be serialised by the caller.
+ */
+#define PT_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int pt_spectral_frames_noalloc(const void* spectra, int layers, int n,
+                               double pixel_scale, const double* disp,
+                               int frames, const double* sh_coeffs, int levels,
+                               int crop_offset, int n_out, float scale,
+                               float* out, void* stream);
+
+/*
  * pt_spectral_frames split into its two passes (power-of-two n, <= 16
  * layers, <= 12 subharmonic levels), for callers that schedule or time them
  * separately.  Frames are transformed in pairs: Re(ifft2(X)) equals the

@@ -45,6 +45,7 @@ SIGNATURES = {
     "pt_generate_groups": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp,
                                 _vp]),
     "pt_spectral_frames": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp]),
+    "pt_spectral_frames_noalloc": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp]),
     "pt_spectral_rows": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _vp]),
     "pt_spectral_cols": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp]),
     "pt_spectral_fold": (_i, [_vp, _i, _i, _i, _vp, _vp]),
