@@ -43,9 +43,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "9-layer 512\u00b2 OPD frames/s + Monte-Carlo screens/s, fused vs plain, %HBM peak"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per pass-A launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# counters scaled against a plain copy; the profile summary states the runs.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spectral_frames_512_9layer.md")
+# wave64 VALU instruction issue peak, non-packed FP32: 1024 SIMDs at 2.4 GHz,
+# one wave64 instruction per SIMD every 4 cycles (G instructions/s)
+VALU_PEAK_GIPS = 1024 * 2.4e9 / 4 / 1e9
+# Per-launch counter figures (HBM bytes from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# scaled against a plain copy, wave64 VALU instructions from SQ_INSTS_VALU) of
+# the frame engine and the Monte-Carlo passes; each summary states its runs.
+FRAMES_PROFILE = os.path.join("profiles", "spectral_frames_512_9layer.md")
+MC_PROFILE = os.path.join("profiles", "mc_ensemble_mixed_sizes.md")
 N_FRAMES = 2000
 DT = 1e-3
 N = 512
@@ -150,6 +155,32 @@ def plain_frame(atm, t, ctx):
     return (total + low) * (atm.wavelength / (2.0 * np.pi))
 
 
+def plain_frames_batched(atm, times, ctx):
+    """The same plain algorithm for a batch of frames: one elementwise kernel
+    chain per layer over the whole batch, one batched transform (FP32)."""
+    import torch
+
+    spec, f, fp, basis, coeffs = ctx
+    L = spec.shape[0]
+    tt = torch.tensor(np.asarray(times), dtype=torch.float64, device=spec.device)
+    sx = (tt[:, None] * torch.tensor(atm._vx, device=spec.device)).float()  # (F, L)
+    sy = (tt[:, None] * torch.tensor(atm._vy, device=spec.device)).float()
+    S = None
+    for l in range(L):
+        phx = torch.exp((2j * np.pi) * sx[:, l, None] * f[None, :])
+        phy = torch.exp((2j * np.pi) * sy[:, l, None] * f[None, :])
+        term = spec[l][None] * phx[:, :, None] * phy[:, None, :]
+        S = term if S is None else S + term
+    total = (torch.fft.ifft2(S) * (N * N)).real
+    px = torch.exp((2j * np.pi) * sx[:, None, :, None] * fp[None, :, None, :])  # (F, P, L, 3)
+    py = torch.exp((2j * np.pi) * sy[:, None, :, None] * fp[None, :, None, :])
+    shifted = (coeffs[None] * px[..., :, None] * py[..., None, :]).sum(dim=2)  # (F, P, 3, 3)
+    m = torch.matmul(shifted, basis[None])  # (F, P, 3, n)
+    low = torch.einsum("pax,fpay->fxy", basis, m).real
+    low = low - low.mean(dim=(-2, -1), keepdim=True)
+    return (total + low) * (atm.wavelength / (2.0 * np.pi))
+
+
 def plain_context(atm):
     import torch
 
@@ -207,13 +238,18 @@ def build_mc(dev):
     return classes
 
 
-def run_mc(classes):
-    """Every size class in one pass over all its L0 groups (pt_generate_groups)."""
+def ops_generate_groups(c):
+    """One size class, all its L0 groups, one pt_generate_groups call."""
     from pyturb_amd import ops
 
+    ops.generate_groups(c["noise"], c["shn"], c["amp"], c["sh_amp"], c["count"],
+                        screen_scale=c["scale"], out=c["out"], work=c["work"])
+
+
+def run_mc(classes):
+    """Every size class in one pass over all its L0 groups (pt_generate_groups)."""
     for c in classes:
-        ops.generate_groups(c["noise"], c["shn"], c["amp"], c["sh_amp"], c["count"],
-                            screen_scale=c["scale"], out=c["out"], work=c["work"])
+        ops_generate_groups(c)
 
 
 def run_mc_plain(classes):
@@ -308,19 +344,22 @@ def run_extrude(ext, thx, thy, mags, out, steps, t0=0):
         ext.integrate_dirs(thx, thy, scale=scale, mags=mags, out=out)
 
 
-def measured_traffic(order, what="traffic_bytes_per_launch"):
-    """HBM bytes (or, with ``what="valu_insts_per_launch"``, wave64 VALU
-    instructions) per pass-A launch of the given transform order from the
-    committed counter summary, or None."""
-    key = what + "_" + ("column_first:" if order == 0 else "row_first:")
+def profile_figures(path, prefix):
+    """``{name: number}`` from the ``prefix<name>: <number>`` lines of a
+    committed counter summary (empty if the file or the lines are absent)."""
+    out = {}
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(os.path.join(ROOT, path)) as fh:
             for line in fh:
-                if line.startswith(key):
-                    return int(line.split(":", 1)[1].strip().split()[0])
-    except (OSError, ValueError, IndexError):
+                if line.startswith(prefix) and ":" in line:
+                    name, val = line[len(prefix):].split(":", 1)
+                    try:
+                        out[name.strip()] = float(val.strip().split()[0])
+                    except (ValueError, IndexError):
+                        pass
+    except OSError:
         pass
-    return None
+    return out
 
 
 def timed(fn, steps, warmup):
@@ -336,8 +375,34 @@ def timed(fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(atm, budget_s=12.0):
-    """FP64 oracle frames on this host, single thread, bounded sample."""
+def host_description():
+    """CPU model, core counts and library versions of this host."""
+    import platform
+
+    import scipy
+
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        import numba  # noqa: F401
+
+        numba_v = numba.__version__
+    except ImportError:
+        numba_v = None
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(),
+            "cores_available": len(os.sched_getaffinity(0)),
+            "numpy": np.__version__, "scipy": scipy.__version__, "numba": numba_v}
+
+
+def cpu_frames(atm, budget_s):
+    """FP64 oracle frames of the headline workload for about budget_s."""
     import oracle as O
 
     spectra = atm._spectra.cpu().numpy().astype(np.complex128)
@@ -351,9 +416,59 @@ def cpu_baseline(atm, budget_s=12.0):
                     atm._vy * t)
         done += 1
         el = time.perf_counter() - t0
-        if el > budget_s or done >= 400:
+        if el > budget_s:
             break
     return done / el, done, el
+
+
+def cpu_mc(budget_s):
+    """FP64 oracle screens of the mixed ensemble (both L0 groups summed, per-
+    screen r0), in the classes' 4:3:1 proportion (a round is 8 screens at 256^2,
+    6 at 512^2 and 2 at 1024^2), for about budget_s."""
+    import oracle as O
+
+    rng = np.random.default_rng(4)
+    rounds = []
+    for n, count in ((256, 8), (512, 6), (1024, 2)):
+        filts = [O.build_filters(n, 8.0 / n, 1.0, L0, subharmonics=8) for L0 in MC_L0]
+        noise = [O.draw_generate_noise(rng, n, count, 8) for _ in MC_L0]
+        gain = (rng.uniform(0.08, 0.25, count) * 0.5 ** -0.6) ** (-5.0 / 6.0)
+        rounds.append((filts, noise, gain, count))
+    done, t0 = 0, time.perf_counter()
+    while True:
+        for filts, noise, gain, count in rounds:
+            tot = sum(O.generate(fl, nz[0], nz[1], count) for fl, nz in zip(filts, noise))
+            tot *= gain[:, None, None]
+            done += count
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    return done / el, done, el
+
+
+def cpu_baseline(atm, budget_s=7.0):
+    """The FP64 NumPy oracle on this host: frames and Monte-Carlo screens, each
+    on 1 thread (NumPy pocketfft, BLAS limited to one thread) and on every
+    available core (scipy.fft workers, all BLAS threads)."""
+    import oracle as O
+    from threadpoolctl import threadpool_limits
+
+    host = host_description()
+    cores = host["cores_available"]
+    rows = []
+    for threads in (1, cores):
+        O.set_fft_workers(None if threads == 1 else threads)
+        try:
+            with threadpool_limits(limits=threads):
+                fr, fd, fe = cpu_frames(atm, budget_s)
+                mr, md, me = cpu_mc(budget_s)
+        finally:
+            O.set_fft_workers(None)
+        rows.append({"workload": "frames", "threads": threads, "value": round(fr, 2),
+                     "unit": "frames/s", "sample": f"{fd} frames in {fe:.1f} s"})
+        rows.append({"workload": "monte_carlo", "threads": threads, "value": round(mr, 2),
+                     "unit": "screens/s", "sample": f"{md} mixed screens in {me:.1f} s"})
+    return rows, host
 
 
 def parity(atm, out, frames=(0, 1000, 1999)):
@@ -478,17 +593,44 @@ def main(argv=None):
     tapi = timed(frames_api, 1, 1)
     extra["frames_api_frames_per_s"] = round(N_FRAMES / tapi, 1)
     if not args.no_plain:
+        # plain vs fused on matched call patterns: a batch of frames per call
+        # (the headline's 2000-frame call against the plain form in batches of
+        # 50), and one frame per call (eager against eager)
         ctx = plain_context(atm)
         nplain = 200
         tp = timed(lambda: [plain_frame(atm, float(t), ctx) for t in times[:nplain]], 1, 1)
         extra["plain_frames_per_s"] = round(nplain / tp, 1)
-        extra["fused_vs_plain_frames"] = round(fps / (nplain / tp), 1)
+        chunk = 50
+        tpb = timed(lambda: [plain_frames_batched(atm, times[k:k + chunk], ctx)
+                             for k in range(0, N_FRAMES, chunk)], 1, 1)
+        extra["plain_batched_frames_per_s"] = round(N_FRAMES / tpb, 1)
+        extra["fused_vs_plain_frames_batched"] = round(fps / (N_FRAMES / tpb), 1)
+        if "loop_eager_frames_per_s" in extra:
+            extra["fused_vs_plain_frames_one_per_call"] = round(
+                extra["loop_eager_frames_per_s"] / (nplain / tp), 1)
     if not args.no_mc:
         print("bench: building Monte-Carlo ensemble", file=sys.stderr, flush=True)
         classes = build_mc(dev)
         nscr = sum(c for _, c in MC_SIZES)
         tm = timed(lambda: run_mc(classes), max(1, args.steps // 2), 1)
         mc_rate = nscr * max(1, args.steps // 2) / tm
+        # each class's pt_generate_groups call on its own, HIP events on the
+        # launch stream (row pass + subharmonic prep + column pass)
+        cls_ms = {}
+        for c in classes:
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            reps = 5
+            ev[0].record()
+            for _ in range(reps):
+                ops_generate_groups(c)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / reps
+            alg = (4.0 * len(MC_L0) + 4.0) * c["n"] * c["n"] * c["count"]
+            cls_ms[str(c["n"])] = {"screens": c["count"], "call_ms": round(ms, 4),
+                                   "algorithmic_GBs": round(alg / ms / 1e6, 1),
+                                   "frac_hbm": round(alg / ms / 1e6 / HBM_PEAK_GBS, 3)}
+        extra["mc_classes"] = cls_ms
         px = sum(n * n * c for n, c in MC_SIZES)
         mc_bytes = (4.0 * len(MC_L0) + 4.0) * px
         extra["mc_screens_per_s"] = round(mc_rate, 1)
@@ -590,18 +732,44 @@ def main(argv=None):
     from pyturb_amd import tune as _tune
 
     order = _tune.get("frames_order", N)
-    traffic = measured_traffic(order)
-    valu = measured_traffic(order, "valu_insts_per_launch")
-    # VALU issue share of pass A at the 2.4 GHz peak clock: 1024 SIMDs, 4 cycles per wave64
-    # FP32 instruction (packed FP32 is not used)
-    valu_frac = round(valu * 4 / 1024 / 2.4e9 / t_a, 3) if valu else None
+    tag = "column_first" if order == 0 else "row_first"
+    fprof = profile_figures(FRAMES_PROFILE, "")
+    traffic_a = fprof.get(f"traffic_bytes_per_launch_{tag}")
+    traffic_b = fprof.get(f"passb_traffic_bytes_per_launch_{tag}")
+    valu_a = fprof.get(f"valu_insts_per_launch_{tag}")
+    src = (f"{FRAMES_PROFILE}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy) "
+           "and SQ_INSTS_VALU over the kernels of each call, bench.py --steps 2 --warmup 1")
+    passes = [{
+        "pass": "B: prep + " + ("k_frame_rows_epi<512>" if order == 0 else "k_gen_cols_set<512>"),
+        "bound": "hbm", "call_ms": round(t_b * 1e3, 4),
+        "algorithmic_bytes": bytes_b, "algorithmic_GBs": round(bytes_b / t_b / 1e9, 1),
+        "frac": round(bytes_b / t_b / 1e9 / HBM_PEAK_GBS, 3),
+        "traffic": traffic_b,
+        "traffic_GBs": round(traffic_b / t_b / 1e9, 1) if traffic_b else None,
+        "traffic_frac": round(traffic_b / t_b / 1e9 / HBM_PEAK_GBS, 3) if traffic_b else None}]
+    mprof = profile_figures(MC_PROFILE, "mc_pass ")
+    for name in sorted({k.rsplit("_", 1)[0] for k in mprof}):
+        us, alg, tr = (mprof.get(f"{name}_us"), mprof.get(f"{name}_algorithmic"),
+                       mprof.get(f"{name}_traffic"))
+        if not us:
+            continue
+        passes.append({"pass": f"monte-carlo {name}", "bound": "hbm", "call_ms": us * 1e-3,
+                       "source": f"{MC_PROFILE} (kernel trace and counters of the profiled run)",
+                       "algorithmic_GBs": round(alg / us / 1e3, 1) if alg else None,
+                       "frac": round(alg / us / 1e3 / HBM_PEAK_GBS, 3) if alg else None,
+                       "traffic": tr,
+                       "traffic_GBs": round(tr / us / 1e3, 1) if tr else None,
+                       "traffic_frac": round(tr / us / 1e3 / HBM_PEAK_GBS, 3) if tr else None})
     cpu = None
     if not args.no_cpu:
-        print("bench: CPU baseline (FP64 oracle, 1 thread)", file=sys.stderr, flush=True)
-        rate, done, el = cpu_baseline(atm)
-        cpu = {"value": round(rate, 2), "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"{done} frames of the same 9-layer 512^2 workload, FP64 NumPy "
-                         f"oracle (pocketfft), {el:.1f} s"}
+        print("bench: CPU baseline (FP64 oracle: 1 thread, then every core)", file=sys.stderr,
+              flush=True)
+        rows, host = cpu_baseline(atm)
+        first = rows[0]
+        cpu = {"value": first["value"], "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"{first['sample']} of the same 9-layer 512^2 workload, FP64 NumPy "
+                         "oracle (pocketfft), 1 thread",
+               "rows": rows, "host": host}
     line = {
         "metric": METRIC,
         "value": round(fps, 1),
@@ -617,16 +785,26 @@ def main(argv=None):
         "data": "synthetic (seeded host-drawn layer spectra and winds; device-drawn MC noise)",
         "config": {"workload": "9-layer 512^2 spectral frozen flow, 2000 frames, dt=1 ms",
                    "layers": L, "n": N, "frames_per_step": N_FRAMES, "subharmonics": 8},
-        "roofline": {"bound": "hbm", "achieved": round(ach_a, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach_a / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "traffic_GBs": round(traffic / t_a / 1e9, 1) if traffic else None,
-                     "kernel": ("k_frame_pair_cols<512> (pass A, column-first order)" if order == 0
-                                else "k_frame_pair_rows<512> (pass A, row-first order)"),
-                     "pass_a_ms": round(t_a * 1e3, 4), "pass_b_ms": round(t_b * 1e3, 4),
-                     "pass_a_valu_issue_frac": valu_frac,
-                     "frame_algorithmic_GBs": round(frame_bytes * fps / 1e9, 1),
-                     "pass_b_GBs": round(bytes_b / t_b / 1e9, 1)},
+        # pass A (pt_spectral_rows: k_frame_step + fold + pair tables + pass A),
+        # timed with HIP events on its stream.  Its stack is served from L2, so
+        # the algorithmic bytes exceed the HBM peak and do not bound it: it is
+        # bound by VALU issue (wave64 VALU instructions per launch from the
+        # profile over the live launch time, against the non-packed peak)
+        "roofline": {
+            "kernel": ("pass A: k_frame_pair_cols<512> (column-first)" if order == 0
+                       else "pass A: k_frame_pair_rows<512> (row-first)"),
+            "bound": "valu",
+            "achieved": round(valu_a / t_a / 1e9, 1) if valu_a else None,
+            "peak": VALU_PEAK_GIPS, "unit": "G wave64 VALU instructions/s",
+            "frac": round(valu_a / t_a / 1e9 / VALU_PEAK_GIPS, 3) if valu_a else None,
+            "traffic": traffic_a, "traffic_source": src,
+            "traffic_GBs": round(traffic_a / t_a / 1e9, 1) if traffic_a else None,
+            "hbm_frac": round(traffic_a / t_a / 1e9 / HBM_PEAK_GBS, 3) if traffic_a else None,
+            "call_ms": round(t_a * 1e3, 4),
+            "algorithmic_bytes": bytes_a,
+            "algorithmic_GBs_L2_served": round(ach_a, 1),
+            "frame_algorithmic_GBs": round(frame_bytes * fps / 1e9, 1),
+            "passes": passes},
         "cpu_baseline": cpu,
         "autotuned_shapes": tuned,
         "parity_rel_rms_vs_fp64": float(f"{par:.3g}"),

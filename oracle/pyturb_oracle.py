@@ -52,6 +52,7 @@ __all__ = [
     "build_filters",
     "draw_generate_noise",
     "ifft2_unnormalised",
+    "set_fft_workers",
     "generate",
     "subharmonic_field",
     "draw_flow",
@@ -200,10 +201,28 @@ def draw_generate_noise(rng, n, count, levels, dtype="float64"):
     return main, sh
 
 
+_FFT_WORKERS: Optional[int] = None
+
+
+def set_fft_workers(workers: Optional[int]) -> None:
+    """Threads of the oracle's 2-D transforms: None is NumPy's single-thread
+    pocketfft (the reference's default, ``backend.py:48-50``), an integer
+    runs ``scipy.fft.ifft2(workers=...)``, as the reference does when its
+    global worker count is set (``backend.py:92-113``).  Used by the CPU
+    baseline's all-cores row."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = None if workers is None else int(workers)
+
+
 def ifft2_unnormalised(x):
     """``ifft2 * n0*n1``: the +i, unnormalised 2-D DFT over the last two axes."""
     x = np.asarray(x, dtype=np.complex128)
-    return np.fft.ifft2(x, axes=(-2, -1)) * (x.shape[-1] * x.shape[-2])
+    scale = x.shape[-1] * x.shape[-2]
+    if _FFT_WORKERS is not None:
+        import scipy.fft
+
+        return scipy.fft.ifft2(x, axes=(-2, -1), workers=_FFT_WORKERS) * scale
+    return np.fft.ifft2(x, axes=(-2, -1)) * scale
 
 
 def subharmonic_field(coeffs, basis):
@@ -213,7 +232,12 @@ def subharmonic_field(coeffs, basis):
     (..., n, n) real: ``Re(sum_p basis_p^T c_p basis_p)`` minus its mean over
     the grid (fourier.py:274-278; atmosphere.py:931-934).
     """
-    low = np.einsum("pax,...pab,pby->...xy", basis, coeffs, basis).real
+    coeffs = np.asarray(coeffs)
+    P, n = basis.shape[0], basis.shape[-1]
+    lead = coeffs.shape[:-3]
+    m = np.matmul(coeffs, basis)  # (..., P, 3, n): two batched products, as
+    m = m.reshape(lead + (3 * P, n))  # fourier.py:274-277 forms them
+    low = np.matmul(basis.reshape(3 * P, n).T, m).real
     return low - low.mean(axis=(-2, -1), keepdims=True)
 
 
