@@ -295,8 +295,9 @@ LGS_ALT = 90e3
 LGS_FRAMES = 512  # spectral LGS-cone frames per timed call (9 per-layer transforms each)
 
 
-def build_extrude(dev):
-    """9-layer 1024^2 extruder, 30" field, innovations from seed 3 in HBM."""
+def build_extrude(dev, far_points=32):
+    """9-layer 1024^2 extruder, 30" field, innovations from seed 3 in HBM;
+    stencil of 2 full rows plus ``far_points`` far pixels."""
     import torch
 
     from pyturb_amd.extrude import ExtrudedAtmosphere
@@ -327,7 +328,7 @@ def build_extrude(dev):
     ext = ExtrudedAtmosphere(n=EXT_N, pixel_scale=dx, layer_r0=list(r0s), layer_L0=[25.0] * 9,
                              layer_wind=winds, layer_altitude_los=HEIGHTS,
                              field_of_view_pix=fov, interp="linear", seeds=list(range(30, 39)),
-                             innovations=innov)
+                             innovations=innov, far_points=far_points)
     ang = np.radians(np.arange(6) * 60.0)
     thx = np.concatenate([np.tan(30 * arcsec) * np.cos(ang), [0.0]])
     thy = np.concatenate([np.tan(30 * arcsec) * np.sin(ang), [0.0]])
@@ -714,21 +715,26 @@ def main(argv=None):
         del latm
         torch.cuda.empty_cache()
     if not args.no_extrude:
-        print("bench: building 9-layer 1024^2 extruder (host FP64 A/B)", file=sys.stderr, flush=True)
-        tb = time.perf_counter()
-        ext, thx, thy, mags, eout = build_extrude(dev)
-        extra["extrude_setup_s"] = round(time.perf_counter() - tb, 1)
-        run_extrude(ext, thx, thy, mags, eout, 20)
-        torch.cuda.synchronize()
-        te0 = time.perf_counter()
-        run_extrude(ext, thx, thy, mags, eout, EXT_STEPS - 20, t0=20)
-        torch.cuda.synchronize()
-        te = time.perf_counter() - te0
-        extra["extrude_steps_per_s"] = round((EXT_STEPS - 20) / te, 1)
-        extra["extrude_frames_per_s_7dirs"] = round(7 * (EXT_STEPS - 20) / te, 1)
-        extra["extrude_width"] = ext.width
-        del ext, eout
-        torch.cuda.empty_cache()
+        # the stated stencil (2 rows + 32 far points), then the reference's
+        # full-row stencil alone
+        for far, key in ((32, "extrude"), (0, "extrude_rows_only")):
+            print(f"bench: building 9-layer 1024^2 extruder, {far} far points (host FP64 A/B)",
+                  file=sys.stderr, flush=True)
+            tb = time.perf_counter()
+            ext, thx, thy, mags, eout = build_extrude(dev, far)
+            extra[f"{key}_setup_s"] = round(time.perf_counter() - tb, 1)
+            run_extrude(ext, thx, thy, mags, eout, 20)
+            torch.cuda.synchronize()
+            te0 = time.perf_counter()
+            run_extrude(ext, thx, thy, mags, eout, EXT_STEPS - 20, t0=20)
+            torch.cuda.synchronize()
+            te = time.perf_counter() - te0
+            extra[f"{key}_steps_per_s"] = round((EXT_STEPS - 20) / te, 1)
+            extra[f"{key}_frames_per_s_7dirs"] = round(7 * (EXT_STEPS - 20) / te, 1)
+            extra["extrude_width"] = ext.width
+            extra["extrude_far_points"] = 32
+            del ext, eout
+            torch.cuda.empty_cache()
     from pyturb_amd import tune as _tune
 
     order = _tune.get("frames_order", N)

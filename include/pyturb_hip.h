@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 4
+#define PT_ABI_VERSION 5
 
 #define PT_OK 0
 #define PT_ERR_ARG 1         /* invalid argument (shape, count, NULL)      */
@@ -248,6 +248,25 @@ int pt_extrude_rows(const float* A, int width, int ld, int stencil_rows,
                     const float* bb, float* buf, long long cap, int nact,
                     const int* layer, const int* zrows, const int* trow,
                     const int* bbrow, const float* bscale, void* stream);
+
+/*
+ * pt_extrude_rows with a stencil of m full rows plus `far_points` single
+ * pixels further back (far_points = 8 R, R <= 4 far rows of 8 points).
+ * No counterpart in the reference, whose stencil is full rows only
+ * (src/pyturb/extrude.py:272-275); with far points the new row is
+ *   A @ concat(stencil rows, far points) + bscale * bb[bbrow]
+ * where far point q is column (2 (q % 8) + 1) * width / 16 (integer
+ * division) of far row q / 8.  A is (width, m*ld + round_up(far_points, 16))
+ * with the far points in columns [m*ld, m*ld + far_points) (zero padding
+ * after them).  HOST array far_rows (nact, R) int: the physical ring row of
+ * each far row of each entry.  Other arguments as pt_extrude_rows;
+ * far_points = 0 is pt_extrude_rows.
+ */
+int pt_extrude_rows_far(const float* A, int width, int ld, int stencil_rows,
+                        int far_points, const float* bb, float* buf,
+                        long long cap, int nact, const int* layer,
+                        const int* zrows, const int* far_rows, const int* trow,
+                        const int* bbrow, const float* bscale, void* stream);
 
 /*
  * Fused multi-layer, multi-direction readout of the ring buffers

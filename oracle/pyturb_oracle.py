@@ -65,6 +65,7 @@ __all__ = [
     "phase_covariance",
     "build_extrusion",
     "compact_row_ring",
+    "far_stencil",
     "catmull_rom_weights",
     "lanczos_weights",
     "readout",
@@ -586,17 +587,45 @@ def _spd_solve(spd, rhs):
         return np.linalg.lstsq(spd, rhs, rcond=None)[0]
 
 
-def build_extrusion(width, stencil_rows, dx, L0, r0=1.0):
+def far_stencil(width, stencil_rows, far_points):
+    """The optional far points of the extrusion stencil (no reference
+    counterpart; the reference's stencil is full rows, extrude.py:272-275).
+
+    ``far_points`` = 8 R: R far rows, row j (1..R) lying
+    ``(m - 1) + round((W // 2) ** (j / R))`` rows behind the newest stencil row
+    (raised by one where needed so the distances strictly increase), each with
+    8 pixels at columns ``(2 i + 1) W // 16``.  Returns (distances, columns).
+    """
+    m, W, R = int(stencil_rows), int(width), int(far_points) // 8
+    horizon = max(2, W // 2)
+    dist = []
+    for j in range(1, R + 1):
+        d = (m - 1) + int(round(horizon ** (j / R)))
+        if dist and d <= dist[-1]:
+            d = dist[-1] + 1
+        dist.append(max(d, m))
+    cols = [((2 * i + 1) * W) // 16 for i in range(8)]
+    return dist, cols
+
+
+def build_extrusion(width, stencil_rows, dx, L0, r0=1.0, far_points=0):
     """extrude.py:250-304 / infinite.py:231-269 -- (A, B) in float64.
 
     Stencil rows at y = 0..m-1, new row at y = m, columns x = 0..W-1;
     ``A = C_xz C_zz^-1`` and ``B = V sqrt(max(lambda, 0))`` of the
-    symmetrised residual covariance.
+    symmetrised residual covariance.  With ``far_points`` the stencil also
+    holds the far points of ``far_stencil`` (rows y = m - 1 - d), after the
+    full rows, far row by far row.
     """
     m, W = int(stencil_rows), int(width)
     yy, xx = np.meshgrid(np.arange(m, dtype=float), np.arange(W, dtype=float),
                          indexing="ij")
     zx, zy = xx.ravel(), yy.ravel()
+    if far_points:
+        dist, cols = far_stencil(W, m, far_points)
+        fy, fx = np.meshgrid([m - 1.0 - d for d in dist], np.asarray(cols, float),
+                             indexing="ij")
+        zx, zy = np.concatenate([zx, fx.ravel()]), np.concatenate([zy, fy.ravel()])
     nx, ny = np.arange(W, dtype=float), np.full(W, float(m))
 
     def cov(ax, ay, bx, by):
@@ -617,10 +646,13 @@ def build_extrusion(width, stencil_rows, dx, L0, r0=1.0):
 # ---------------------------------------------------------------------------
 # ring buffer, interpolation, readout  (ring.py, extrude.py, infinite.py)
 # ---------------------------------------------------------------------------
-def compact_row_ring(buf, base, fill, stencil_rows, first_readable):
-    """ring.py:9-29 -- drop consumed rows, keep the stencil tail."""
+def compact_row_ring(buf, base, fill, stencil_rows, first_readable, far_back=None):
+    """ring.py:9-29 -- drop consumed rows, keep the stencil tail (and, with
+    far points, the farthest far row ``far_back`` rows behind the newest)."""
     keep_from = min(int(math.floor(first_readable)) - 1,
                     base + fill - stencil_rows) - base
+    if far_back is not None:
+        keep_from = min(keep_from, fill - 1 - int(far_back))
     keep_from = max(1, keep_from)
     keep = fill - keep_from
     buf[:keep] = buf[keep_from:fill].copy()
@@ -723,8 +755,9 @@ class OracleExtruder:
 
     def __init__(self, n, dx, layer_r0, layer_L0, layer_wind, layer_alt,
                  fov_pix=0.0, stencil_rows=2, lgs_alt=None, A=None, B_unit=None,
-                 tau_boil=None, boil_seed=None):
+                 tau_boil=None, boil_seed=None, far_points=0):
         self.n, self.dx, self.m = int(n), float(dx), int(stencil_rows)
+        self.far_points = int(far_points)
         L = len(layer_r0)
         self.tau_boil = [np.inf] * L if tau_boil is None else [float(t) for t in tau_boil]
         fov = [float(fov_pix)] * L if np.ndim(fov_pix) == 0 else list(fov_pix)
@@ -732,11 +765,17 @@ class OracleExtruder:
         ext, spans = zip(*[along_extent_and_span(self.n, vx, vy, mg, fv)
                            for (vx, vy), mg, fv in zip(layer_wind, mags, fov)])
         self.W = int(math.ceil(2.0 * max(ext) + 2.0 * max(fov))) + 4
-        self.cap = max(spans) + self.n + self.m + 32
-        self.spans = list(spans)
+        self.far_dist, self.far_cols = (far_stencil(self.W, self.m, far_points) if far_points
+                                        else ([], []))
+        reach = self.far_dist[-1] if self.far_dist else 0
+        self.cap = max(spans) + self.n + self.m + 32 + reach
+        # far rows need history behind the first extruded row: extra seed rows
+        self.extra = [max(0, reach + 1 - s) if reach else 0 for s in spans]
+        self.spans = [s + e for s, e in zip(spans, self.extra)]
         self.L = L
         if A is None:
-            A, B_unit = build_extrusion(self.W, self.m, self.dx, layer_L0[0])
+            A, B_unit = build_extrusion(self.W, self.m, self.dx, layer_L0[0],
+                                        far_points=far_points)
         self.A = np.asarray(A, np.float64)
         self.B = [np.asarray(B_unit, np.float64) * r ** (-5.0 / 6.0) for r in layer_r0]
         self.buf = np.zeros((L, self.cap, self.W))
@@ -759,7 +798,7 @@ class OracleExtruder:
             self.perp.append(p)
             self.along_min.append(float(a.min()))
             self.along_max.append(float(a.max()))
-            self.base[l] = int(math.floor(a.min() - fov[l])) - 2
+            self.base[l] = int(math.floor(a.min() - fov[l])) - 2 - self.extra[l]
         self.innov_count = np.zeros(L, np.int64)
         # boiling (extrude.py:586-654): S = S_unit r0^(-5/6), one stream
         self.S = None
@@ -778,9 +817,14 @@ class OracleExtruder:
         if self.fill[l] == self.cap:
             self.base[l], self.fill[l] = compact_row_ring(
                 self.buf[l], int(self.base[l]), int(self.fill[l]), self.m,
-                self.along_min[l] - self.fov[l] + self.travel[l])
+                self.along_min[l] - self.fov[l] + self.travel[l],
+                self.far_dist[-1] if self.far_dist else None)
         f = int(self.fill[l])
         z = self.buf[l, f - self.m:f].ravel()
+        if self.far_dist:
+            rows = [f - 1 - d for d in self.far_dist]
+            assert min(rows) >= 0, "far row behind the oldest ring row"
+            z = np.concatenate([z, self.buf[l][np.ix_(rows, self.far_cols)].ravel()])
         beta = np.asarray(innovation(l, int(self.innov_count[l])), np.float64)
         self.innov_count[l] += 1
         self.buf[l, f] = self.A @ z + self.B[l] @ beta

@@ -27,7 +27,7 @@ PT_ERR_UNSUPPORTED = 3
 PT_TUNE_GEN_ROWS = 0
 PT_TUNE_GEN_COLS = 1
 PT_TUNE_FRAMES = 2
-ABI_VERSION = 4  # PT_ABI_VERSION of include/pyturb_hip.h
+ABI_VERSION = 5  # PT_ABI_VERSION of include/pyturb_hip.h
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -53,6 +53,8 @@ SIGNATURES = {
     "pt_innovation_gemm": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "pt_extrude_rows": (_i, [_vp, _i, _i, _i, _vp, _vp, ctypes.c_longlong, _i, _vp, _vp, _vp,
                              _vp, _vp, _vp]),
+    "pt_extrude_rows_far": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, ctypes.c_longlong, _i, _vp, _vp,
+                                 _vp, _vp, _vp, _vp, _vp]),
     "pt_readout": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _f,
                         _i, _vp, _vp]),
     "pt_readout_adjoint_work_elems": (ctypes.c_longlong, [_i]),

@@ -126,7 +126,9 @@ class Atmosphere:
                  interp: str = "cubic", lgs_altitude: Optional[float] = None,
                  dispersion: Optional[str] = None, wet_fraction: float = 0.0,
                  device: str = "mi355x", dtype: str = "float32",
-                 seed: Optional[int] = None) -> None:
+                 seed: Optional[int] = None, far_points: int = 0) -> None:
+        if far_points and engine != "extrude":
+            raise ValueError("far_points applies to engine='extrude' only")
         config = AtmosphereConfig.create(
             layers, r0, seeing, wavelength, zenith_angle, diameter, n, L0, power_law,
             inner_scale, subharmonics, field_of_view, tau_boil, engine, interp,
@@ -253,7 +255,7 @@ class Atmosphere:
                                   else self.lgs_altitude * self.airmass),
                 device=self.device, dtype=self.dtype, seeds=ext_seeds,
                 tau_boil=(list(self.tau_boil) if config.has_tau_boil else None),
-                boil_seed=self._ext_boil_seed)
+                boil_seed=self._ext_boil_seed, far_points=far_points)
             self._ext = ExtrudedAtmosphere(**self._ext_kwargs)
 
     # ------------------------------------------------------------------

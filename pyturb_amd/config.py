@@ -183,11 +183,12 @@ class ExtrusionConfig:
     interp: str
     lgs_altitude_los: Optional[float]
     seeds: Tuple[Any, ...]
+    far_points: int = 0
 
     @classmethod
     def create(cls, n, pixel_scale, layer_r0, layer_L0, layer_wind, layer_altitude_los,
                field_of_view_pix, stencil_rows, interp, lgs_altitude_los, device, dtype,
-               seeds):
+               seeds, far_points=0):
         grid = GridConfig.create(n, pixel_scale, device, dtype)
         r0s = tuple(float(v) for v in layer_r0)
         L0s = tuple(float(v) for v in layer_L0)
@@ -230,5 +231,10 @@ class ExtrusionConfig:
             sd = tuple(seeds)
             if len(sd) != L:
                 raise ValueError("seeds must contain one value per layer")
+        # far stencil points (no reference counterpart): 8 per far row, <= 4 rows
+        if (not isinstance(far_points, (int, np.integer)) or far_points < 0
+                or far_points % 8 or far_points > 32):
+            raise ValueError("far_points must be 0, 8, 16, 24 or 32")
         return cls(grid, r0s, L0s, winds, alts, fov, int(stencil_rows), interp,
-                   None if lgs_altitude_los is None else float(lgs_altitude_los), sd)
+                   None if lgs_altitude_los is None else float(lgs_altitude_los), sd,
+                   int(far_points))

@@ -36,17 +36,22 @@ namespace pt {
 constexpr int VT = 16;         // vectors per MFMA column tile
 constexpr int VMAX = 2 * VT;   // vectors per launch
 constexpr int MMAX = 4;        // stencil rows (segments)
+constexpr int FAR_RMAX = 4;    // far stencil rows
+constexpr int FAR_COLS = 8;    // far points per far row
 
 struct GemvArgs {
   const float* M;
-  long long ldm;  // matrix row pitch (m * ld)
+  long long ldm;  // matrix row pitch (m * ld, + the far segment)
   int W, ld, m, nv;
-  int zw;  // ZV4 == false: valid floats per vector (4-byte aligned, no padding)
+  int zw;    // ZV4 == false: valid floats per vector (4-byte aligned, no padding)
+  int nfar;  // far points: matrix columns [m*ld, m*ld + nfar) (zero padded to 16)
   const float* v[VMAX][MMAX];
+  const float* far[VMAX][FAR_RMAX];  // far row r of vector e (ring rows)
   float* out[VMAX];
   const float* add[VMAX];
   float scale[VMAX];
 };
+static_assert(sizeof(GemvArgs) <= 4096, "kernel argument block");
 
 // C (W x nv) = M (W x m*ld) * Z (m*ld x nv) on the FP32 matrix cores
 // (v_mfma_f32_16x16x4f32: A lane l = M[l%16][l/16], B lane l = Z[l/16][l%16],
@@ -108,6 +113,44 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_mfma_gemv(GemvArgs a) {
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z.y, acc[j], 0, 0, 0);
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z.z, acc[j], 0, 0, 0);
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z.w, acc[j], 0, 0, 0);
+      }
+    }
+  }
+  // Far points (ring rows only): point q of the segment is column
+  // (2 (q % 8) + 1) W / 16 of far row q / 8.  In chunk c, lane quarter kq
+  // covers q = 16 c + 4 kq + {0..3}: far row 2 c + kq / 2, columns
+  // 4 (kq % 2) + {0..3}.
+  if (ZV4 && a.nfar > 0) {
+    const float* fz[NT][FAR_RMAX];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int r = 0; r < FAR_RMAX; ++r) {
+        fz[j][r] = nullptr;
+#pragma unroll
+        for (int e = 0; e < VT; ++e)
+          if (e == ri) fz[j][r] = a.far[j * VT + e][r];
+      }
+    const float* as = arow + (size_t)a.m * a.ld;
+    const int fch = (a.nfar + 15) >> 4;
+    for (int c = wave; c < fch; c += MF_WAVES) {
+      const int k = 16 * c + 4 * kq;
+      const float4 x = *(const float4*)(as + k);
+      const int r = 2 * c + (kq >> 1);
+      const int i0 = 4 * (kq & 1);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const bool live = j * VT + ri < a.nv && k < a.nfar;  // nfar % 8 == 0: whole quarters
+        float z[4] = {0.f, 0.f, 0.f, 0.f};
+        if (live) {
+          const float* fr = r == 0 ? fz[j][0] : r == 1 ? fz[j][1] : r == 2 ? fz[j][2] : fz[j][3];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) z[u] = fr[((2 * (i0 + u) + 1) * a.W) / (2 * FAR_COLS)];
+        }
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z[0], acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z[1], acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z[2], acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z[3], acc[j], 0, 0, 0);
       }
     }
   }
@@ -587,33 +630,44 @@ extern "C" int pt_innovation_gemm(const float* const* beta_rows, const float* B,
   return PT_OK;
 }
 
-extern "C" int pt_extrude_rows(const float* A, int width, int ld, int stencil_rows, const float* bb, float* buf,
-                               long long cap, int nact, const int* layer, const int* zrows, const int* trow,
-                               const int* bbrow, const float* bscale, void* stream) {
-  if (!A || !buf || !layer || !zrows || !trow || !bbrow || !bscale || !bb)
-    return fail(PT_ERR_ARG, "pt_extrude_rows: NULL array");
+static int extrude_rows_impl(const char* who, const float* A, int width, int ld, int stencil_rows, int far_points,
+                             const float* bb, float* buf, long long cap, int nact, const int* layer,
+                             const int* zrows, const int* far_rows, const int* trow, const int* bbrow,
+                             const float* bscale, void* stream) {
+  if (!A || !buf || !layer || !zrows || !trow || !bbrow || !bscale || !bb || (far_points && !far_rows))
+    return fail(PT_ERR_ARG, "%s: NULL array", who);
   if (width < 1 || ld < width || (ld & 3) || stencil_rows < 1 || cap < stencil_rows + 1 || nact < 0)
-    return fail(PT_ERR_ARG, "pt_extrude_rows: bad sizes (ld %% 4 == 0, ld >= width)");
-  if (stencil_rows > MMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_rows: stencil_rows > %d", MMAX);
-  if (!aligned16(A) || !aligned16(buf) || !aligned16(bb)) return fail(PT_ERR_ARG, "pt_extrude_rows: arrays must be 16-byte aligned");
+    return fail(PT_ERR_ARG, "%s: bad sizes (ld %% 4 == 0, ld >= width)", who);
+  if (stencil_rows > MMAX) return fail(PT_ERR_UNSUPPORTED, "%s: stencil_rows > %d", who, MMAX);
+  if (far_points < 0 || far_points % FAR_COLS || far_points > FAR_COLS * FAR_RMAX)
+    return fail(PT_ERR_ARG, "%s: far_points must be a multiple of %d in [0, %d]", who, FAR_COLS,
+                FAR_COLS * FAR_RMAX);
+  if (!aligned16(A) || !aligned16(buf) || !aligned16(bb)) return fail(PT_ERR_ARG, "%s: arrays must be 16-byte aligned", who);
+  const int R = far_points / FAR_COLS;
   hipStream_t st = (hipStream_t)stream;
   for (int e0 = 0; e0 < nact; e0 += VMAX) {
     GemvArgs a = {};
     a.M = A;
-    a.ldm = (long long)stencil_rows * ld;
+    a.ldm = (long long)stencil_rows * ld + ((far_points + 15) / 16) * 16;
     a.W = width;
     a.ld = ld;
     a.m = stencil_rows;
+    a.nfar = far_points;
     a.nv = nact - e0 < VMAX ? nact - e0 : VMAX;
     for (int e = 0; e < VMAX; ++e) {
       const int src = e0 + (e < a.nv ? e : 0);
       const size_t slab = (size_t)layer[src] * cap;
       for (int s = 0; s < MMAX; ++s) {
         const int r = zrows[src * stencil_rows + (s < stencil_rows ? s : 0)];
-        if (r < 0 || r >= cap) return fail(PT_ERR_ARG, "pt_extrude_rows: stencil row %d outside ring", r);
+        if (r < 0 || r >= cap) return fail(PT_ERR_ARG, "%s: stencil row %d outside ring", who, r);
         a.v[e][s] = buf + (slab + r) * ld;
       }
-      if (trow[src] < 0 || trow[src] >= cap) return fail(PT_ERR_ARG, "pt_extrude_rows: target row outside ring");
+      for (int j = 0; j < FAR_RMAX; ++j) {
+        const int r = R ? far_rows[src * R + (j < R ? j : 0)] : 0;
+        if (r < 0 || r >= cap) return fail(PT_ERR_ARG, "%s: far row %d outside ring", who, r);
+        a.far[e][j] = buf + (slab + r) * ld;
+      }
+      if (trow[src] < 0 || trow[src] >= cap) return fail(PT_ERR_ARG, "%s: target row outside ring", who);
       a.out[e] = buf + (slab + trow[src]) * ld;
       a.add[e] = bb + (size_t)bbrow[src] * ld;
       a.scale[e] = bscale[src];
@@ -622,6 +676,21 @@ extern "C" int pt_extrude_rows(const float* A, int width, int ld, int stencil_ro
     if (rc) return rc;
   }
   return PT_OK;
+}
+
+extern "C" int pt_extrude_rows(const float* A, int width, int ld, int stencil_rows, const float* bb, float* buf,
+                               long long cap, int nact, const int* layer, const int* zrows, const int* trow,
+                               const int* bbrow, const float* bscale, void* stream) {
+  return extrude_rows_impl("pt_extrude_rows", A, width, ld, stencil_rows, 0, bb, buf, cap, nact, layer, zrows,
+                           nullptr, trow, bbrow, bscale, stream);
+}
+
+extern "C" int pt_extrude_rows_far(const float* A, int width, int ld, int stencil_rows, int far_points,
+                                   const float* bb, float* buf, long long cap, int nact, const int* layer,
+                                   const int* zrows, const int* far_rows, const int* trow, const int* bbrow,
+                                   const float* bscale, void* stream) {
+  return extrude_rows_impl("pt_extrude_rows_far", A, width, ld, stencil_rows, far_points, bb, buf, cap, nact,
+                           layer, zrows, far_rows, trow, bbrow, bscale, stream);
 }
 
 // Geometry of one chunk of up to RD directions x RL layers: host-split

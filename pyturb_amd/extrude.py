@@ -15,7 +15,15 @@ exactly (host integers and float64); the data path is on the device:
 
 Innovations default to the reference's host stream (each layer's generator,
 after its seed screen); ``innovations=callable(layer, k) -> (k, W) device
-tensor`` injects them instead.  Boiling is not provided by this backend.
+tensor`` injects them instead.
+
+``far_points`` (0, 8, 16, 24 or 32; no reference counterpart) adds single
+pixels further back to the m full stencil rows: 8 per far row, far rows at
+(m - 1) + round(H^(j/R)) rows behind the newest stencil row (H = W // 2),
+columns (2 i + 1) W / 16.  They condition each new row on the large scales
+that two rows cannot see.  The ring keeps the farthest of them; with
+far_points = 0 everything is the reference's algorithm, bit for bit the
+output of a build without the option.
 """
 
 from __future__ import annotations
@@ -30,7 +38,8 @@ from . import _hip
 from .backend import resolve_device
 from .config import ExtrusionConfig
 from .fourier import PhaseScreen
-from .infinite import device_extrusion, device_seed_factor, extrude_rounds, readout, row_pitch
+from .infinite import (device_extrusion, device_seed_factor, extrude_rounds, far_rows_back, readout,
+                       row_pitch)
 from .ring import Ring
 
 __all__ = ["ExtrudedAtmosphere", "along_extent_and_span"]
@@ -62,10 +71,15 @@ class ExtrudedAtmosphere:
                  lgs_altitude_los: Optional[float] = None, device: str = "mi355x",
                  dtype: Union[str, np.dtype] = "float32", seeds=None, innovations=None,
                  seed_screens=None, tau_boil: Optional[Sequence[float]] = None,
-                 boil_seed: Optional[int] = None) -> None:
+                 boil_seed: Optional[int] = None, far_points: int = 0) -> None:
         config = ExtrusionConfig.create(n, pixel_scale, layer_r0, layer_L0, layer_wind,
                                         layer_altitude_los, field_of_view_pix, stencil_rows,
-                                        interp, lgs_altitude_los, device, dtype, seeds)
+                                        interp, lgs_altitude_los, device, dtype, seeds,
+                                        far_points)
+        if config.far_points and tau_boil is not None and any(
+                np.isfinite(float(v)) for v in tau_boil):
+            raise ValueError("far_points cannot be combined with boiling (tau_boil): fresh "
+                             "boiling windows are extruded from full stencil rows only")
         self.config = config
         self.n = config.grid.n
         self.dx = config.grid.pixel_scale
@@ -84,7 +98,9 @@ class ExtrudedAtmosphere:
         ext_span = [along_extent_and_span(self.n, vx, vy, mg, fv)
                     for (vx, vy), mg, fv in zip(config.layer_wind, mags, fovs)]
         self.width = int(np.ceil(2.0 * max(e for e, _ in ext_span) + 2.0 * max(fovs))) + 4
-        self.capacity = max(s for _, s in ext_span) + self.n + self.m + 32
+        self.far_points = config.far_points
+        far = far_rows_back(self.width, self.m, self.far_points) if self.far_points else ()
+        self.capacity = max(s for _, s in ext_span) + self.n + self.m + 32 + (far[-1] if far else 0)
         W, cap = self.width, self.capacity
         t = _hip.torch()
         self._buf = t.zeros((L, cap, row_pitch(W)), dtype=t.float32, device=self._dev)
@@ -107,7 +123,8 @@ class ExtrudedAtmosphere:
             ly.along_min = float(along.min())
             ly.along_max = float(along.max())
             span = (int(np.ceil(ly.along_max + ly.fov)) - int(np.floor(ly.along_min - ly.fov)) + 3)
-            ly.A, B = device_extrusion(W, self.m, self.dx, L0, 1.0, self._dev)
+            ly.A, B = device_extrusion(W, self.m, self.dx, L0, 1.0, self._dev,
+                                       far_points=self.far_points)
             ly.B = B
             ly.bscale = float(r0) ** (-5.0 / 6.0)
             ly.L0 = float(L0)
@@ -119,8 +136,12 @@ class ExtrudedAtmosphere:
                 gen = PhaseScreen(W, self.dx, float(r0), float(L0), seed=ly.rng,
                                   device=self.device, dtype=self.dtype.name)
                 init = gen.generate()
-            self._buf[i, :span, :W] = init[:span].float()
-            ly.ring = Ring(cap, self.m, int(np.floor(ly.along_min - ly.fov)) - 2, span)
+            # far rows need history behind the first new row: seed `extra`
+            # more rows of the seed screen below the reference's window
+            extra = max(0, far[-1] + 1 - span) if far else 0
+            self._buf[i, :span + extra, :W] = init[:span + extra].float()
+            ly.ring = Ring(cap, self.m, int(np.floor(ly.along_min - ly.fov)) - 2 - extra,
+                           span + extra, far)
             ly.travel = 0.0
             self.layers.append(ly)
         self._wind_cs = [(ly.cos, ly.sin) for ly in self.layers]
@@ -160,7 +181,7 @@ class ExtrudedAtmosphere:
             betas = self._betas([len(p) for p in plans])
             extrude_rounds([ly.A for ly in self.layers], [ly.B for ly in self.layers],
                            self._buf, self.capacity, self.width, self.m, plans, betas,
-                           [ly.bscale for ly in self.layers])
+                           [ly.bscale for ly in self.layers], far_points=self.far_points)
 
     def boil_step(self, dt: float) -> None:
         """One AR(1) blend of every boiling layer's readable window toward a

@@ -52,16 +52,60 @@ def _spd_solve(spd, rhs):
         return np.linalg.lstsq(spd, rhs, rcond=None)[0]
 
 
+FAR_COLUMNS = 8  # far points per far row
+
+
+def far_rows_back(width: int, stencil_rows: int, far_points: int) -> Tuple[int, ...]:
+    """Distances (in rows, behind the newest stencil row) of the far rows.
+
+    ``far_points`` = 8 R single pixels in R rows: row j = 1..R lies
+    ``(m - 1) + round(H^(j/R))`` rows behind the newest stencil row, with the
+    horizon H = width // 2 (strictly increasing), so the first far row is just
+    behind the full-row stencil and the last half a width back.  Each far row
+    holds the pixels at columns ``far_columns(width)``.
+    """
+    R = int(far_points) // FAR_COLUMNS
+    H = max(2, int(width) // 2)
+    m = int(stencil_rows)
+    out, prev = [], m - 1
+    for j in range(1, R + 1):
+        d = max(prev + 1, (m - 1) + int(round(H ** (j / R))))
+        out.append(d)
+        prev = d
+    return tuple(out)
+
+
+def far_columns(width: int) -> Tuple[int, ...]:
+    """Columns of the far points of a row: (2 i + 1) width // 16, i < 8."""
+    return tuple(((2 * i + 1) * int(width)) // (2 * FAR_COLUMNS) for i in range(FAR_COLUMNS))
+
+
+def stencil_points(width: int, stencil_rows: int, far_points: int = 0):
+    """(x, y) of the stencil: m full rows at y = 0..m-1 in row-major order,
+    then the far points, far row by far row (nearest first), 8 columns each;
+    the new row is at y = m."""
+    m, W = int(stencil_rows), int(width)
+    zy, zx = np.divmod(np.arange(m * W, dtype=np.float64), W)
+    if far_points:
+        cols = np.array(far_columns(W), dtype=np.float64)
+        ys = [(m - 1) - d for d in far_rows_back(W, m, far_points)]
+        zx = np.concatenate([zx, np.tile(cols, len(ys))])
+        zy = np.concatenate([zy, np.repeat(np.array(ys, dtype=np.float64), len(cols))])
+    return zx, zy
+
+
 def build_extrusion(width: int, stencil_rows: int, pixel_scale: float, L0: float,
-                    r0: float = 1.0) -> Tuple[np.ndarray, np.ndarray]:
+                    r0: float = 1.0, far_points: int = 0) -> Tuple[np.ndarray, np.ndarray]:
     """``(A, B)`` float64 for extruding one ``width``-pixel row.
 
-    Stencil rows at y = 0..m-1, new row at y = m; ``A = C_xz C_zz^-1``,
+    Stencil rows at y = 0..m-1 (plus ``far_points`` single pixels further
+    back, see ``stencil_points``), new row at y = m; ``A = C_xz C_zz^-1``,
     ``B = V sqrt(max(lambda, 0))`` of the symmetrised conditional covariance
-    (extrude.py:250-304, infinite.py:231-269).
+    (extrude.py:250-304, infinite.py:231-269; far points have no reference
+    counterpart and leave the construction unchanged).
     """
     m, W = int(stencil_rows), int(width)
-    zy, zx = np.divmod(np.arange(m * W, dtype=np.float64), W)  # row-major (y, x)
+    zx, zy = stencil_points(W, m, far_points)
     nx = np.arange(W, dtype=np.float64)
     ny = np.full(W, float(m))
 
@@ -87,22 +131,30 @@ def row_pitch(width: int) -> int:
     return (int(width) + 63) // 64 * 64
 
 
-def device_extrusion(width, stencil_rows, pixel_scale, L0, r0, dev):
+def far_pitch(far_points: int) -> int:
+    """Columns of A's far-point segment: far_points rounded up to 16."""
+    return (int(far_points) + 15) // 16 * 16
+
+
+def device_extrusion(width, stencil_rows, pixel_scale, L0, r0, dev, far_points=0):
     """(A, B) as float32 device tensors with pitched, zero-padded rows.
 
-    A is (W, m*ld) with stencil row s in columns [s*ld, s*ld + W); B is
+    A is (W, m*ld + far_pitch(F)) with stencil row s in columns
+    [s*ld, s*ld + W) and the F far points in [m*ld, m*ld + F); B is
     (W, ld).  Cached per geometry and device.
     """
     key = (int(width), int(stencil_rows), float(pixel_scale), round(float(L0), 9),
-           float(r0), str(dev))
+           float(r0), str(dev), int(far_points))
     hit = _MATRIX_CACHE.get(key)
     if hit is None:
-        W, m = int(width), int(stencil_rows)
+        W, m, F = int(width), int(stencil_rows), int(far_points)
         ld = row_pitch(W)
-        a, b = build_extrusion(W, m, pixel_scale, L0, r0)
-        ap = np.zeros((W, m * ld), dtype=np.float32)
+        a, b = build_extrusion(W, m, pixel_scale, L0, r0, far_points=F)
+        ap = np.zeros((W, m * ld + far_pitch(F)), dtype=np.float32)
         for s_ in range(m):
             ap[:, s_ * ld:s_ * ld + W] = a[:, s_ * W:(s_ + 1) * W]
+        if F:
+            ap[:, m * ld:m * ld + F] = a[:, m * W:]
         bp = np.zeros((W, ld), dtype=np.float32)
         bp[:, :W] = b
         hit = (_hip.to_device(ap, None, dev), _hip.to_device(bp, None, dev))
@@ -137,10 +189,11 @@ def device_seed_factor(width, stencil_rows, pixel_scale, L0, dev):
     return hit
 
 
-def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales):
+def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, far_points=0):
     """Run the planned rows of several layers on the device.
 
-    ``plans[l]``: list of (target_phys, z_phys) for layer l, in order;
+    ``plans[l]``: list of (target_phys, z_phys) -- or (target_phys, z_phys,
+    far_phys) with ``far_points`` > 0 -- for layer l, in order;
     ``betas[l]``: (k_l, width) float32 device innovations; ``bscales[l]``:
     r0^(-5/6) factor.  ``A``/``B`` may be per-layer lists (outer-scale
     groups) or shared tensors.
@@ -195,6 +248,14 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales):
             tr = (ctypes_int * n)(*[plans[l][r][0] for l in grp])
             br = (ctypes_int * n)(*[bbrow[l] + r for l in grp])
             sc = (ctypes_float * n)(*[float(bscales[l]) for l in grp])
+            if far_points:
+                R = len(plans[grp[0]][r][2])
+                fr = (ctypes_int * (n * R))(*[z for l in grp for z in plans[l][r][2]])
+                rc = lib.pt_extrude_rows_far(_hip.ptr(As[grp[0]]), width, ld, m, int(far_points),
+                                             _hip.ptr(bb_of[grp[0]]), _hip.ptr(buf), cap, n,
+                                             layer, zr, fr, tr, br, sc, st)
+                _hip.check(rc, "pt_extrude_rows_far")
+                continue
             rc = lib.pt_extrude_rows(_hip.ptr(As[grp[0]]), width, ld, m, _hip.ptr(bb_of[grp[0]]),
                                      _hip.ptr(buf), cap, n, layer, zr, tr, br, sc, st)
             _hip.check(rc, "pt_extrude_rows")

@@ -16,14 +16,20 @@ from typing import List, Tuple
 
 
 class Ring:
-    __slots__ = ("cap", "m", "base", "fill", "start")
+    """``far`` (optional): distances behind the newest stencil row of far
+    stencil rows; compaction then also keeps the farthest of them, and
+    ``plan`` lists their physical rows."""
 
-    def __init__(self, cap: int, stencil_rows: int, base: int, fill: int) -> None:
+    __slots__ = ("cap", "m", "base", "fill", "start", "far")
+
+    def __init__(self, cap: int, stencil_rows: int, base: int, fill: int,
+                 far: Tuple[int, ...] = ()) -> None:
         self.cap = int(cap)
         self.m = int(stencil_rows)
         self.base = int(base)
         self.fill = int(fill)
         self.start = 0
+        self.far = tuple(int(d) for d in far)
 
     def phys(self, local: int) -> int:
         return (self.start + local) % self.cap
@@ -32,6 +38,8 @@ class Ring:
         """ring.py:9-29: keep rows from min(floor(first)-1, last m) on."""
         target = int(floor(first_readable_row)) - 1
         stencil = self.base + self.fill - self.m
+        if self.far:  # the farthest far row of the next extrusion
+            stencil = min(stencil, self.base + self.fill - 1 - self.far[-1])
         keep_from = max(1, min(target, stencil) - self.base)
         self.start = (self.start + keep_from) % self.cap
         self.base += keep_from
@@ -40,7 +48,8 @@ class Ring:
     def plan(self, top_virtual: int, first_readable_row: float) -> List[Tuple[int, Tuple[int, ...]]]:
         """Rows to extrude until virtual row ``top_virtual`` exists.
 
-        Returns ``[(target_phys, (z_phys...)), ...]`` in order and advances
+        Returns ``[(target_phys, (z_phys...)), ...]`` in order -- with far
+        rows, ``(target_phys, (z_phys...), (far_phys...))`` -- and advances
         the bookkeeping (compacting when full, as the reference does before
         each extrusion).
         """
@@ -50,6 +59,11 @@ class Ring:
                 self.compact(first_readable_row)
             f = self.fill
             z = tuple(self.phys(f - self.m + k) for k in range(self.m))
-            rows.append((self.phys(f), z))
+            if self.far:
+                if f - 1 - self.far[-1] < 0:
+                    raise RuntimeError("far stencil row behind the ring's oldest row")
+                rows.append((self.phys(f), z, tuple(self.phys(f - 1 - d) for d in self.far)))
+            else:
+                rows.append((self.phys(f), z))
             self.fill = f + 1
         return rows

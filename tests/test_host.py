@@ -60,7 +60,7 @@ def test_library_loads_and_exports_every_header_symbol():
     for name in syms:
         assert hasattr(so, name), name
         assert name in _hip.SIGNATURES, name
-    assert _hip.lib().pt_abi_version() == _hip.ABI_VERSION == 4
+    assert _hip.lib().pt_abi_version() == _hip.ABI_VERSION == 5
     assert "pt_work_elems" in syms
     # host-only sizing: power-of-two fused path vs general path
     # frames: pairs of frames share one transform, plus the folded stack, its

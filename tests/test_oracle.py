@@ -200,6 +200,73 @@ def test_extrusion_matrices_identities_and_golden():
     np.testing.assert_allclose(B @ B.T, cxx - A @ cxz.T, atol=1e-8 * np.abs(cxx).max())
 
 
+def test_far_stencil_geometry_known_values():
+    """The far points' definition: R = F / 8 far rows at (m - 1) +
+    round((W // 2)^(j / R)) rows behind the newest stencil row, 8 columns
+    (2 i + 1) W // 16 each; values worked by hand from that definition."""
+    dist, cols = O.far_stencil(2194, 2, 32)  # H = 1097: 1097^(1/4) = 5.76, ^(1/2) = 33.1, ^(3/4) = 190.7
+    assert dist == [7, 34, 192, 1098]
+    assert cols == [137, 411, 685, 959, 1234, 1508, 1782, 2056]
+    assert O.far_stencil(40, 2, 32)[0] == [3, 5, 10, 21]  # H = 20: 2.11, 4.47, 9.46, 20
+    assert O.far_stencil(40, 3, 16)[0] == [6, 22]  # m - 1 = 2: 2 + round(4.47), 2 + 20
+    assert O.far_stencil(10, 2, 32)[0] == [2, 3, 4, 6]  # H = 5: 1.50 -> 1, 2.24 -> 2, 3.34 -> 3, 5
+    assert O.far_stencil(40, 2, 0)[0] == []
+    # the product's host geometry is the same definition
+    from pyturb_amd.infinite import far_columns, far_rows_back
+    for W, m, F in ((2194, 2, 32), (40, 2, 32), (40, 3, 16), (10, 2, 32), (68, 2, 24)):
+        d, c = O.far_stencil(W, m, F)
+        assert list(far_rows_back(W, m, F)) == d
+        assert list(far_columns(W)) == c
+
+
+def test_far_point_extrusion_identities():
+    """With far points, A C_zz = C_xz and B B^T = C_xx - A C_xz^T over the
+    full stencil (m rows, then the far points row by row); the extra data can
+    only lower the conditional variance; far_points = 0 is the plain build;
+    the product's FP64 build is the same."""
+    W, m, F, dx, L0 = 24, 2, 16, 0.1, 25.0
+    A, B = O.build_extrusion(W, m, dx, L0, far_points=F)
+    assert A.shape == (W, m * W + F) and B.shape == (W, W)
+    dist, cols = O.far_stencil(W, m, F)
+    pts = [(x, y) for y in range(m) for x in range(W)]
+    pts += [(x, m - 1 - d) for d in dist for x in cols]
+    zx, zy = np.array(pts, float).T
+    nx, ny = np.arange(W, dtype=float), np.full(W, float(m))
+    cov = lambda ax, ay, bx, by: O.phase_covariance(
+        np.hypot(ax[:, None] - bx[None], ay[:, None] - by[None]) * dx, 1.0, L0)
+    czz, cxz, cxx = cov(zx, zy, zx, zy), cov(nx, ny, zx, zy), cov(nx, ny, nx, ny)
+    np.testing.assert_allclose(A @ czz, cxz, atol=1e-6 * np.abs(cxz).max())
+    np.testing.assert_allclose(B @ B.T, cxx - A @ cxz.T, atol=1e-8 * np.abs(cxx).max())
+    A0, B0 = O.build_extrusion(W, m, dx, L0)
+    assert np.trace(B @ B.T) < np.trace(B0 @ B0.T)
+    A00, B00 = O.build_extrusion(W, m, dx, L0, far_points=0)
+    assert np.array_equal(A00, A0) and np.array_equal(B00, B0)
+    from pyturb_amd.infinite import build_extrusion as product_build
+    Ap, Bp = product_build(W, m, dx, L0, far_points=F)
+    np.testing.assert_allclose(Ap, A, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(Bp @ Bp.T, B @ B.T, rtol=1e-9, atol=1e-12)
+
+
+def test_far_point_extruder_keeps_far_rows_and_big_jump_equals_small_steps():
+    """The far rows stay in the ring through compactions (the oracle asserts
+    each one is there), and one jump equals many small steps."""
+    def run(times):
+        ext = O.OracleExtruder(n=16, dx=0.5, layer_r0=[0.2, 0.3], layer_L0=[20.0, 20.0],
+                               layer_wind=[(7.0, 3.0), (-2.0, -9.0)], layer_alt=[0.0, 3000.0],
+                               far_points=32)
+        rng = np.random.default_rng(4)
+        ext.seed([rng.standard_normal((s, ext.W)) for s in ext.spans])
+        streams = [np.random.default_rng(100 + l) for l in range(2)]
+        innov = lambda l, k: streams[l].standard_normal(ext.W)
+        for t in times:
+            ext.set_time(t, innov)
+        return ext.integrate(), ext
+    big, ext = run([20.0])
+    small, _ = run(np.linspace(0, 20.0, 101))
+    assert ext.innov_count.min() > 3 * ext.cap  # several compactions
+    np.testing.assert_allclose(big, small, rtol=0, atol=1e-12 * 40)
+
+
 # ---------------------------------------------------------------- extruder
 def _reference_like_extruder(interp_seeds=(11, 12)):
     """Oracle extruder fed with the reference's own seed screens/innovations."""
