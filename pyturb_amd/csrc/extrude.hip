@@ -433,150 +433,200 @@ __global__ __launch_bounds__(256) void k_readout(ReadoutArgs a) {
 }
 
 constexpr int TB = 32;  // tile edge
-// Box edge per tap count: for magnification <= 1 (every NGS direction and
-// LGS cone) a tile's footprint spans at most floor(31 sqrt 2) = 43 rows and
-// columns between its extreme pixels, plus the taps and a spare on each
-// side (box_geom): 48 / 50 / 52.  A larger footprint takes the direct
-// gather.  Smaller boxes leave LDS for more workgroups per CU (bilinear:
-// 18.8 KiB, 8 workgroups per CU, against 5 with 60-wide boxes).
-template <int TAPS>
-struct BoxCfg {
-  static constexpr int E = 46 + TAPS;
-  static constexpr int ROWS = (E + 3) / 4;  // box rows per wave
-};
 
-// Footprint box of one (layer, direction) for a tile: local ring rows rb0..
-// and buffer columns cb0.. (before clamping), extents nr x nc.
-struct BoxGeom {
-  int rb0, cb0, nr, nc;
+// Geometry of the tiled readout, per (direction, layer), in FP64:
+//   row(i, j) = R0 + i P + j Q,   col(i, j) = C0 - i Q + j P,
+// P = mag cos, Q = mag sin, R0 = sa - h (P + Q), C0 = W/2 + sp + h (Q - P),
+// h = (n - 1) / 2 -- the reference's along/perp grids plus the shifts
+// (extrude.py:376-383, 807-822).  Each workgroup evaluates its tile origin in
+// FP64 and the pixels relative to it in FP32 (offsets < 32 px), so a
+// coordinate is exact to ~1e-6 px at any grid size.
+constexpr int TRL = 12, TRD = 8;  // layers, directions per launch
+struct TileArgs {
+  const float* buf;  // (L, cap, ld)
+  float* out;        // (D, n, n)
+  int cap, W, ld, n, L, D, accumulate;
+  float scale;
+  int start[TRL];
+  int fillm1[TRL];
+  double R0[TRD][TRL], C0[TRD][TRL], P[TRD][TRL], Q[TRD][TRL];
+};
+static_assert(sizeof(TileArgs) <= 4096, "kernel argument block");
+
+// Box of a tile's footprint in one layer: local ring rows r0.. and buffer
+// columns c0.. (before clamping), extents nr x nc; (fr, fc) is the tile
+// origin pixel relative to the box origin.
+struct TileGeo {
+  int r0, c0, nr, nc;
+  float fr, fc, p, q;
   bool fits;
 };
 
+// Box edge per tap count: for magnification <= 1 (every NGS direction and
+// LGS cone) a tile's footprint spans at most floor(31 sqrt 2) = 43 rows and
+// columns between its extreme pixels, plus the taps and a spare on each
+// side: 48 / 50 / 52 rows, one lane per column.  A larger footprint takes
+// the direct gather.
 template <int TAPS>
-__device__ __forceinline__ BoxGeom box_geom(const ReadoutArgs& a, int d, int l, int i0, int j0, int ti1, int tj1,
-                                            float h) {
-  constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
-  const float c = a.cosw[l], s = a.sinw[l];
-  const float mg = a.mag[d][l], rfr = a.r_frac[d][l], cfr = a.c_frac[d][l];
-  float rmin = 3e38f, rmax = -3e38f, cmin = 3e38f, cmax = -3e38f;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {  // affine map: the extremes are at the corners
-    const float gi = ((float)(q & 1 ? ti1 : i0) - h) * mg, gj = ((float)(q & 2 ? tj1 : j0) - h) * mg;
-    const float r = fmaf(gi, c, fmaf(gj, s, rfr)), cc = fmaf(-gi, s, fmaf(gj, c, cfr));
-    rmin = fminf(rmin, r);
-    rmax = fmaxf(rmax, r);
-    cmin = fminf(cmin, cc);
-    cmax = fmaxf(cmax, cc);
-  }
-  // one spare row/column each side absorbs last-bit rounding of interior pixels
-  BoxGeom b;
-  b.rb0 = (int)floorf(rmin) + OFF - 1;
-  b.cb0 = (int)floorf(cmin) + OFF - 1;
-  b.nr = (int)floorf(rmax) + OFF + TAPS + 1 - b.rb0;
-  b.nc = (int)floorf(cmax) + OFF + TAPS + 1 - b.cb0;
-  b.fits = b.nr <= BoxCfg<TAPS>::E && b.nc <= BoxCfg<TAPS>::E;
-  return b;
+struct TileCfg {
+  static constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
+  static constexpr int BH = 46 + TAPS;       // box rows
+  static constexpr int BW = 64;              // box columns (lanes)
+  static constexpr int BP = BW + 1;          // LDS pitch
+  static constexpr int KR = (BH + 3) / 4;    // box rows per wave
+};
+
+template <int TAPS>
+__device__ __forceinline__ TileGeo tile_geo(const TileArgs& a, int d, int l, int i0, int j0) {
+  constexpr int OFF = TileCfg<TAPS>::OFF;
+  const double P = a.P[d][l], Q = a.Q[d][l];
+  const double rb = fma((double)i0, P, fma((double)j0, Q, a.R0[d][l]));
+  const double cb = fma((double)i0, -Q, fma((double)j0, P, a.C0[d][l]));
+  TileGeo g;
+  g.p = (float)P;
+  g.q = (float)Q;
+  // the map is affine: a full tile's extremes are at its corners, offsets
+  // 31 (min / max of 0 and 31 p, 31 q); an edge tile uses the same box
+  constexpr float E = (float)(TB - 1);
+  const float erl = E * (fminf(g.p, 0.f) + fminf(g.q, 0.f)), erh = E * (fmaxf(g.p, 0.f) + fmaxf(g.q, 0.f));
+  const float ecl = E * (fminf(-g.q, 0.f) + fminf(g.p, 0.f)), ech = E * (fmaxf(-g.q, 0.f) + fmaxf(g.p, 0.f));
+  // one spare row/column each side absorbs the FP32 rounding of interior pixels
+  g.r0 = (int)floor(rb + (double)erl) + OFF - 1;
+  g.c0 = (int)floor(cb + (double)ecl) + OFF - 1;
+  g.nr = (int)floor(rb + (double)erh) + OFF + TAPS + 1 - g.r0;
+  g.nc = (int)floor(cb + (double)ech) + OFF + TAPS + 1 - g.c0;
+  g.fits = g.nr <= TileCfg<TAPS>::BH && g.nc <= TileCfg<TAPS>::BW;
+  g.fr = (float)(rb - (double)g.r0);
+  g.fc = (float)(cb - (double)g.c0);
+  return g;
 }
 
-// Box-staged readout for n >= 32: a workgroup owns a 32 x 32 pixel tile of
-// one direction.  For each layer the tile's footprint in the ring buffer
-// (an affine image of the tile) plus the tap margins is staged in LDS with
-// coalesced row reads, rows clamped to the valid ring rows and columns to
-// [0, W-1] exactly as the per-tap clamp does; the taps are then read from
-// LDS.  The box of layer l + 1 is loaded into registers while layer l is
-// interpolated (two LDS buffers), so the gather latency overlaps the
-// arithmetic.  A footprint larger than the box (magnification > 1) falls
-// back to the direct gather for that layer.
+// Tiled readout for n >= 32: a workgroup owns a 32 x 32 pixel tile of one
+// direction.  For each layer the tile's footprint in the ring buffer plus
+// the tap margins is staged in LDS (one lane per column, coalesced rows;
+// rows clamped to the valid ring rows and columns to [0, W - 1] exactly as
+// the per-tap clamp does), and the taps are read from LDS.  The box of layer
+// l + 1 is loaded into registers while layer l is interpolated (two LDS
+// buffers, one barrier per layer).  A footprint larger than the box
+// (magnification > 1) takes the direct gather for that layer.
+//  * Workgroups run in XCD-contiguous order over (tile, direction),
+//    direction fastest: one XCD takes a band of neighbouring tiles with all
+//    their directions, whose footprints (overlapping box margins, the low
+//    layers' near-identical boxes across directions) meet in its L2.
+//  * Staging: lane k of each wave works out the ring offset of the wave's
+//    box row k once per layer; the loads take it with v_readlane, so a box
+//    row costs one address add.  Every lane loads (columns past the box are
+//    clamped reads of the row), no lane masking.
 template <int TAPS>
-__global__ __launch_bounds__(256) void k_readout_box(ReadoutArgs a) {
-  constexpr int OFF = TAPS == 2 ? 0 : (TAPS == 4 ? -1 : -2);
-  constexpr int BOXE = BoxCfg<TAPS>::E, BOX_ROWS = BoxCfg<TAPS>::ROWS;
-  __shared__ float box[2][BOXE][BOXE + 1];
+__global__ __launch_bounds__(256) void k_readout_tile(TileArgs a, int tiles_x, int ntiles) {
+  using C = TileCfg<TAPS>;
+  constexpr int OFF = C::OFF, BP = C::BP, KR = C::KR;
+  static_assert(KR <= 64, "one lane per box row of a wave");
+  __shared__ float box[2][C::BH * BP];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int i0 = blockIdx.y * TB, j0 = blockIdx.x * TB;
-  const int jl = tid & 31, il = tid >> 5;  // pixel (i0 + il + 8k, j0 + jl), k < 4
-  const float h = 0.5f * (float)(a.n - 1);
-  const int ti1 = min(i0 + TB, a.n) - 1, tj1 = min(j0 + TB, a.n) - 1;
-  const int d = blockIdx.z;  // one direction per block: short dependency chains
+  const int t = xcd_tile((int)blockIdx.x, ntiles);
+  const int d = t % a.D, tile = t / a.D;
+  const int i0 = (tile / tiles_x) * TB, j0 = (tile % tiles_x) * TB;
+  const int jl = tid & 31, il = tid >> 5;  // pixels (i0 + il + 8k, j0 + jl), k < 4
+  const float fj = (float)jl;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  float pre[BOX_ROWS];
-  // box loads of layer l into registers (nothing when it does not fit)
-  auto fetch = [&](int l, const BoxGeom& b) {
-    if (!b.fits || lane >= b.nc) return;
-    const float* bl = a.buf + (size_t)l * a.cap * a.ld;
-    const int fm = a.fillm1[l], st = a.start[l];
-    const int cc = min(max(b.cb0 + a.c_int[d][l] + lane, 0), a.W - 1);
+  float pre[KR];
+  const int wv = __builtin_amdgcn_readfirstlane(wave);  // wave-uniform box-row tests
+  auto fetch = [&](int l, const TileGeo& g) {
+    if (!g.fits) return;
+    // the slab is < 2^30 floats: 32-bit byte offsets from its base
+    const char* bl = (const char*)(a.buf + (size_t)l * a.cap * a.ld);
+    const int fm = a.fillm1[l];
+    const unsigned ph = (unsigned)(a.start[l] + min(max(g.r0 + wave + 4 * min(lane, KR - 1), 0), fm));
+    const unsigned rowoff = __umul24(min(ph, ph - (unsigned)a.cap), (unsigned)a.ld) << 2;  // ph mod cap
+    const unsigned coloff = (unsigned)min(max(g.c0 + lane, 0), a.W - 1) << 2;
 #pragma unroll
-    for (int k = 0; k < BOX_ROWS; ++k) {
-      const int br = wave + 4 * k;
-      if (br < b.nr) {
-        int ph = st + min(max(b.rb0 + a.r_int[d][l] + br, 0), fm);
-        ph = ph >= a.cap ? ph - a.cap : ph;
-        pre[k] = bl[(size_t)ph * a.ld + cc];
+    for (int k = 0; k < KR; ++k) {
+      if (wv + 4 * k < g.nr) {
+        const unsigned ro = (unsigned)__builtin_amdgcn_readlane((int)rowoff, k);
+        pre[k] = *(const float*)(bl + (ro + coloff));
       }
     }
   };
-  auto commit = [&](int sel, const BoxGeom& b) {
-    if (!b.fits || lane >= b.nc) return;
+  auto commit = [&](int sel, const TileGeo& g) {
+    if (!g.fits) return;
 #pragma unroll
-    for (int k = 0; k < BOX_ROWS; ++k) {
-      const int br = wave + 4 * k;
-      if (br < b.nr) box[sel][br][lane] = pre[k];
+    for (int k = 0; k < KR; ++k) {
+      if (wv + 4 * k < g.nr) box[sel][(wave + 4 * k) * BP + lane] = pre[k];
     }
   };
-  BoxGeom cur = box_geom<TAPS>(a, d, 0, i0, j0, ti1, tj1, h);
+  TileGeo cur = tile_geo<TAPS>(a, d, 0, i0, j0);
   fetch(0, cur);
   commit(0, cur);
   __syncthreads();
   for (int l = 0; l < a.L; ++l) {
     const int sel = l & 1;
-    BoxGeom nxt;
+    TileGeo nxt;
     nxt.fits = false;
     if (l + 1 < a.L) {
-      nxt = box_geom<TAPS>(a, d, l + 1, i0, j0, ti1, tj1, h);
+      nxt = tile_geo<TAPS>(a, d, l + 1, i0, j0);
       fetch(l + 1, nxt);  // in flight while layer l is interpolated
     }
-    const float c = a.cosw[l], s = a.sinw[l];
-    const float mg = a.mag[d][l], rfr = a.r_frac[d][l], cfr = a.c_frac[d][l];
-    const int rint = a.r_int[d][l], cint = a.c_int[d][l];
-    const float* bl = a.buf + (size_t)l * a.cap * a.ld;
-    const int fm = a.fillm1[l], st = a.start[l];
+    // per-thread row terms: row = rj + di p, col = cj - di q
+    const float rj = fmaf(fj, cur.q, cur.fr), cj = fmaf(fj, cur.p, cur.fc);
+    if (cur.fits) {
+      const float* bx = box[sel];
+      // coordinates kept inside the box (a guard: interior pixels are at
+      // least one spare row/column from its edges)
+      const float rhi = (float)(cur.nr - TAPS - OFF), chi = (float)(cur.nc - TAPS - OFF);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int i = i0 + il + 8 * k, j = j0 + jl;
-      if (i >= a.n || j >= a.n) continue;
-      const float gi = ((float)i - h) * mg, gj = ((float)j - h) * mg;
-      const float row = fmaf(gi, c, fmaf(gj, s, rfr));
-      const float col = fmaf(-gi, s, fmaf(gj, c, cfr));
-      const float rf = floorf(row), cf = floorf(col);
-      float wr[TAPS], wc[TAPS];
-      taps_weights<TAPS>(row - rf, wr);
-      taps_weights<TAPS>(col - cf, wc);
-      float val = 0.f;
-      if (cur.fits) {
-        const int br = min(max((int)rf + OFF - cur.rb0, 0), cur.nr - TAPS);
-        const int bc = min(max((int)cf + OFF - cur.cb0, 0), cur.nc - TAPS);
+      for (int k = 0; k < 4; ++k) {
+        const float di = (float)(il + 8 * k);
+        const float row = __builtin_amdgcn_fmed3f(fmaf(di, cur.p, rj), (float)-OFF, rhi);
+        const float col = __builtin_amdgcn_fmed3f(fmaf(-di, cur.q, cj), (float)-OFF, chi);
+        const float rf = floorf(row), cf = floorf(col);
+        const float* bp = bx + ((int)fmaf(rf, (float)BP, cf) + OFF * (BP + 1));  // tap (0, 0)
+        float val;
+        if constexpr (TAPS == 2) {
+          const float tr = row - rf, tc = col - cf;
+          const float top = fmaf(tc, bp[1] - bp[0], bp[0]);
+          const float bot = fmaf(tc, bp[BP + 1] - bp[BP], bp[BP]);
+          val = fmaf(tr, bot - top, top);
+        } else {
+          float wr[TAPS], wc[TAPS];
+          taps_weights<TAPS>(row - rf, wr);
+          taps_weights<TAPS>(col - cf, wc);
+          val = 0.f;
 #pragma unroll
-        for (int u = 0; u < TAPS; ++u) {
-          float rs = 0.f;
+          for (int u = 0; u < TAPS; ++u) {
+            float rs = 0.f;
 #pragma unroll
-          for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], box[sel][br + u][bc + v], rs);
-          val = fmaf(wr[u], rs, val);
+            for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], bp[u * BP + v], rs);
+            val = fmaf(wr[u], rs, val);
+          }
         }
-      } else {
-        const int r0 = (int)rf + rint, c0 = (int)cf + cint;
+        acc[k] += val;
+      }
+    } else {  // direct gather with the per-tap clamps
+      const float* bl = a.buf + (size_t)l * a.cap * a.ld;
+      const int fm = a.fillm1[l], st = a.start[l];
+#pragma unroll 1
+      for (int k = 0; k < 4; ++k) {
+        const float di = (float)(il + 8 * k);
+        const float row = fmaf(di, cur.p, rj), col = fmaf(-di, cur.q, cj);
+        const float rf = floorf(row), cf = floorf(col);
+        const int r0 = (int)rf + cur.r0, c0 = (int)cf + cur.c0;
+        float wr[TAPS], wc[TAPS];
+        taps_weights<TAPS>(row - rf, wr);
+        taps_weights<TAPS>(col - cf, wc);
+        float val = 0.f;
 #pragma unroll
         for (int u = 0; u < TAPS; ++u) {
           int ph = st + min(max(r0 + OFF + u, 0), fm);
           ph = ph >= a.cap ? ph - a.cap : ph;
-          const float* brow = bl + (size_t)ph * a.ld;
+          const float* brow = bl + (unsigned)ph * (unsigned)a.ld;
           float rs = 0.f;
 #pragma unroll
           for (int v = 0; v < TAPS; ++v) rs = fmaf(wc[v], brow[min(max(c0 + OFF + v, 0), a.W - 1)], rs);
           val = fmaf(wr[u], rs, val);
         }
+        acc[k] += val;
       }
-      acc[k] += val;
     }
     if (l + 1 < a.L) {
       // buffer sel ^ 1 was last read in iteration l - 1, before the previous barrier
@@ -745,6 +795,44 @@ static int readout_check(const char* who, const float* buf, int layers, long lon
   return PT_OK;
 }
 
+// Geometry of one chunk of up to TRD directions x TRL layers for the tiled
+// readout (FP64 affine coefficients, see TileArgs).
+static int pack_tile(TileArgs& a, const float* buf, int layers, long long cap, int width, int ld, const int* start,
+                     const long long* fill, const double* wind_cs, const double* dir_geom, int dirs, int n, int d0,
+                     int l0) {
+  a.buf = buf + (size_t)l0 * cap * ld;
+  a.out = nullptr;
+  a.cap = (int)cap;
+  a.W = width;
+  a.ld = ld;
+  a.n = n;
+  a.L = layers - l0 < TRL ? layers - l0 : TRL;
+  a.D = dirs - d0 < TRD ? dirs - d0 : TRD;
+  a.accumulate = 0;
+  a.scale = 1.f;
+  const double h = 0.5 * (double)(n - 1);
+  for (int l = 0; l < TRL; ++l) {
+    const int src = l0 + (l < a.L ? l : 0);
+    a.start[l] = start[src];
+    a.fillm1[l] = (int)(fill[src] - 1);
+  }
+  for (int d = 0; d < TRD; ++d)
+    for (int l = 0; l < TRL; ++l) {
+      const int sd = d0 + (d < a.D ? d : 0), sl = l0 + (l < a.L ? l : 0);
+      const double* gp = dir_geom + ((size_t)sd * layers + sl) * 3;
+      const double c = wind_cs[2 * sl], s = wind_cs[2 * sl + 1], mg = gp[2];
+      const double P = mg * c, Q = mg * s;
+      const double R0 = gp[0] - h * (P + Q), C0 = 0.5 * width + gp[1] + h * (Q - P);
+      if (!(fabs(R0) <= 1e9) || !(fabs(C0) <= 1e9) || !(fabs(P) <= 1e3) || !(fabs(Q) <= 1e3))
+        return fail(PT_ERR_ARG, "readout: geometry out of range");
+      a.R0[d][l] = R0;
+      a.C0[d][l] = C0;
+      a.P[d][l] = P;
+      a.Q[d][l] = Q;
+    }
+  return PT_OK;
+}
+
 extern "C" int pt_readout(const float* buf, int layers, long long cap, int width, int ld, const int* start,
                           const long long* fill, const double* wind_cs, const double* dir_geom, int dirs, int n,
                           int interp, float scale, int accumulate, float* out, void* stream) {
@@ -752,6 +840,27 @@ extern "C" int pt_readout(const float* buf, int layers, long long cap, int width
   int rc = readout_check("pt_readout", buf, layers, cap, width, ld, start, fill, wind_cs, dir_geom, dirs, n, interp);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
+  if (n >= TB) {
+    for (int d0 = 0; d0 < dirs; d0 += TRD) {
+      for (int l0 = 0; l0 < layers; l0 += TRL) {
+        TileArgs a;
+        rc = pack_tile(a, buf, layers, cap, width, ld, start, fill, wind_cs, dir_geom, dirs, n, d0, l0);
+        if (rc) return rc;
+        a.out = out + (size_t)d0 * n * n;
+        a.accumulate = (accumulate || l0 > 0) ? 1 : 0;
+        a.scale = scale;
+        const int tx = (n + TB - 1) / TB, nt = tx * tx * a.D;
+        if (interp == 0)
+          k_readout_tile<2><<<nt, 256, 0, st>>>(a, tx, nt);
+        else if (interp == 1)
+          k_readout_tile<4><<<nt, 256, 0, st>>>(a, tx, nt);
+        else
+          k_readout_tile<6><<<nt, 256, 0, st>>>(a, tx, nt);
+        PT_LAUNCH_CHECK();
+      }
+    }
+    return PT_OK;
+  }
   for (int d0 = 0; d0 < dirs; d0 += RD) {
     for (int l0 = 0; l0 < layers; l0 += RL) {
       ReadoutArgs a;
@@ -761,21 +870,12 @@ extern "C" int pt_readout(const float* buf, int layers, long long cap, int width
       a.accumulate = (accumulate || l0 > 0) ? 1 : 0;
       a.scale = scale;
       const dim3 g((n + 31) / 32, (n + 7) / 8, a.D);
-      if (n >= TB) {
-        const dim3 gb((n + TB - 1) / TB, (n + TB - 1) / TB, a.D);
-        if (interp == 0)
-          k_readout_box<2><<<gb, 256, 0, st>>>(a);
-        else if (interp == 1)
-          k_readout_box<4><<<gb, 256, 0, st>>>(a);
-        else
-          k_readout_box<6><<<gb, 256, 0, st>>>(a);
-      } else if (interp == 0) {
+      if (interp == 0)
         k_readout<2><<<g, 256, 0, st>>>(a);
-      } else if (interp == 1) {
+      else if (interp == 1)
         k_readout<4><<<g, 256, 0, st>>>(a);
-      } else {
+      else
         k_readout<6><<<g, 256, 0, st>>>(a);
-      }
       PT_LAUNCH_CHECK();
     }
   }
