@@ -92,13 +92,32 @@ struct GenArgs {
   float scale;
 };
 
+// At n = 1024 the subharmonic staging block (ColSub) lives in the
+// transform tile, which is free once the last stage has read it, so the
+// basis at the base rows can be staged there too (it used to be evaluated
+// per thread and level, a third of that column pass's VALU work).  Smaller
+// sizes keep the block beside the tile.
+template <int N>
+constexpr bool gen_cols_alias() { return N == 1024; }
+template <int N, int T, int M>
+using GenColSub = ColSub<N, M, 2, T, (N <= 1024)>;
+template <int N, int T, int M>
+constexpr size_t gen_cols_smem() {
+  constexpr size_t tile = (size_t)Layout<N, T, M, true>::SIZE * sizeof(float2);
+  constexpr size_t sub = sizeof(GenColSub<N, T, M>);
+  return gen_cols_alias<N>() ? (tile > sub ? tile : sub) : tile + sub;
+}
+
 template <int N, bool ACC, int TC = kColsT, bool XCD = true>
 __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
   constexpr int T = ColsCfg<N, TC>::T, M = ColsCfg<N, TC>::M;
-  __shared__ float2 lds[Layout<N, T, M, true>::SIZE];
+  constexpr bool ALIAS = gen_cols_alias<N>();
+  constexpr size_t TILE_B = (size_t)Layout<N, T, M, true>::SIZE * sizeof(float2);
+  using Sub = GenColSub<N, T, M>;
+  __shared__ __align__(16) unsigned char smem[gen_cols_smem<N, T, M>()];
   __shared__ float2 tw[N];
-  using Sub = ColSub<N, M, 2, T>;
-  __shared__ Sub sub;
+  float2* lds = reinterpret_cast<float2*>(smem);
+  Sub& sub = *reinterpret_cast<Sub*>(smem + (ALIAS ? 0 : TILE_B));
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
   const int c0 = (XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * M;
@@ -125,7 +144,10 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
     ++cnt;
   };
   wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
-  if (P > 0) sub.commit(pre, P, tid);
+  if (P > 0) {
+    if (ALIAS) __syncthreads();  // every lane has read its last-stage inputs from the tile
+    sub.commit(pre, P, tid);
+  }
   __syncthreads();
   constexpr int RL = Plan<N>::radix(Plan<N>::NST - 1);
   const int m = tid % M, x0 = tid / M;

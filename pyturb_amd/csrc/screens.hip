@@ -827,16 +827,12 @@ static int set_dyn_lds(K kernel, size_t bytes) {
   return PT_OK;
 }
 
-template <int N>
-static size_t cols_static_lds() {
-  return (size_t)Layout<N, ColsCfg<N>::T, ColsCfg<N>::M, true>::SIZE * 8 + (size_t)N * 8;
-}
-
 // Fused column epilogues keep the per-column subharmonic terms in registers
-// (up to PREG levels), staged through a small LDS block (ColSub).
+// (up to PREG levels), staged through a small LDS block (ColSub) that shares
+// the transform tile.
 template <int N>
 static bool cols_fused_ok(int levels) {
-  return levels <= PREG && cols_static_lds<N>() + sizeof(ColSub<N, ColsCfg<N>::M, 2, ColsCfg<N>::T>) <= kLdsMax;
+  return levels <= PREG && gen_cols_smem<N, ColsCfg<N>::T, ColsCfg<N>::M>() + (size_t)N * 8 <= kLdsMax;
 }
 
 template <int N>

@@ -132,12 +132,13 @@ __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* di
 // global loads into registers before the transform (they complete under
 // the tile load and the FFT stages); commit() writes them to LDS after the
 // transform, followed by one barrier, so the epilogue never waits on HBM.
-template <int N, int M, int S, int T>
+template <int N, int M, int S, int T, bool UXL_ = (N <= 512)>
 struct ColSub {
-  // The basis at the N/8 base rows is staged in LDS up to N = 512; larger
-  // tiles would then exceed the 80 KiB that two workgroups per CU allow, so
-  // there e_p(x0) is evaluated per thread instead.
-  static constexpr bool UXL = N <= 512;
+  // UXL: the basis at the N/8 base rows is staged in LDS (else e_p(x0) is
+  // evaluated per thread and level, a sine and cosine each).  Staging it
+  // takes P N/8 entries; callers that place the block in the transform tile
+  // (free once the last stage has read it) stage it at any size.
+  static constexpr bool UXL = UXL_;
   static constexpr int UXP = UXL ? PREG : 1, UXN = UXL ? N / 8 : 1;
   static constexpr int EMAX = UXP * UXN * (UXL ? 1 : 0) + S * PREG * M + S * M;
   static constexpr int R = (EMAX + T - 1) / T;  // prefetched values per thread
