@@ -578,13 +578,18 @@ def main(argv=None):
         print("bench: one frame per call, eager vs HIP-graph replay", file=sys.stderr, flush=True)
         from pyturb_amd.graph import CapturedFrames
 
+        from pyturb_amd.graph import FoldedFrames
+
         cap = CapturedFrames(atm, 1)
+        loop = FoldedFrames(atm, 1)
         ncall = 500
         tg = timed(lambda: [cap([float(t)]) for t in times[:ncall]], 1, 1)
+        tf = timed(lambda: [loop([float(t)]) for t in times[:ncall]], 1, 1)
         te = timed(lambda: [atm.opd(float(t)) for t in times[:ncall]], 1, 1)
         extra["loop_eager_frames_per_s"] = round(ncall / te, 1)
         extra["loop_graph_frames_per_s"] = round(ncall / tg, 1)
-        del cap
+        extra["loop_folded_frames_per_s"] = round(ncall / tf, 1)
+        del cap, loop
     # the public generator (chunked device calls, one yield per frame)
     def frames_api():
         for _ in atm.frames(DT, N_FRAMES):

@@ -215,6 +215,22 @@ int pt_spectral_rows_folded(int layers, int n, double pixel_scale,
                             void* stream);
 
 /*
+ * pt_spectral_rows_folded + pt_spectral_cols in one call, on the fold that
+ * pt_spectral_fold left in `work` (same rules).  For calls of at most 8
+ * frames the pair tables and the subharmonic tables are built in one launch
+ * (three kernels per call instead of four), which is what a loop of a few
+ * frames per call, replayed from a HIP graph, waits on.  `disp` may point to
+ * pinned host memory (hipHostMalloc): only the table kernels read it.
+ * Same results as the two calls.  No counterpart in the reference (its
+ * per-frame layer sum is atmosphere.py:910-917).  Power-of-two n only.
+ */
+int pt_spectral_frames_folded(int layers, int n, double pixel_scale,
+                              const double* disp, int frames,
+                              const double* sh_coeffs, int levels,
+                              int crop_offset, int n_out, float scale,
+                              float* out, void* work, void* stream);
+
+/*
  * Ring-buffer and matrix rows below have a pitch `ld` >= width, a multiple
  * of 4 floats, with zero padding past `width`; arrays are 16-byte aligned.
  *

@@ -50,6 +50,7 @@ SIGNATURES = {
     "pt_spectral_cols": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp]),
     "pt_spectral_fold": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "pt_spectral_rows_folded": (_i, [_i, _i, _d, _vp, _i, _vp, _vp]),
+    "pt_spectral_frames_folded": (_i, [_i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp]),
     "pt_innovation_gemm": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "pt_extrude_rows": (_i, [_vp, _i, _i, _i, _vp, _vp, ctypes.c_longlong, _i, _vp, _vp, _vp,
                              _vp, _vp, _vp]),

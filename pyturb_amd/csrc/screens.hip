@@ -60,18 +60,27 @@ __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_n
   sub_fill_set(C, P, n, s, t, !ubasis && s == 0, &s_mean, tid, 256, 0, 1, ubasis ? t.U : nullptr);
 }
 
+// One workgroup's share (`part` of `parts`) of frame f's subharmonic tables;
+// `dyn` holds (2 P L + 9 P) double2.
+__device__ __forceinline__ void frame_prep_body(unsigned char* dyn, const double* __restrict__ disp,
+                                                const double2* __restrict__ coeffs, size_t cstride, int L, int P,
+                                                int n, double inv_period, SubTables t, int tr, int ubasis, int f,
+                                                int part, int parts) {
+  __shared__ double s_mean;
+  double2* Q = (double2*)dyn;
+  double2* SB = Q + 2 * (size_t)P * L;
+  const int tid = threadIdx.x;
+  frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256, tr != 0);
+  sub_fill_set(SB, P, n, f, t, !ubasis && f == 0, &s_mean, tid, 256, part, parts, ubasis ? t.U : nullptr);
+}
+
 __global__ __launch_bounds__(256) void k_frame_prep(const double* __restrict__ disp,
                                                     const double2* __restrict__ coeffs, size_t cstride, int L,
                                                     int P, int n, double inv_period, SubTables t, int tr,
                                                     int ubasis) {
   extern __shared__ __align__(16) unsigned char dyn[];
-  __shared__ double s_mean;
-  double2* Q = (double2*)dyn;
-  double2* SB = Q + 2 * (size_t)P * L;
-  const int f = blockIdx.x, tid = threadIdx.x;
-  frame_sh_blocks(Q, SB, disp, coeffs + (size_t)f * cstride, f, L, P, inv_period, tid, 256, tr != 0);
-  sub_fill_set(SB, P, n, f, t, !ubasis && f == 0, &s_mean, tid, 256, blockIdx.y, gridDim.y,
-               ubasis ? t.U : nullptr);
+  frame_prep_body(dyn, disp, coeffs, cstride, L, P, n, inv_period, t, tr, ubasis, blockIdx.x, blockIdx.y,
+                  gridDim.y);
 }
 
 // Workgroups per frame (or pair) of the small per-call table kernels: one
@@ -314,17 +323,15 @@ __host__ __device__ constexpr int pair_tab_stride() { return N / 8 + 9; }
 // the pair stays Hermitian in both parts.  The last pair of an odd frame
 // count includes that second frame as well (it is not written).
 template <int N, bool TR = false>
-__global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restrict__ disp,
-                                                           const float2* __restrict__ side, int L, int F,
-                                                           double inv_period, const uint32_t* __restrict__ step,
-                                                           float2* __restrict__ tab, float2* __restrict__ corr,
-                                                           const uint32_t* __restrict__ guard) {
+__device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp, const float2* __restrict__ side,
+                                                 int L, int F, double inv_period, const uint32_t* __restrict__ step,
+                                                 float2* __restrict__ tab, float2* __restrict__ corr,
+                                                 const uint32_t* __restrict__ guard, int pr, int part, int parts) {
   constexpr int NB = N / 8, S = pair_tab_stride<N>();
-  const int pr = blockIdx.x;
   // a reused fold (guard set) must be the plain one, else NaN tables
   const bool poison = guard && *guard != FOLD_PLAIN;
   const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
-  const int first = blockIdx.y * 256 + threadIdx.x, stride = gridDim.y * 256;  // entries split over blocks
+  const int first = part * 256 + threadIdx.x, stride = parts * 256;  // entries split over blocks
   const bool uni = step_uniform(step);
   // fixed-point phases of the pair's frames, once per workgroup:
   // u[f][l][axis]; frame 2p + 1 of a uniform step is u(2p) + d, and a frame
@@ -408,6 +415,37 @@ __global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restr
     else
       v = make_float2(__uint_as_float(uo), 0.f);
     tp[q] = poison && e < NB + 8 ? nan2 : v;
+  }
+}
+
+template <int N, bool TR = false>
+__global__ __launch_bounds__(256) void k_frame_pair_tables(const double* __restrict__ disp,
+                                                           const float2* __restrict__ side, int L, int F,
+                                                           double inv_period, const uint32_t* __restrict__ step,
+                                                           float2* __restrict__ tab, float2* __restrict__ corr,
+                                                           const uint32_t* __restrict__ guard) {
+  pair_tables_body<N, TR>(disp, side, L, F, inv_period, step, tab, corr, guard, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
+// Both per-call table kernels of a reused fold in one launch (calls of a few
+// frames, where each is a few microseconds of latency on a handful of CUs):
+// blocks [0, pairs * ta) build the pair tables (ta per pair), the rest the
+// subharmonic tables (tb per frame).  The two are independent.
+template <int N, bool TR>
+__global__ __launch_bounds__(256) void k_frame_tables_prep(const double* __restrict__ disp,
+                                                           const float2* __restrict__ side, int L, int F,
+                                                           double inv_period, float2* __restrict__ tab,
+                                                           float2* __restrict__ corr,
+                                                           const uint32_t* __restrict__ guard, int ta,
+                                                           const double2* __restrict__ coeffs, int P,
+                                                           SubTables t, int tb) {
+  extern __shared__ __align__(16) unsigned char dyn[];
+  const int b = blockIdx.x, na = ((F + 1) / 2) * ta;
+  if (b < na) {
+    pair_tables_body<N, TR>(disp, side, L, F, inv_period, nullptr, tab, corr, guard, b / ta, b % ta, ta);
+  } else {
+    const int q = b - na;
+    frame_prep_body(dyn, disp, coeffs, 0, L, P, N, inv_period, t, TR ? 1 : 0, 0, q / tb, q % tb, tb);
   }
 }
 
@@ -547,6 +585,9 @@ __global__ __launch_bounds__(PairCfg<N>::T) void k_frame_pair_rows(const float2*
 // two passes is then a run of at least MT * 8 bytes (a full 128-byte line
 // for MT = 16, n <= 512) or a whole row, where the row-first order's column
 // pass writes its float32 frames in MT * 4-byte runs (64 B at n = 512).
+// (Narrower tiles for calls of one frame pair, 4x the workgroups, measured
+// no faster: a one-pair pass A is bound by its chain of per-layer load
+// latencies, not by the CUs it occupies.)
 template <int N>
 struct PairColsCfg {
   static constexpr int MT = N <= 512 ? 16 : 8192 / N;  // columns per tile (<= 1024 threads)
@@ -906,9 +947,39 @@ static FramesLayout frames_layout(int n, int F, int L, int P) {
 // pass A (PAIRS), or both.
 enum { STAGE_FOLD = 1, STAGE_PAIRS = 2, STAGE_ALL = 3 };
 
+// Subharmonic tables built in the same launch as the pair tables
+// (k_frame_tables_prep), for pt_spectral_frames_folded's calls of a few
+// frames.
+struct MergedPrep {
+  const double2* coeffs;
+  int P;
+  SubTables tab;
+};
+
+// The pair tables of a reused fold (guard = the fold marker), alone or with
+// the subharmonic tables of the same frames in one launch.
+template <int N, bool TR>
+static int launch_pair_tables(const double* disp, const float2* side, int L, int F, double inv_period,
+                              const uint32_t* step, float2* ptab, float2* corr, const uint32_t* guard,
+                              const MergedPrep* mp, hipStream_t st) {
+  const int pairs = (F + 1) / 2, ta = table_split(pairs);
+  if (mp && mp->P > 0) {
+    const int tb = table_split(F);
+    const size_t dyn = (size_t)(2 * mp->P * L + 9 * mp->P) * sizeof(double2);
+    if (dyn > 60 * 1024) return fail(PT_ERR_UNSUPPORTED, "frames: %d layers x %d levels exceed LDS", L, mp->P);
+    k_frame_tables_prep<N, TR><<<pairs * ta + F * tb, 256, dyn, st>>>(disp, side, L, F, inv_period, ptab, corr,
+                                                                      guard, ta, mp->coeffs, mp->P, mp->tab, tb);
+  } else {
+    k_frame_pair_tables<N, TR><<<dim3(pairs, ta), 256, 0, st>>>(disp, side, L, F, inv_period, step, ptab, corr,
+                                                                guard);
+  }
+  PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
 template <int N>
 static int frames_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
-                            float2* work, hipStream_t st, int stages = STAGE_ALL) {
+                            float2* work, hipStream_t st, int stages = STAGE_ALL, const MergedPrep* mp = nullptr) {
   constexpr int MR = PairCfg<N>::M;
   const size_t dyn_r = pair_rows_lds<N>(L);
   if (dyn_r + (size_t)Layout<N, PairCfg<N>::T, MR, false>::SIZE * 8 + (size_t)N * 8 > kLdsMax)
@@ -933,9 +1004,9 @@ static int frames_rows_pow2(const float2* spectra, int L, const double* disp, in
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
-  k_frame_pair_tables<N><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
-      disp, side, L, F, inv_period, step, ptab, corr, stages == STAGE_PAIRS ? mark : nullptr);
-  PT_LAUNCH_CHECK();
+  rc = launch_pair_tables<N, false>(disp, side, L, F, inv_period, step, ptab, corr,
+                                    stages == STAGE_PAIRS ? mark : nullptr, mp, st);
+  if (rc) return rc;
   k_frame_pair_rows<N><<<dim3(N / MR, (F + 1) / 2), PairCfg<N>::T, dyn_r, st>>>(fold, corr, L, ptab, F, step,
                                                                                  work);
   PT_LAUNCH_CHECK();
@@ -950,7 +1021,8 @@ static int frames_cols_pow2(GenArgs a, hipStream_t st) {
 // Column-first order: fold into column tiles, transposed pair tables, pass A'.
 template <int N>
 static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp, int F, double inv_period,
-                               float2* work, hipStream_t st, int stages = STAGE_ALL) {
+                               float2* work, hipStream_t st, int stages = STAGE_ALL,
+                               const MergedPrep* mp = nullptr) {
   constexpr int MT = PairColsCfg<N>::MT;
   const size_t dyn = pair_cols_lds<N>(L);
   if (dyn + (size_t)N * 8 > kLdsMax)
@@ -973,11 +1045,11 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
-  k_frame_pair_tables<N, true><<<dim3((F + 1) / 2, table_split((F + 1) / 2)), 256, 0, st>>>(
-      disp, side, L, F, inv_period, step, ptab, corr, stages == STAGE_PAIRS ? mark : nullptr);
-  PT_LAUNCH_CHECK();
-  k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(fold, corr, L, ptab, F, step,
-                                                                                  work);
+  rc = launch_pair_tables<N, true>(disp, side, L, F, inv_period, step, ptab, corr,
+                                   stages == STAGE_PAIRS ? mark : nullptr, mp, st);
+  if (rc) return rc;
+  k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(
+      fold, corr, L, ptab, F, step, work);
   PT_LAUNCH_CHECK();
   return PT_OK;
 }
@@ -1151,17 +1223,19 @@ static int frames_check(int layers, int n, double pixel_scale, const double* dis
 static int frames_order(int n) { return tune_choice(PT_TUNE_FRAMES, n); }
 
 static int spectral_rows_impl(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
-                              int frames, void* work, int order, hipStream_t st, int stages = STAGE_ALL) {
+                              int frames, void* work, int order, hipStream_t st, int stages = STAGE_ALL,
+                              const MergedPrep* mp = nullptr) {
   const double inv_period = 1.0 / ((double)n * pixel_scale);
   if (order == 0) {
 #define CALL(N)                                                                                              \
   return frames_cf_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st, \
-                                stages);
+                                stages, mp);
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   } else {
-#define CALL(N)                                                                                                \
-  return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st, stages);
+#define CALL(N)                                                                                              \
+  return frames_rows_pow2<N>((const float2*)spectra, layers, disp, frames, inv_period, (float2*)work, st, \
+                             stages, mp);
     PT_POW2_SWITCH(n, CALL)
 #undef CALL
   }
@@ -1190,6 +1264,39 @@ extern "C" int pt_spectral_rows_folded(int layers, int n, double pixel_scale, co
                             STAGE_PAIRS);
 }
 
+// Calls of at most this many frames build both table sets in one launch.
+constexpr int FOLDED_SMALL_F = 8;
+
+static int spectral_cols_impl(void* work, int layers, int n, double pixel_scale, const double* disp, int frames,
+                              const double* sh_coeffs, int levels, int crop_offset, int n_out, float scale,
+                              float* out, int order, hipStream_t st, bool prep_done = false);
+
+extern "C" int pt_spectral_frames_folded(int layers, int n, double pixel_scale, const double* disp, int frames,
+                                         const double* sh_coeffs, int levels, int crop_offset, int n_out,
+                                         float scale, float* out, void* work, void* stream) {
+  if (!out) return fail(PT_ERR_ARG, "pt_spectral_frames_folded: NULL out");
+  int rc = frames_check(layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out, work);
+  if (rc || frames == 0) return rc;
+  if (!frames_fused(n, layers, levels))
+    return fail(PT_ERR_UNSUPPORTED,
+                "pt_spectral_frames_folded: power-of-two n 32..2048, <= %d layers and <= %d levels only",
+                FOLD_LMAX, PREG);
+  const int order = frames_order(n);  // read once: both passes use the same order
+  hipStream_t st = (hipStream_t)stream;
+  const bool merge = frames <= FOLDED_SMALL_F && levels > 0;
+  MergedPrep mp;
+  if (merge) {
+    mp.coeffs = (const double2*)sh_coeffs;
+    mp.P = levels;
+    mp.tab = sub_tables((float2*)work + frames_layout(n, frames, layers, levels).tab, levels, n, frames);
+  }
+  rc = spectral_rows_impl(nullptr, layers, n, pixel_scale, disp, frames, work, order, st, STAGE_PAIRS,
+                          merge ? &mp : nullptr);
+  if (rc) return rc;
+  return spectral_cols_impl(work, layers, n, pixel_scale, disp, frames, sh_coeffs, levels, crop_offset, n_out,
+                            scale, out, order, st, merge);
+}
+
 extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double pixel_scale, const double* disp,
                                 int frames, void* work, void* stream) {
   if (!spectra) return fail(PT_ERR_ARG, "pt_spectral_rows: NULL spectra");
@@ -1202,7 +1309,7 @@ extern "C" int pt_spectral_rows(const void* spectra, int layers, int n, double p
 }
 static int spectral_cols_impl(void* work, int layers, int n, double pixel_scale, const double* disp, int frames,
                               const double* sh_coeffs, int levels, int crop_offset, int n_out, float scale,
-                              float* out, int order, hipStream_t st) {
+                              float* out, int order, hipStream_t st, bool prep_done) {
   GenArgs a;
   a.work = (const float2*)work;
   a.out = out;
@@ -1217,9 +1324,11 @@ static int spectral_cols_impl(void* work, int layers, int n, double pixel_scale,
   a.s_mul = 2;
   a.s_add = 1;
   a.scale = scale;
-  int rc = launch_frame_prep(disp, (const double2*)sh_coeffs, frames, layers, levels, n,
-                             1.0 / ((double)n * pixel_scale), a.tab, st, 0, order == 0);
-  if (rc) return rc;
+  if (!prep_done) {
+    const int rc = launch_frame_prep(disp, (const double2*)sh_coeffs, frames, layers, levels, n,
+                                     1.0 / ((double)n * pixel_scale), a.tab, st, 0, order == 0);
+    if (rc) return rc;
+  }
   if (order == 0) {
 #define CALL(N) return frames_cf_cols_pow2<N>(a, st);
     PT_POW2_SWITCH(n, CALL)

@@ -13,8 +13,15 @@ one launch from the host.
   a time, over and over, and each eager call is then dominated by fixed
   costs: five launches through ctypes, a displacement upload and the fold
   of the unchanged layer stack.  The stack is folded once at construction
-  (``ops.spectral_fold``); each call refreshes only the displacement table,
-  with one host-to-device copy, and replays the four per-call kernels.
+  (``ops.spectral_fold``); each call writes the displacement table into
+  pinned host memory, which the table kernels read in place (no upload),
+  and replays the per-call kernels (three for calls of up to 8 frames).
+  Two pinned tables and two graphs alternate, so the host fills one while
+  the GPU may still read the other.
+* ``FoldedFrames(atm, frames, directions)`` is the same call pattern without
+  a graph: one ``pt_spectral_frames_folded`` call per step through ctypes.
+  On ROCm a graph launch costs the host about 23 us, three kernel launches
+  about 9 us, so for calls of a few frames this is the faster loop.
 """
 
 from __future__ import annotations
@@ -79,25 +86,32 @@ class CapturedFrames:
         self._vel = np.stack([atm._vx, atm._vy], axis=-1)  # (L, 2)
         self._base = np.stack([atm._alt[None, :] * self.ox[:, None],
                                atm._alt[None, :] * self.oy[:, None]], axis=-1)  # (D, L, 2)
-        self._disp_host = t.empty((F, L, 2), dtype=t.float64, pin_memory=True)
-        self._disp_view = self._disp_host.numpy().reshape(self.frames, self.D, L, 2)
-        self.disp = t.zeros((F, L, 2), dtype=t.float64, device=dev)
         self.out = t.empty((F, atm.n, atm.n), dtype=t.float32, device=dev)
         P = 0 if atm._sh_coeffs is None else int(atm._sh_coeffs.shape[0])
         self.work = t.empty((ops.frames_work_elems(atm.n_screen, F, L, P),), dtype=t.complex64,
                             device=dev)
-        self._copied = t.cuda.Event()  # the pinned table is free once this completes
         ns = atm.n_screen
         # the fused engine's fold depends only on the (fixed) spectra: fold
         # once here and record only the per-call kernels
         self.folded = (ns & (ns - 1)) == 0 and 32 <= ns <= 2048 and L <= 16 and P <= 12
         if self.folded:
             ops.spectral_fold(atm._spectra, F, self.work)
-        self._cap = Captured(self._launch, dev)
+            # the table kernels read the pinned tables in place
+            self._disp = [t.zeros((F, L, 2), dtype=t.float64, pin_memory=True) for _ in range(2)]
+            self._host = self._disp
+        else:
+            # the general path reads its displacements on the device: one
+            # upload per call from the pinned table
+            self._host = [t.zeros((F, L, 2), dtype=t.float64, pin_memory=True) for _ in range(2)]
+            self._disp = [t.zeros((F, L, 2), dtype=t.float64, device=dev) for _ in range(2)]
+        self._views = [h.numpy().reshape(self.frames, self.D, L, 2) for h in self._host]
+        self._done = [t.cuda.Event(), t.cuda.Event()]  # table k is free once this completes
+        self._caps = [Captured(lambda k=k: self._launch(k), dev) for k in range(2)]
+        self._next = 0
 
-    def _launch(self) -> None:
+    def _launch(self, k: int) -> None:
         a = self.atm
-        ops.spectral_frames(a._spectra, self.disp, a._sh_coeffs, a.pixel_scale,
+        ops.spectral_frames(a._spectra, self._disp[k], a._sh_coeffs, a.pixel_scale,
                             crop_offset=a.margin_pix, n_out=a.n, scale=self.scale,
                             out=self.out, work=self.work, folded=self.folded)
 
@@ -106,15 +120,88 @@ class CapturedFrames:
         times = np.asarray(times, dtype=np.float64).ravel()
         if times.size != self.frames:
             raise ValueError(f"expected {self.frames} times, got {times.size}")
-        self._copied.synchronize()
-        v = self._disp_view
+        k = self._next
+        self._next ^= 1
+        self._done[k].synchronize()  # the replay that last read table k has finished
+        v = self._views[k]
         np.multiply(times[:, None, None, None], self._vel[None, None], out=v)
         np.add(v, self._base[None], out=v)
         if not a._wrap_warned:
             a._check_wrap(v[..., 0], v[..., 1])
-        self.disp.copy_(self._disp_host, non_blocking=True)
-        self._copied.record()
-        self._cap.replay()
+        if not self.folded:
+            self._disp[k].copy_(self._host[k], non_blocking=True)
+        self._caps[k].replay()
+        self._done[k].record()
+        out = a._cast(self.out)
+        if self.D > 1:
+            out = out.view(self.frames, self.D, a.n, a.n)
+        return out
+
+
+class FoldedFrames:
+    """``atm.opd_batch(times, directions)`` for a fixed number of times and
+    directions, as one ``pt_spectral_frames_folded`` call per step on a stack
+    folded once.  The displacement tables stay in pinned host memory (two,
+    alternating, each reused once the call that read it has finished).  The
+    returned tensor is reused by the next call: copy it if it must be kept."""
+
+    def __init__(self, atm, frames: int, directions: Optional[Sequence] = None,
+                 wavelength: Optional[float] = None) -> None:
+        import ctypes
+
+        t = _hip.torch()
+        if atm.engine != "spectral" or atm._lgs_mag is not None or atm._boiling:
+            raise ValueError("FoldedFrames needs a frozen-flow spectral atmosphere "
+                             "(no LGS cone, no boiling)")
+        ns = atm.n_screen
+        L = len(atm._layers)
+        P = 0 if atm._sh_coeffs is None else int(atm._sh_coeffs.shape[0])
+        if not ((ns & (ns - 1)) == 0 and 32 <= ns <= 2048 and L <= 16 and P <= 12):
+            raise ValueError("FoldedFrames needs a power-of-two screen of 32..2048, <= 16 layers "
+                             "and <= 12 subharmonic levels")
+        self.atm = atm
+        self.frames = int(frames)
+        if directions is None:
+            ox, oy = np.zeros(1), np.zeros(1)
+        else:
+            ox, oy = atm._dirs_tan(directions)
+        self.D = len(ox)
+        F = self.frames * self.D
+        dev = atm._dev
+        self._vel = np.stack([atm._vx, atm._vy], axis=-1)
+        self._base = np.stack([atm._alt[None, :] * ox[:, None], atm._alt[None, :] * oy[:, None]],
+                              axis=-1)
+        self._host = [t.zeros((F, L, 2), dtype=t.float64, pin_memory=True) for _ in range(2)]
+        self._views = [h.numpy().reshape(self.frames, self.D, L, 2) for h in self._host]
+        self._done = [t.cuda.Event(), t.cuda.Event()]
+        self.out = t.empty((F, atm.n, atm.n), dtype=t.float32, device=dev)
+        self.work = t.empty((ops.frames_work_elems(ns, F, L, P),), dtype=t.complex64, device=dev)
+        ops.spectral_fold(atm._spectra, F, self.work)
+        lib = _hip.lib()
+        self._fn = lib.pt_spectral_frames_folded
+        sh = _hip.ptr(atm._sh_coeffs) if P else None
+        scale = float(atm._scale(wavelength))
+        # everything but the displacement table is fixed: prebuilt arguments
+        self._args = [(L, ns, float(atm.pixel_scale), ctypes.c_void_p(h.data_ptr()), F,
+                       sh, P, int(atm.margin_pix), int(atm.n), scale, _hip.ptr(self.out),
+                       _hip.ptr(self.work)) for h in self._host]
+        self._next = 0
+
+    def __call__(self, times: Sequence[float]) -> Any:
+        a = self.atm
+        times = np.asarray(times, dtype=np.float64).ravel()
+        if times.size != self.frames:
+            raise ValueError(f"expected {self.frames} times, got {times.size}")
+        k = self._next
+        self._next ^= 1
+        self._done[k].synchronize()  # the call that last read table k has finished
+        v = self._views[k]
+        np.multiply(times[:, None, None, None], self._vel[None, None], out=v)
+        np.add(v, self._base[None], out=v)
+        if not a._wrap_warned:
+            a._check_wrap(v[..., 0], v[..., 1])
+        _hip.check(self._fn(*self._args[k], _hip.stream_ptr()), "pt_spectral_frames_folded")
+        self._done[k].record()
         out = a._cast(self.out)
         if self.D > 1:
             out = out.view(self.frames, self.D, a.n, a.n)

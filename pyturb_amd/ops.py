@@ -171,14 +171,23 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
     ``spectra`` (L, n, n) complex64; ``disp`` (F, L, 2) float64 metres;
     ``sh_coeffs`` (P, L, 3, 3) complex128 or None.  Returns (F, n_out, n_out).
     With ``folded``, ``work`` already holds the fold of these spectra for
-    this frame count (``spectral_fold``): pt_spectral_rows_folded +
-    pt_spectral_cols (power-of-two n, <= 16 layers, <= 12 levels).
+    this frame count (``spectral_fold``): pt_spectral_frames_folded
+    (power-of-two n, <= 16 layers, <= 12 levels); ``disp`` may then be a
+    pinned host tensor, read by the table kernels in place.
     """
     t = _t()
     L, n = int(spectra.shape[0]), int(spectra.shape[-1])
     _require(spectra, t.complex64, (L, n, n), "spectra")
     F = int(disp.shape[0])
-    _require(disp, t.float64, (F, L, 2), "disp")
+    if folded and not disp.is_cuda:
+        # a reused fold reads the displacements in its table kernels only, so
+        # they may stay in pinned host memory (no upload per call)
+        if not disp.is_pinned() or disp.dtype != t.float64 or tuple(disp.shape) != (F, L, 2) \
+                or not disp.is_contiguous():
+            raise ValueError(f"disp: expected a contiguous ({F}, {L}, 2) float64 device or "
+                             "pinned host tensor")
+    else:
+        _require(disp, t.float64, (F, L, 2), "disp")
     P = 0 if sh_coeffs is None else int(sh_coeffs.shape[0])
     if P:
         _require(sh_coeffs, t.complex128, (P, L, 3, 3), "sh_coeffs")
@@ -190,14 +199,12 @@ def spectral_frames(spectra, disp, sh_coeffs, pixel_scale: float, *, crop_offset
     if folded:
         if work is None or work.numel() < need:
             raise ValueError("folded: pass the workspace that spectral_fold filled")
-        lib = _hip.lib()
-        rc = lib.pt_spectral_rows_folded(L, n, float(pixel_scale), _hip.ptr(disp), F, _hip.ptr(work),
-                                         _hip.stream_ptr())
-        _hip.check(rc, "pt_spectral_rows_folded")
-        rc = lib.pt_spectral_cols(_hip.ptr(work), L, n, float(pixel_scale), _hip.ptr(disp), F,
-                                  _hip.ptr(sh_coeffs) if P else None, P, int(crop_offset), n_out,
-                                  float(scale), _hip.ptr(out), _hip.stream_ptr())
-        _hip.check(rc, "pt_spectral_cols")
+        dptr = disp.data_ptr() if not disp.is_cuda else _hip.ptr(disp)
+        rc = _hip.lib().pt_spectral_frames_folded(L, n, float(pixel_scale), dptr, F,
+                                                  _hip.ptr(sh_coeffs) if P else None, P,
+                                                  int(crop_offset), n_out, float(scale),
+                                                  _hip.ptr(out), _hip.ptr(work), _hip.stream_ptr())
+        _hip.check(rc, "pt_spectral_frames_folded")
         return out
     if work is None or work.numel() < need:
         work = t.empty((need,), dtype=t.complex64, device=spectra.device)

@@ -33,6 +33,56 @@ def test_captured_frames_replays_match_eager(hip):
         assert torch.equal(got, want), k
 
 
+@pytest.mark.parametrize("frames,ndirs", [(1, 1), (3, 1), (2, 3)])
+def test_captured_few_frames_replays_match_eager(hip, frames, ndirs):
+    """Calls of at most 8 frames build both table sets in one launch and read
+    the displacements from pinned host memory; every replay (the two graphs
+    alternate) equals the eager call, with eager work between replays."""
+    import torch
+
+    import pyturb_amd as pa
+    from pyturb_amd.graph import CapturedFrames
+
+    print(f"graph replay n=256, {frames} frames x {ndirs} directions", flush=True)
+    atm = pa.Atmosphere([pa.Layer(*ly) for ly in config2_layer_tuples()], r0=0.15, n=256,
+                        field_of_view=20.0, seed=2, device="mi355x")
+    dirs = [(0.0, 0.0), (10.0, -5.0), (-12.0, 3.0)][:ndirs]
+    cap = CapturedFrames(atm, frames, directions=dirs if ndirs > 1 else None)
+    rng = np.random.default_rng(frames)
+    for k in range(6):
+        times = np.sort(rng.uniform(0.0, 0.2, frames))
+        got = cap(times).clone()
+        torch.cuda.synchronize()
+        want = (atm.opd_batch(times, directions=dirs) if ndirs > 1 else atm.opd_batch(times))
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), k
+
+
+@pytest.mark.parametrize("frames,ndirs", [(1, 1), (2, 3), (12, 1)])
+def test_folded_frames_loop_matches_eager(hip, frames, ndirs):
+    """FoldedFrames (one pt_spectral_frames_folded call per step on a fold
+    made once, pinned displacement tables) equals the eager call."""
+    import torch
+
+    import pyturb_amd as pa
+    from pyturb_amd.graph import FoldedFrames
+
+    # a 41.2" field widens the 256-pixel pupil to a 512-pixel screen (the
+    # fused engine needs a power of two); directions up to 41"
+    atm = pa.Atmosphere([pa.Layer(*ly) for ly in config2_layer_tuples()], r0=0.15, n=256,
+                        field_of_view=41.2 if ndirs > 1 else 0.0, seed=2, device="mi355x")
+    assert atm.n_screen in (256, 512)
+    dirs = [(0.0, 0.0), (30.0, -15.0), (-22.0, 33.0)][:ndirs]
+    loop = FoldedFrames(atm, frames, directions=dirs if ndirs > 1 else None)
+    rng = np.random.default_rng(frames + 10)
+    for k in range(6):
+        times = np.sort(rng.uniform(0.0, 0.2, frames))
+        got = loop(times).clone()
+        want = (atm.opd_batch(times, directions=dirs) if ndirs > 1 else atm.opd_batch(times))
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), k
+
+
 def test_captured_mixed_ensemble_replays_match_eager(hip):
     """Two size classes and two outer-scale groups recorded once; the noise is
     redrawn in place before each replay and the result must equal the eager
