@@ -319,8 +319,9 @@ def build_extrude(dev, far_points=32):
     ext = None
 
     def innov(l, k):
-        if l not in pool:
-            pool[l] = torch.randn((int(rows[l]) + 64, ext.width), generator=gen, device=dev)
+        if l not in pool:  # rows with a pitch of a multiple of 4 floats (16-byte aligned)
+            pitch = (ext.width + 3) // 4 * 4
+            pool[l] = torch.randn((int(rows[l]) + 64, pitch), generator=gen, device=dev)[:, :ext.width]
         out = pool[l][used[l]:used[l] + k]
         used[l] += k
         return out

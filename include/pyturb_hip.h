@@ -285,6 +285,41 @@ int pt_extrude_rows_far(const float* A, int width, int ld, int stencil_rows,
                         const int* bbrow, const float* bscale, void* stream);
 
 /*
+ * One extrusion step in one launch: the innovation terms of every new row
+ * and every round of the recurrence (pt_innovation_gemm followed by the
+ * pt_extrude_rows_far rounds, bit-identical results), for layers that share
+ * one A and one B.  Layer l extrudes rows[l] rows at physical ring rows
+ * (first_row[l] + i) mod cap, i < rows[l], from its innovations
+ * beta[l] + i * beta_pitch[l] (HOST arrays of device pointers, 16-byte
+ * aligned, and pitches in floats, multiples of 4; a row is read up to its
+ * width rounded up to 4, the floats past the width are ignored), each row's
+ * stencil being the m ring rows before it (and the
+ * far rows far_rows_back[j] + 1 behind it).  A cooperative launch: each
+ * workgroup keeps its 16 rows of A in registers for the whole step, and
+ * the rounds are separated by a grid barrier on `sync`, PT_EXTRUDE_SYNC_BYTES
+ * of device memory (8-byte aligned) that is zero before the first call and
+ * is passed unchanged to every later call on the same stream (it counts the
+ * barriers passed).  A barrier wait is bounded by spin_limit polls; on
+ * expiry, or when the abort word (the second 8-byte word of sync) is
+ * nonzero, every workgroup leaves the kernel, *status (host-visible, e.g.
+ * pinned) is set to 1 and the rows of the step are incomplete: zero sync
+ * again before the next call.  PT_ERR_UNSUPPORTED (nothing launched) for
+ * more than 16 layers, more than 256 new rows, stencil_rows > 2, a row
+ * pitch ld above 2304 or not a multiple of 16, or a GPU that cannot hold
+ * the grid; the caller then uses the
+ * round-by-round entry points.  Replaces src/pyturb/extrude.py:424-430 for
+ * every layer of a set_time step.  Other arguments as pt_extrude_rows_far.
+ */
+#define PT_EXTRUDE_SYNC_BYTES 2048
+int pt_extrude_step(const float* A, const float* B, int width, int ld,
+                    int stencil_rows, int far_points, const int* far_rows_back,
+                    float* buf, long long cap, int layers,
+                    const int* first_row, const int* rows,
+                    const float* const* beta, const long long* beta_pitch,
+                    const float* bscale, void* sync, int* status,
+                    unsigned spin_limit, void* stream);
+
+/*
  * Fused multi-layer, multi-direction readout of the ring buffers
  * (src/pyturb/extrude.py:754-915 and the CUDA readout kernels
  * extrude.py:91-198; one launch instead of one per direction):

@@ -56,6 +56,8 @@ SIGNATURES = {
                              _vp, _vp, _vp]),
     "pt_extrude_rows_far": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, ctypes.c_longlong, _i, _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp]),
+    "pt_extrude_step": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, ctypes.c_longlong, _i, _vp, _vp, _vp,
+                             _vp, _vp, _vp, _vp, ctypes.c_uint, _vp]),
     "pt_readout": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _f,
                         _i, _vp, _vp]),
     "pt_readout_adjoint_work_elems": (ctypes.c_longlong, [_i]),

@@ -70,6 +70,10 @@ static_assert(sizeof(GemvArgs) <= 4096, "kernel argument block");
 constexpr int MF_WAVES = 16;
 typedef float mfma_v4f __attribute__((ext_vector_type(4)));
 
+// s + a * b as one explicit fused multiply-add (whatever the contraction
+// flags), so every kernel that finishes a row this way gives the same bits
+__device__ __forceinline__ float add_product(float s, float a, float b) { return __builtin_fmaf(a, b, s); }
+
 template <bool ZV4, int NT>
 __global__ __launch_bounds__(64 * MF_WAVES) void k_mfma_gemv(GemvArgs a) {
   __shared__ float red[NT][MF_WAVES][16][17];
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_mfma_gemv(GemvArgs a) {
           add = a.add[k];
           sc = a.scale[k];
         }
-      out[row0 + rr] = sum + (add ? sc * add[row0 + rr] : 0.f);
+      out[row0 + rr] = add ? add_product(sum, sc, add[row0 + rr]) : sum;
     }
   }
 }
@@ -198,6 +202,393 @@ static int launch_gemv(GemvArgs& a, bool zv4, hipStream_t st) {
 }
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+// ---------------------------------------------------------------------------
+// One extrusion step in one cooperative launch (k_extrude_step)
+// ---------------------------------------------------------------------------
+// The step's innovation terms and every recurrence round of k_mfma_gemv in a
+// single launch: workgroup g owns matrix rows [16 g, 16 g + 16) of A and B
+// (one workgroup per CU), keeps its slice of A in registers for the whole
+// step (A is read from HBM once per step instead of once per round), keeps
+// its share of the innovation terms B beta in LDS, and the rounds are
+// separated by a grid barrier.  Within a workgroup the arithmetic is
+// k_mfma_gemv's (same chunk order, same MFMA tiles, same cross-wave sum, the
+// same fused multiply-add at the end), so the rows are bit-identical to
+// pt_innovation_gemm + pt_extrude_rows_far.
+//
+// Grid barrier: every workgroup owns a 64-bit flag, the number of barriers
+// it has passed since the sync words were zeroed; at a barrier it writes its
+// flag (after its rows are written back) and one wave polls all flags of the
+// grid until each has reached the barrier's number -- no read-modify-write
+// atomics, so nothing serialises on one address.  sync[0] holds the barrier
+// count at the start of the next call, sync[1] the abort word.  A wait is
+// bounded by spin_limit polls: on expiry the workgroup sets the abort word
+// and *status and leaves, and a workgroup that sees the abort word leaves
+// too (the word is checked before the flags), so every workgroup ends.
+constexpr int XS_LMAX = 16;              // layers per launch
+constexpr int XS_VMAX = 256;             // innovation rows per step
+constexpr int XS_CWMAX = 9;              // A chunks per segment and virtual wave in registers (row pitch <= 2304)
+constexpr int XI_GRP = 3;                // innovation: chunks per prefetch group
+constexpr int XI_NG = (XS_CWMAX + XI_GRP - 1) / XI_GRP;  // innovation: groups per virtual wave
+constexpr int XS_GRP = 3;                // rounds: chunks per prefetch group
+constexpr int XS_NG = (XS_CWMAX + XS_GRP - 1) / XS_GRP;  // rounds: groups per virtual wave and segment
+constexpr int XS_NBUF = 4;               // rounds: prefetch ring (loads run XS_NBUF - 1 groups ahead)
+constexpr int XS_NT = 2;                 // innovation vector tiles per pass (32 vectors)
+constexpr int XS_GRIDMAX = 144;          // workgroups (row pitch <= 2304)
+constexpr int XS_FLAGS = 16;             // sync word of workgroup g's flag: XS_FLAGS + g
+static_assert((XS_FLAGS + XS_GRIDMAX) * 8 <= PT_EXTRUDE_SYNC_BYTES, "sync words");
+
+struct StepArgs {
+  const float* A;  // (W, ldm): m stencil segments of ld, then the far segment
+  const float* B;  // (W, ld)
+  float* buf;      // (L, cap, ld) rings
+  long long ldm, cap;
+  int W, ld, m, nfar, L, rounds, nvec, cw;  // cw: A chunks per wave and segment
+  int far_d[FAR_RMAX];                      // far rows behind the newest stencil row
+  int t0[XS_LMAX], cnt[XS_LMAX], boff[XS_LMAX];
+  long long bpitch[XS_LMAX];                // floats between a layer's innovation rows
+  const float* beta[XS_LMAX];
+  float scale[XS_LMAX];
+  unsigned long long* sync;
+  int* status;
+  unsigned spin_limit;
+};
+static_assert(sizeof(StepArgs) <= 4096, "kernel argument block");
+
+__device__ __forceinline__ long long ring_row(long long r, long long cap) {
+  r %= cap;
+  return r < 0 ? r + cap : r;
+}
+
+// Index of the (e+1)-th set bit of `live` (the layer in column e), or -1.
+__device__ __forceinline__ int nth_layer(unsigned live, int e) {
+  unsigned m = live;
+#pragma unroll
+  for (int j = 0; j < XS_LMAX - 1; ++j)
+    if (j < e) m &= m - 1;
+  return m ? __ffs(m) - 1 : -1;
+}
+
+// Grid barrier number `target`; false when it timed out here or anywhere in
+// the grid.  Only wave 0 of the workgroup has stored rows since the last
+// barrier, so its write-back covers them.
+__device__ bool step_grid_sync(const StepArgs& a, unsigned long long target) {
+  __shared__ int ok;
+  if (threadIdx.x < 64) {
+    unsigned long long* flags = a.sync + XS_FLAGS;
+    __threadfence();  // this wave's row stores reach memory before the flag
+    if (threadIdx.x == 0) __hip_atomic_store(flags + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int good = 1;
+    unsigned it = 0;
+    for (;;) {
+      const unsigned long long abort = __hip_atomic_load(a.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int done = 1;
+      for (int q = threadIdx.x; q < (int)gridDim.x; q += 64)
+        done &= __hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+      if (__any(abort != 0) || ++it > a.spin_limit) {  // the abort word is checked first
+        good = 0;
+        break;
+      }
+      if (__all(done)) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();  // the other workgroups' rows are read fresh after this
+    if (threadIdx.x == 0) {
+      if (!good) {
+        __hip_atomic_store(a.sync + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      ok = good;
+    }
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// 512 threads: each wave plays two of k_mfma_gemv's 16 waves (virtual waves
+// wave and wave + 8), with separate accumulators, so the A slices of both
+// fit its registers (two waves per SIMD) and the sums keep k_mfma_gemv's
+// order exactly.  Loads run ahead of the MFMAs: one group of XI_GRP chunks
+// in the innovation phase, XS_NBUF - 1 groups of XS_GRP chunks in a round.
+constexpr int XS_WAVES = MF_WAVES / 2;
+
+// The innovation terms B beta of every new row of the step into bbl, with
+// k_mfma_gemv<false, NT>'s arithmetic (same chunks per virtual wave, same
+// MFMA tiles, same cross-wave sum).  Every load is unconditional, from an
+// address clamped into its row, and out-of-range values are selected away
+// afterwards: a load under a per-lane branch would make the compiler drain
+// all loads in flight where the branches join.  Innovation rows are 16-byte
+// aligned with a pitch of a multiple of 4 floats (float4 loads; a row is
+// read up to its width rounded up to 4).
+
+__device__ __forceinline__ void step_innovations(const StepArgs& a, float (&red)[XS_NT][MF_WAVES][16][17],
+                                                 float (&bbl)[XS_VMAX][16], int wave, int ri, int kq, int row0,
+                                                 int cps) {
+  const float* brow = a.B + (size_t)min(row0 + ri, a.W - 1) * a.ld;
+  for (int v0 = 0; v0 < a.nvec; v0 += XS_NT * VT) {
+    const int nv = a.nvec - v0 < XS_NT * VT ? a.nvec - v0 : XS_NT * VT;
+    const float* zs[XS_NT];
+    bool zl[XS_NT];
+#pragma unroll
+    for (int j = 0; j < XS_NT; ++j) {
+      const int v = v0 + j * VT + ri;
+      zs[j] = brow;  // any readable row; its values are selected away
+      zl[j] = false;
+      if (j * VT + ri < nv) {
+        // the layer of innovation row v (kernel arguments read with uniform indices only)
+        for (int q = 0; q < a.L; ++q)
+          if (a.cnt[q] > 0 && v >= a.boff[q] && v < a.boff[q] + a.cnt[q]) {
+            zs[j] = a.beta[q] + (size_t)(v - a.boff[q]) * a.bpitch[q];
+            zl[j] = true;
+          }
+      }
+    }
+    mfma_v4f acc[2][XS_NT];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < XS_NT; ++j) acc[h][j] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+    float4 xb[2][XI_GRP], zb[2][XS_NT][XI_GRP];
+    // group g: virtual wave wave + 8 (g / XI_NG), chunks XI_GRP (g % XI_NG) + [0, XI_GRP)
+    auto load_group = [&](int g, float4 (&x)[XI_GRP], float4 (&z)[XS_NT][XI_GRP]) {
+      const int vw = wave + XS_WAVES * (g / XI_NG);
+#pragma unroll
+      for (int u = 0; u < XI_GRP; ++u) {
+        const int i = XI_GRP * (g % XI_NG) + u, c = vw + MF_WAVES * i;
+        const bool ex = i < XS_CWMAX && i < a.cw && c < cps;  // wave-uniform; MFMAs skipped when false
+        const int k = ex ? 16 * c + 4 * kq : 4 * kq;             // ld % 16 == 0: k + 3 < ld
+        x[u] = *(const float4*)(brow + k);
+#pragma unroll
+        for (int j = 0; j < XS_NT; ++j) {
+          // floats k..k+3 of the row: the 4-float window [kb, kb + 4) with
+          // kb = min(k, W - 4) lies inside the row; the last chunk shifts it by
+          // d = k - kb, and values past the row's end are +0
+          // floats past the row's end (k + q >= W) are masked to +0 bitwise, so
+          // whatever the row's padding holds never reaches the MFMA
+          const float4 zz = *(const float4*)(zs[j] + (k < a.W ? k : 0));
+          const int n = zl[j] ? a.W - k : 0;  // valid floats of this float4 (<= 0: none)
+          z[j][u] = make_float4(__uint_as_float(__float_as_uint(zz.x) & (n > 0 ? ~0u : 0u)),
+                                __uint_as_float(__float_as_uint(zz.y) & (n > 1 ? ~0u : 0u)),
+                                __uint_as_float(__float_as_uint(zz.z) & (n > 2 ? ~0u : 0u)),
+                                __uint_as_float(__float_as_uint(zz.w) & (n > 3 ? ~0u : 0u)));
+        }
+      }
+    };
+    auto mfma_group = [&](int g, const float4 (&x)[XI_GRP], const float4 (&z)[XS_NT][XI_GRP]) {
+      const int h = g / XI_NG, vw = wave + XS_WAVES * h;
+#pragma unroll
+      for (int u = 0; u < XI_GRP; ++u) {
+        const int i = XI_GRP * (g % XI_NG) + u, c = vw + MF_WAVES * i;
+        if (i < XS_CWMAX && i < a.cw && c < cps) {  // wave-uniform
+#pragma unroll
+          for (int j = 0; j < XS_NT; ++j) {
+            if (j * VT >= nv) break;  // tiles in use (as k_mfma_gemv<false, NT>)
+            acc[h][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u].x, z[j][u].x, acc[h][j], 0, 0, 0);
+            acc[h][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u].y, z[j][u].y, acc[h][j], 0, 0, 0);
+            acc[h][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u].z, z[j][u].z, acc[h][j], 0, 0, 0);
+            acc[h][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u].w, z[j][u].w, acc[h][j], 0, 0, 0);
+          }
+        }
+      }
+    };
+    load_group(0, xb[0], zb[0]);
+#pragma unroll
+    for (int g = 0; g < 2 * XI_NG; ++g) {  // (virtual wave, group)
+      if (g + 1 < 2 * XI_NG) load_group(g + 1, xb[(g + 1) & 1], zb[(g + 1) & 1]);
+      mfma_group(g, xb[g & 1], zb[g & 1]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < XS_NT; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[j][wave + XS_WAVES * h][4 * kq + r][ri] = acc[h][j][r];
+    __syncthreads();
+    for (int q = threadIdx.x; q < XS_NT * 256; q += 64 * XS_WAVES) {
+      const int j = q >> 8, rr = (q >> 4) & 15, e = j * VT + (q & 15);
+      if (e < nv) {
+        float sum = 0.f;
+#pragma unroll
+        for (int w = 0; w < MF_WAVES; ++w) sum += red[j][w][rr][q & 15];
+        bbl[v0 + e][rr] = sum;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
+  __shared__ float red[XS_NT][MF_WAVES][16][17];
+  __shared__ float bbl[XS_VMAX][16];
+  __shared__ float rowv[16][17];
+  __shared__ float4 afar[2][64];  // the far segment of A (virtual waves 0 and 1 at most)
+  __shared__ float4 alast[2][M][XS_WAVES][64];  // chunk XS_CWMAX - 1 of every segment (registers are full)
+  __shared__ unsigned long long base_s;
+  __shared__ int lay_cnt[XS_LMAX], lay_t0[XS_LMAX], lay_boff[XS_LMAX];
+  __shared__ float lay_sc[XS_LMAX];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row0 = blockIdx.x * 16;
+  const int ri = lane & 15, kq = lane >> 4;
+  if (threadIdx.x == 0) {
+    base_s = __hip_atomic_load(a.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the per-layer tables, for the rounds' per-lane lookups (kernel arguments
+    // are read with uniform indices only)
+#pragma unroll
+    for (int q = 0; q < XS_LMAX; ++q) {
+      const bool in = q < a.L;
+      lay_cnt[q] = in ? a.cnt[q] : 0;
+      lay_t0[q] = in ? a.t0[q] : 0;
+      lay_boff[q] = in ? a.boff[q] : 0;
+      lay_sc[q] = in ? a.scale[q] : 0.f;
+    }
+  }
+  const int cps = (a.ld + 15) >> 4;
+  // --- innovation terms B beta of every new row, into LDS
+  step_innovations(a, red, bbl, wave, ri, kq, row0, cps);
+  // --- this workgroup's rows of A, in registers for the whole step: for
+  // virtual wave vw, chunk i of segment s is chunk vw + 16 i of the segment;
+  // the far segment's chunk is vw (static register indices throughout)
+  const float* arow = a.A + (size_t)min(row0 + ri, a.W - 1) * a.ldm;
+  const int fch = (a.nfar + 15) >> 4;
+  float4 areg[2][M][XS_CWMAX - 1];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int vw = wave + XS_WAVES * h;
+#pragma unroll
+    for (int s = 0; s < M; ++s)
+#pragma unroll
+      for (int i = 0; i < XS_CWMAX; ++i) {
+        const int c = vw + MF_WAVES * i;
+        const bool ex = i < a.cw && c < cps;  // chunks that do not exist are never multiplied
+        const float4 v = *(const float4*)(arow + (size_t)s * a.ld + (ex ? 16 * c + 4 * kq : 4 * kq));
+        if (i < XS_CWMAX - 1)
+          areg[h][s][i] = v;
+        else
+          alast[h][s][wave][lane] = v;
+      }
+  }
+  if (wave < fch) afar[wave][lane] = *(const float4*)(arow + (size_t)M * a.ld + 16 * wave + 4 * kq);
+  __syncthreads();
+  const unsigned long long base = base_s;
+  // --- the rounds: round r extrudes row r of every layer that has one
+  for (int r = 0; r < a.rounds; ++r) {
+    // entries of this round: column e is the e-th layer with cnt > r; ring
+    // rows as 32-bit float offsets from buf (the rings hold fewer than 2^31 floats)
+    unsigned live = 0;
+#pragma unroll
+    for (int q = 0; q < XS_LMAX; ++q) live |= (lay_cnt[q] > r ? 1u : 0u) << q;
+    const int le = nth_layer(live, ri);
+    const long long t0 = le >= 0 ? lay_t0[le] : 0;
+    int zo[M], fo[FAR_RMAX];
+#pragma unroll
+    for (int s = 0; s < M; ++s) zo[s] = 0;
+#pragma unroll
+    for (int j = 0; j < FAR_RMAX; ++j) fo[j] = 0;
+    if (le >= 0) {
+      const long long slab = (long long)le * a.cap;
+      const long long target = ring_row(t0 + r, a.cap);
+#pragma unroll
+      for (int s = 0; s < M; ++s) zo[s] = (int)((slab + ring_row(target - M + s, a.cap)) * a.ld);
+#pragma unroll
+      for (int j = 0; j < FAR_RMAX; ++j)
+        if (8 * j < a.nfar) fo[j] = (int)((slab + ring_row(target - 1 - a.far_d[j], a.cap)) * a.ld);
+    }
+    // far points of both virtual waves (k_mfma_gemv's far segment)
+    float zf[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int vw = wave + XS_WAVES * h;
+      const int k = 16 * vw + 4 * kq, fr = 2 * vw + (kq >> 1), i0 = 4 * (kq & 1);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) zf[h][u] = 0.f;
+      if (vw < fch) {  // wave-uniform
+        const int fb = fr == 0 ? fo[0] : fr == 1 ? fo[1] : fr == 2 ? fo[2] : fo[3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float v = a.buf[(unsigned)(fb + ((2 * (i0 + u) + 1) * a.W) / (2 * FAR_COLS))];
+          zf[h][u] = k < a.nfar ? v : 0.f;  // points past nfar meet A's zero padding: keep them finite
+        }
+      }
+    }
+    // stencil rows in groups g = (h, s, gi): chunks XS_GRP gi + [0, XS_GRP), loaded XS_NBUF - 1 groups ahead
+    constexpr int G = 2 * M * XS_NG;
+    float4 zb[XS_NBUF][XS_GRP];
+    mfma_v4f acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+    auto load_group = [&](int g, float4 (&z)[XS_GRP]) {
+      const int h = g / (M * XS_NG), s = (g / XS_NG) % M, vw = wave + XS_WAVES * h;
+#pragma unroll
+      for (int u = 0; u < XS_GRP; ++u) {
+        const int i = XS_GRP * (g % XS_NG) + u, c = vw + MF_WAVES * i;
+        // unconditional loads (the compiler drains loads in flight where
+        // branches around loads join): a chunk that does not exist is never
+        // multiplied, and a column without a layer (le < 0, zo = 0) only
+        // feeds its own unused output column
+        const int k = (i < XS_CWMAX && i < a.cw && c < cps) ? 16 * c + 4 * kq : 4 * kq;
+        z[u] = *(const float4*)(a.buf + (unsigned)(zo[s] + k));
+      }
+    };
+#pragma unroll
+    for (int g = 0; g < XS_NBUF - 1; ++g)
+      if (g < G) load_group(g, zb[g]);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (g + XS_NBUF - 1 < G) load_group(g + XS_NBUF - 1, zb[(g + XS_NBUF - 1) % XS_NBUF]);
+      const int h = g / (M * XS_NG), s = (g / XS_NG) % M, vw = wave + XS_WAVES * h;
+#pragma unroll
+      for (int u = 0; u < XS_GRP; ++u) {
+        const int i = XS_GRP * (g % XS_NG) + u, c = vw + MF_WAVES * i;
+        if (i < XS_CWMAX && i < a.cw && c < cps) {  // wave-uniform
+          const float4 x = i < XS_CWMAX - 1 ? areg[h][s][i] : alast[h][s][wave][lane], z = zb[g % XS_NBUF][u];
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z.x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z.y, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z.z, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z.w, acc, 0, 0, 0);
+        }
+      }
+      if (g % (M * XS_NG) == M * XS_NG - 1) {  // the last group of virtual wave vw: far points, then its partial tile
+        if (vw < fch) {
+          const float4 x = afar[vw][lane];
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, zf[h][0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, zf[h][1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, zf[h][2], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, zf[h][3], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[0][vw][4 * kq + q][ri] = acc[q];
+        acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+      const int rr = threadIdx.x >> 4, e = threadIdx.x & 15;
+      const int l = nth_layer(live, e);
+      if (l >= 0) {
+        float sum = 0.f;
+#pragma unroll
+        for (int w = 0; w < MF_WAVES; ++w) sum += red[0][w][rr][e];
+        rowv[rr][e] = add_product(sum, lay_sc[l], bbl[lay_boff[l] + r][rr]);
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {  // one wave stores the rows, so one write-back publishes them
+      const int e = lane & 15;
+      const int l = nth_layer(live, e);
+      if (l >= 0) {
+        float* dst = a.buf + ((size_t)l * a.cap + ring_row((long long)lay_t0[l] + r, a.cap)) * a.ld + row0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = (lane >> 4) + 4 * q;
+          if (row0 + rr < a.W) dst[rr] = rowv[rr][e];
+        }
+      }
+    }
+    if (r + 1 < a.rounds && !step_grid_sync(a, base + r + 1)) return;
+  }
+  // the barrier count for the next call (every workgroup read it before barrier 0)
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(a.sync, base + (a.rounds - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ---------------------------------------------------------------------------
 // fused readout
@@ -950,5 +1341,76 @@ extern "C" int pt_rows_blend(float* dst, long long dst_pitch, int dst_start, int
   k_rows_blend<<<dim3((width + 255) / 256, rows), 256, 0, (hipStream_t)stream>>>(dst, dst_pitch, dst_start, dst_cap,
                                                                                   src, src_pitch, width, a, b);
   PT_LAUNCH_CHECK();
+  return PT_OK;
+}
+
+extern "C" int pt_extrude_step(const float* A, const float* B, int width, int ld, int stencil_rows, int far_points,
+                               const int* far_rows_back, float* buf, long long cap, int layers, const int* first_row,
+                               const int* rows, const float* const* beta, const long long* beta_pitch,
+                               const float* bscale, void* sync, int* status, unsigned spin_limit,
+                               void* stream) {
+  if (!A || !B || !buf || !first_row || !rows || !beta || !beta_pitch || !bscale || !sync || !status ||
+      (far_points && !far_rows_back))
+    return fail(PT_ERR_ARG, "pt_extrude_step: NULL array");
+  if (width < 4 || ld < width || (ld & 3) || stencil_rows < 1 || stencil_rows > MMAX || layers < 1 || cap < 1)
+    return fail(PT_ERR_ARG, "pt_extrude_step: bad sizes");
+  if (ld & 15) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: row pitch not a multiple of 16 floats");
+  if (far_points < 0 || far_points % FAR_COLS || far_points > FAR_COLS * FAR_RMAX)
+    return fail(PT_ERR_ARG, "pt_extrude_step: far_points must be a multiple of %d in [0, %d]", FAR_COLS,
+                FAR_COLS * FAR_RMAX);
+  if (!aligned16(A) || !aligned16(B) || !aligned16(buf) || ((uintptr_t)sync & 7))
+    return fail(PT_ERR_ARG, "pt_extrude_step: A, B and buf must be 16-byte aligned, sync 8-byte aligned");
+  StepArgs a = {};
+  a.A = A;
+  a.B = B;
+  a.buf = buf;
+  a.ld = ld;
+  a.W = width;
+  a.m = stencil_rows;
+  a.nfar = far_points;
+  a.ldm = (long long)stencil_rows * ld + ((far_points + 15) / 16) * 16;
+  a.cap = cap;
+  a.L = layers;
+  a.cw = ((ld + 15) / 16 + MF_WAVES - 1) / MF_WAVES;
+  a.sync = (unsigned long long*)sync;
+  a.status = status;
+  a.spin_limit = spin_limit;
+  if ((long long)layers * cap * ld >= (1LL << 31))
+    return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: rings of 2^31 floats or more");
+  if (layers > XS_LMAX || a.cw > XS_CWMAX || stencil_rows > 2)
+    return fail(PT_ERR_UNSUPPORTED,
+                "pt_extrude_step: more than %d layers, more than 2 stencil rows or a row too wide for the register slice",
+                XS_LMAX);
+  for (int j = 0; j < FAR_RMAX; ++j) a.far_d[j] = far_points && j < far_points / FAR_COLS ? far_rows_back[j] : 0;
+  int nvec = 0, rounds = 0;
+  for (int l = 0; l < layers; ++l) {
+    if (rows[l] < 0 || first_row[l] < 0 || first_row[l] >= cap) return fail(PT_ERR_ARG, "pt_extrude_step: bad rows");
+    if (rows[l] > 0 && (!beta[l] || ((uintptr_t)beta[l] & 15) || (beta_pitch[l] & 3) || beta_pitch[l] < width))
+      return fail(PT_ERR_ARG,
+                  "pt_extrude_step: layer %d innovations NULL, not 16-byte aligned, pitch not a multiple of 4 or "
+                  "below width", l);
+    a.t0[l] = first_row[l];
+    a.cnt[l] = rows[l];
+    a.boff[l] = nvec;
+    a.beta[l] = beta[l];
+    a.bpitch[l] = beta_pitch[l];
+    a.scale[l] = bscale[l];
+    nvec += rows[l];
+    rounds = rows[l] > rounds ? rows[l] : rounds;
+  }
+  if (nvec > XS_VMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: more than %d new rows per step", XS_VMAX);
+  if (rounds == 0) return PT_OK;
+  a.nvec = nvec;
+  a.rounds = rounds;
+  const int grid = (width + 15) / 16;
+  if (grid > XS_GRIDMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: width over %d", 16 * XS_GRIDMAX);
+  void* args[] = {&a};
+  const void* kern = stencil_rows == 1 ? (const void*)k_extrude_step<1> : (const void*)k_extrude_step<2>;
+  hipError_t e = hipLaunchCooperativeKernel(kern, dim3(grid), dim3(64 * XS_WAVES), args, 0, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(e == hipErrorCooperativeLaunchTooLarge ? PT_ERR_UNSUPPORTED : PT_ERR_HIP,
+                "pt_extrude_step: cooperative launch of %d workgroups: %s", grid, hipGetErrorString(e));
+  }
   return PT_OK;
 }

@@ -5,10 +5,12 @@ exactly (host integers and float64); the data path is on the device:
 
 * one (layers, cap, W) float32 ring-buffer array in HBM; a compaction is an
   index shift (:class:`pyturb_amd.ring.Ring`), never a copy;
-* ``set_time`` plans every layer's new rows on the host, computes all their
-  innovation terms ``B_unit beta`` in one GEMM, then runs the sequential
-  ``A z`` recurrence round by round, each round one launch for all layers
-  that extrude a row;
+* ``set_time`` plans every layer's new rows on the host, then one
+  cooperative launch (pt_extrude_step) computes all their innovation terms
+  ``B_unit beta`` and runs the sequential ``A z`` recurrence round by round
+  with A held in registers and a grid barrier between rounds; shapes it does
+  not cover run the innovation GEMM and one launch per round instead (the
+  same rows, bit for bit);
 * ``integrate``/``integrate_dirs`` is one fused readout launch for all
   layers and up to 8 directions, each direction with its own per-layer
   magnification (so NGS directions and an LGS cone share a launch).
@@ -38,8 +40,8 @@ from . import _hip
 from .backend import resolve_device
 from .config import ExtrusionConfig
 from .fourier import PhaseScreen
-from .infinite import (device_extrusion, device_seed_factor, extrude_rounds, far_rows_back, readout,
-                       row_pitch)
+from .infinite import (StepState, device_extrusion, device_seed_factor, extrude_rounds, far_rows_back,
+                       readout, row_pitch)
 from .ring import Ring
 
 __all__ = ["ExtrudedAtmosphere", "along_extent_and_span"]
@@ -145,6 +147,9 @@ class ExtrudedAtmosphere:
             ly.travel = 0.0
             self.layers.append(ly)
         self._wind_cs = [(ly.cos, ly.sin) for ly in self.layers]
+        # each set_time is one cooperative launch where its shape allows
+        # (pt_extrude_step), else the innovation GEMM and a launch per round
+        self._step = StepState(self._dev, far)
         # boiling (extrude.py:586-654): one stream for every layer; the device
         # draws it from Philox keyed by boil_seed, counted by (step, layer, kind)
         self.boiling = any(np.isfinite(ly.tau_boil) for ly in self.layers)
@@ -156,16 +161,20 @@ class ExtrudedAtmosphere:
 
     # ------------------------------------------------------------------
     def _betas(self, counts):
-        out = []
+        if self._innovations is not None:
+            return [self._innovations(i, k) if k else None for i, k in enumerate(counts)]
+        # seeded draws: every layer's rows in one pitched array (a pitch of a
+        # multiple of 4 floats, as the one-launch step reads them), one copy
+        W, pitch = self.width, (self.width + 3) // 4 * 4
+        host = np.zeros((sum(counts), pitch), np.float32)
+        spans, off = [], 0
         for i, k in enumerate(counts):
-            if k == 0:
-                out.append(None)
-            elif self._innovations is not None:
-                out.append(self._innovations(i, k))
-            else:
-                draw = self.layers[i].rng.standard_normal((k, self.width), dtype=self.dtype)
-                out.append(_hip.to_device(draw, np.float32, self._dev))
-        return out
+            if k:
+                host[off:off + k, :W] = self.layers[i].rng.standard_normal((k, W), dtype=self.dtype)
+            spans.append((off, k))
+            off += k
+        dev = _hip.to_device(host, None, self._dev)
+        return [dev[o:o + k, :W] if k else None for o, k in spans]
 
     def set_time(self, t: float) -> None:
         """Advance every layer to time ``t`` [s] (monotonic)."""
@@ -181,7 +190,8 @@ class ExtrudedAtmosphere:
             betas = self._betas([len(p) for p in plans])
             extrude_rounds([ly.A for ly in self.layers], [ly.B for ly in self.layers],
                            self._buf, self.capacity, self.width, self.m, plans, betas,
-                           [ly.bscale for ly in self.layers], far_points=self.far_points)
+                           [ly.bscale for ly in self.layers], far_points=self.far_points,
+                           step=self._step)
 
     def boil_step(self, dt: float) -> None:
         """One AR(1) blend of every boiling layer's readable window toward a

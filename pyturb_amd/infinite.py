@@ -189,8 +189,93 @@ def device_seed_factor(width, stencil_rows, pixel_scale, L0, dev):
     return hit
 
 
-def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, far_points=0):
+class StepState:
+    """Device state of the one-launch extrusion step (pt_extrude_step): the
+    grid barrier's words (zeroed once, then carried from call to call) and a
+    pinned status word the kernel sets if a barrier wait times out."""
+
+    SPIN_LIMIT = 1 << 22  # barrier polls before a workgroup gives up (seconds)
+    SYNC_BYTES = 2048     # PT_EXTRUDE_SYNC_BYTES
+
+    def __init__(self, dev, far=()) -> None:
+        t = _hip.torch()
+        self.sync = t.zeros((self.SYNC_BYTES // 8,), dtype=t.int64, device=dev)
+        self.status = t.zeros((1,), dtype=t.int32, pin_memory=True)
+        self.far = tuple(int(d) for d in far)
+        self.enabled = True
+        self.spin_limit = self.SPIN_LIMIT
+        self.launches = 0
+
+    def check(self) -> None:
+        """Raise if an earlier step's barrier timed out (the rows it extruded
+        are then incomplete); the barrier words are reset first."""
+        if int(self.status[0]) != 0:
+            self.status.zero_()
+            self.sync.zero_()
+            raise _hip.HipError("pt_extrude_step: a grid barrier timed out or was aborted; the extrusion state "
+                                "is incomplete")
+
+
+def _extrude_step(step, A, B, buf, cap, width, m, plans, betas, bscales, far_points):
+    """One pt_extrude_step launch for every layer's planned rows, or False when
+    the call shape is outside what the one-launch step implements (then the
+    caller runs the round-by-round path)."""
+    t = _hip.torch()
+    L = len(plans)
+    if L > 16 or m > 2 or any(a is not A[0] for a in A) or any(b is not B[0] for b in B):
+        return False
+    first, rows, ptrs, pitch, keep = [], [], [], [], []
+    for l in range(L):
+        p = plans[l]
+        rows.append(len(p))
+        first.append(p[0][0] if p else 0)
+        if any(p[i][0] != (p[0][0] + i) % cap for i in range(len(p))):
+            return False  # rows of a step are consecutive modulo cap
+        beta = betas[l]
+        if not p:
+            ptrs.append(None)
+            pitch.append(width)
+            continue
+        if beta.dtype != t.float32 or beta.device != buf.device or beta.dim() != 2 \
+                or beta.shape[1] != width or beta.stride(1) != 1:
+            beta = beta.to(device=buf.device, dtype=t.float32).reshape(-1, width).contiguous()
+        if beta.shape[0] != len(p):
+            raise ValueError(f"layer {l}: {beta.shape[0]} innovation rows for {len(p)} new rows")
+        if beta.data_ptr() % 16 or beta.stride(0) % 4:
+            # the kernel reads rows as float4: a 16-byte aligned copy with a
+            # pitch of a multiple of 4 (callers that want no copy supply rows
+            # laid out so, e.g. views of a (k, 4 ceil(W / 4)) tensor)
+            padded = t.zeros((beta.shape[0], (width + 3) // 4 * 4), dtype=t.float32, device=buf.device)
+            padded[:, :width] = beta
+            beta = padded
+        keep.append(beta)
+        ptrs.append(beta.data_ptr())
+        pitch.append(beta.stride(0))
+    if sum(rows) > 256 or sum(rows) == 0:
+        return sum(rows) == 0
+    step.check()
+    lib = _hip.lib()
+    R = far_points // 8
+    far = (ctypes_int * max(R, 1))(*(list(step.far[:R]) or [0]))
+    rc = lib.pt_extrude_step(_hip.ptr(A[0]), _hip.ptr(B[0]), width, int(buf.shape[-1]), m, int(far_points),
+                             far, _hip.ptr(buf), cap, L, (ctypes_int * L)(*first),
+                             (ctypes_int * L)(*rows), (_ct.c_void_p * L)(*ptrs), (ctypes_ll * L)(*pitch),
+                             (ctypes_float * L)(*[float(v) for v in bscales]), _hip.ptr(step.sync),
+                             step.status.data_ptr(), int(step.spin_limit), _hip.stream_ptr())
+    if rc == _hip.PT_ERR_UNSUPPORTED:
+        _hip.lib().pt_last_error()
+        return False
+    _hip.check(rc, "pt_extrude_step")
+    step.launches += 1
+    return True
+
+
+def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, far_points=0, step=None):
     """Run the planned rows of several layers on the device.
+
+    With ``step`` (a StepState) the whole step is one cooperative launch
+    (pt_extrude_step) when its shape allows; otherwise, or without it, the
+    innovation GEMM and one launch per round.
 
     ``plans[l]``: list of (target_phys, z_phys) -- or (target_phys, z_phys,
     far_phys) with ``far_points`` > 0 -- for layer l, in order;
@@ -205,6 +290,9 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, far_points=0
     st = _hip.stream_ptr()
     As = A if isinstance(A, (list, tuple)) else [A] * L
     Bs = B if isinstance(B, (list, tuple)) else [B] * L
+    if step is not None and step.enabled and _extrude_step(step, As, Bs, buf, cap, width, m, plans,
+                                                           betas, bscales, far_points):
+        return
     # innovation GEMM per distinct B (one per outer-scale group); the kernel
     # reads every innovation row in place through a host array of pointers
     bb_of = {}
