@@ -255,7 +255,8 @@ struct StepArgs {
 };
 static_assert(sizeof(StepArgs) <= 4096, "kernel argument block");
 
-__device__ __forceinline__ long long ring_row(long long r, long long cap) {
+// physical ring row of virtual row r (rings smaller than 2^31 floats: 32 bits)
+__device__ __forceinline__ int ring_row32(int r, int cap) {
   r %= cap;
   return r < 0 ? r + cap : r;
 }
@@ -269,21 +270,28 @@ __device__ __forceinline__ int nth_layer(unsigned live, int e) {
   return m ? __ffs(m) - 1 : -1;
 }
 
-// Grid barrier number `target`; false when it timed out here or anywhere in
-// the grid.  Only wave 0 of the workgroup has stored rows since the last
-// barrier, so its write-back covers them.
-__device__ bool step_grid_sync(const StepArgs& a, unsigned long long target) {
+// Grid barrier number `target`, in two halves.  step_arrive (wave 0 only,
+// after it stored the workgroup's rows: only wave 0 stores rows, so its
+// write-back covers them) publishes the workgroup's flag; step_wait (every
+// thread; the last wave polls, so polling starts while wave 0 still writes)
+// returns false when the barrier timed out here or anywhere in the grid.
+__device__ __forceinline__ void step_arrive(const StepArgs& a, unsigned long long target) {
+  __threadfence();  // this wave's row stores reach memory before the flag
+  if (threadIdx.x == 0)
+    __hip_atomic_store(a.sync + XS_FLAGS + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ bool step_wait(const StepArgs& a, unsigned long long target, int poll_wave) {
   __shared__ int ok;
-  if (threadIdx.x < 64) {
-    unsigned long long* flags = a.sync + XS_FLAGS;
-    __threadfence();  // this wave's row stores reach memory before the flag
-    if (threadIdx.x == 0) __hip_atomic_store(flags + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int lane = threadIdx.x & 63;
+  if ((int)(threadIdx.x >> 6) == poll_wave) {
+    const unsigned long long* flags = a.sync + XS_FLAGS;
     int good = 1;
     unsigned it = 0;
     for (;;) {
       const unsigned long long abort = __hip_atomic_load(a.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       int done = 1;
-      for (int q = threadIdx.x; q < (int)gridDim.x; q += 64)
+      for (int q = lane; q < (int)gridDim.x; q += 64)
         done &= __hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
       if (__any(abort != 0) || ++it > a.spin_limit) {  // the abort word is checked first
         good = 0;
@@ -293,7 +301,7 @@ __device__ bool step_grid_sync(const StepArgs& a, unsigned long long target) {
       __builtin_amdgcn_s_sleep(1);
     }
     __threadfence();  // the other workgroups' rows are read fresh after this
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       if (!good) {
         __hip_atomic_store(a.sync + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -471,66 +479,72 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
   if (wave < fch) afar[wave][lane] = *(const float4*)(arow + (size_t)M * a.ld + 16 * wave + 4 * kq);
   __syncthreads();
   const unsigned long long base = base_s;
-  // --- the rounds: round r extrudes row r of every layer that has one
-  for (int r = 0; r < a.rounds; ++r) {
-    // entries of this round: column e is the e-th layer with cnt > r; ring
-    // rows as 32-bit float offsets from buf (the rings hold fewer than 2^31 floats)
-    unsigned live = 0;
+  // --- the rounds: round r extrudes row r of every layer that has one.
+  // Loads of the older stencil rows and the far points of round r + 1 do not
+  // depend on round r, so they are issued before the barrier that ends it.
+  unsigned live = 0;
+  int zo[M], fo[FAR_RMAX];
+  float zf[2][4];
+  // entries of round r: column e is the e-th layer with cnt > r; ring rows as
+  // 32-bit float offsets from buf (the rings hold fewer than 2^31 floats)
+  auto setup = [&](int r) {
+    live = 0;
 #pragma unroll
     for (int q = 0; q < XS_LMAX; ++q) live |= (lay_cnt[q] > r ? 1u : 0u) << q;
     const int le = nth_layer(live, ri);
-    const long long t0 = le >= 0 ? lay_t0[le] : 0;
-    int zo[M], fo[FAR_RMAX];
 #pragma unroll
     for (int s = 0; s < M; ++s) zo[s] = 0;
 #pragma unroll
     for (int j = 0; j < FAR_RMAX; ++j) fo[j] = 0;
     if (le >= 0) {
-      const long long slab = (long long)le * a.cap;
-      const long long target = ring_row(t0 + r, a.cap);
+      const int cap = (int)a.cap, slab = le * cap;  // 32-bit: the rings hold fewer than 2^31 floats
+      const int target = ring_row32(lay_t0[le] + r, cap);
 #pragma unroll
-      for (int s = 0; s < M; ++s) zo[s] = (int)((slab + ring_row(target - M + s, a.cap)) * a.ld);
+      for (int s = 0; s < M; ++s) zo[s] = (slab + ring_row32(target - M + s, cap)) * a.ld;
 #pragma unroll
       for (int j = 0; j < FAR_RMAX; ++j)
-        if (8 * j < a.nfar) fo[j] = (int)((slab + ring_row(target - 1 - a.far_d[j], a.cap)) * a.ld);
+        if (8 * j < a.nfar) fo[j] = (slab + ring_row32(target - 1 - a.far_d[j], cap)) * a.ld;
     }
     // far points of both virtual waves (k_mfma_gemv's far segment)
-    float zf[2][4];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int vw = wave + XS_WAVES * h;
-      const int k = 16 * vw + 4 * kq, fr = 2 * vw + (kq >> 1), i0 = 4 * (kq & 1);
+      const int fr = 2 * vw + (kq >> 1), i0 = 4 * (kq & 1);
 #pragma unroll
       for (int u = 0; u < 4; ++u) zf[h][u] = 0.f;
       if (vw < fch) {  // wave-uniform
         const int fb = fr == 0 ? fo[0] : fr == 1 ? fo[1] : fr == 2 ? fo[2] : fo[3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float v = a.buf[(unsigned)(fb + ((2 * (i0 + u) + 1) * a.W) / (2 * FAR_COLS))];
-          zf[h][u] = k < a.nfar ? v : 0.f;  // points past nfar meet A's zero padding: keep them finite
-        }
+        for (int u = 0; u < 4; ++u) zf[h][u] = a.buf[(unsigned)(fb + ((2 * (i0 + u) + 1) * a.W) / (2 * FAR_COLS))];
       }
     }
-    // stencil rows in groups g = (h, s, gi): chunks XS_GRP gi + [0, XS_GRP), loaded XS_NBUF - 1 groups ahead
-    constexpr int G = 2 * M * XS_NG;
-    float4 zb[XS_NBUF][XS_GRP];
-    mfma_v4f acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
-    auto load_group = [&](int g, float4 (&z)[XS_GRP]) {
-      const int h = g / (M * XS_NG), s = (g / XS_NG) % M, vw = wave + XS_WAVES * h;
+  };
+  // stencil rows in groups g = (h, s, gi): chunks XS_GRP gi + [0, XS_GRP), loaded XS_NBUF - 1 groups ahead
+  constexpr int G = 2 * M * XS_NG;
+  // groups 0 .. EARLY - 1 read segment 0, an older stencil row when M >= 2
+  constexpr int EARLY = M >= 2 ? (XS_NBUF - 1 < XS_NG ? XS_NBUF - 1 : XS_NG) : 0;
+  float4 zb[XS_NBUF][XS_GRP];
+  auto load_group = [&](int g, float4 (&z)[XS_GRP]) {
+    const int h = g / (M * XS_NG), s = (g / XS_NG) % M, vw = wave + XS_WAVES * h;
 #pragma unroll
-      for (int u = 0; u < XS_GRP; ++u) {
-        const int i = XS_GRP * (g % XS_NG) + u, c = vw + MF_WAVES * i;
-        // unconditional loads (the compiler drains loads in flight where
-        // branches around loads join): a chunk that does not exist is never
-        // multiplied, and a column without a layer (le < 0, zo = 0) only
-        // feeds its own unused output column
-        const int k = (i < XS_CWMAX && i < a.cw && c < cps) ? 16 * c + 4 * kq : 4 * kq;
-        z[u] = *(const float4*)(a.buf + (unsigned)(zo[s] + k));
-      }
-    };
+    for (int u = 0; u < XS_GRP; ++u) {
+      const int i = XS_GRP * (g % XS_NG) + u, c = vw + MF_WAVES * i;
+      // unconditional loads (the compiler drains loads in flight where
+      // branches around loads join): a chunk that does not exist is never
+      // multiplied, and a column without a layer (le < 0, zo = 0) only
+      // feeds its own unused output column
+      const int k = (i < XS_CWMAX && i < a.cw && c < cps) ? 16 * c + 4 * kq : 4 * kq;
+      z[u] = *(const float4*)(a.buf + (unsigned)(zo[s] + k));
+    }
+  };
+  setup(0);
 #pragma unroll
-    for (int g = 0; g < XS_NBUF - 1; ++g)
+  for (int g = 0; g < EARLY; ++g) load_group(g, zb[g]);
+  for (int r = 0; r < a.rounds; ++r) {
+#pragma unroll
+    for (int g = EARLY; g < XS_NBUF - 1; ++g)
       if (g < G) load_group(g, zb[g]);
+    mfma_v4f acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (g + XS_NBUF - 1 < G) load_group(g + XS_NBUF - 1, zb[(g + XS_NBUF - 1) % XS_NBUF]);
@@ -548,11 +562,13 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
       }
       if (g % (M * XS_NG) == M * XS_NG - 1) {  // the last group of virtual wave vw: far points, then its partial tile
         if (vw < fch) {
+          // points past nfar meet A's zero padding: zero them so they stay finite
+          const bool fin = 16 * vw + 4 * kq < a.nfar;
           const float4 x = afar[vw][lane];
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, zf[h][0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, zf[h][1], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, zf[h][2], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, zf[h][3], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, fin ? zf[h][0] : 0.f, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, fin ? zf[h][1] : 0.f, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, fin ? zf[h][2] : 0.f, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, fin ? zf[h][3] : 0.f, acc, 0, 0, 0);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) red[0][vw][4 * kq + q][ri] = acc[q];
@@ -571,19 +587,29 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
       }
     }
     __syncthreads();
+    const bool more = r + 1 < a.rounds;
     if (wave == 0) {  // one wave stores the rows, so one write-back publishes them
       const int e = lane & 15;
       const int l = nth_layer(live, e);
       if (l >= 0) {
-        float* dst = a.buf + ((size_t)l * a.cap + ring_row((long long)lay_t0[l] + r, a.cap)) * a.ld + row0;
+        float* dst = a.buf + (unsigned)((l * (int)a.cap + ring_row32(lay_t0[l] + r, (int)a.cap)) * a.ld + row0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int rr = (lane >> 4) + 4 * q;
           if (row0 + rr < a.W) dst[rr] = rowv[rr][e];
         }
       }
+      if (more) step_arrive(a, base + r + 1);
     }
-    if (r + 1 < a.rounds && !step_grid_sync(a, base + r + 1)) return;
+    if (more) {
+      // round r + 1's rows that round r does not write: issued now, used
+      // after the barrier (wave 0 issues them after its flag store, whose
+      // fence would wait for them)
+      setup(r + 1);
+#pragma unroll
+      for (int g = 0; g < EARLY; ++g) load_group(g, zb[g]);
+      if (!step_wait(a, base + r + 1, XS_WAVES - 1)) return;
+    }
   }
   // the barrier count for the next call (every workgroup read it before barrier 0)
   if (blockIdx.x == 0 && threadIdx.x == 0)
