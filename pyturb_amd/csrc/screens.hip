@@ -355,15 +355,27 @@ __device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp
   // come from two small tables, exp(2 pi i 2^-32 s' u) = T8[a] T1[b] with
   // s' + N/2 - 1 = 8a + b, the other axis' N/2 term folded into T8: one
   // complex product per entry instead of a sine and cosine.
-  // Tables of LB layers are built per pair of barriers (at most 34 KiB).
+  // Tables of LB layers are built per pair of barriers (at most 37 KiB):
+  // every layer at once up to n = 512, blocks of 4 above.
   constexpr int QMAX = (2 * N + 255) / 256;
-  constexpr int LB = 4;
+  constexpr int LB = N <= 512 ? FOLD_LMAX : 4;
   __shared__ float2 t8s[LB][2][2][NB], t1s[LB][2][2][8];  // [layer][frame][varying axis]
   float2 acc[QMAX];
 #pragma unroll
   for (int i = 0; i < QMAX; ++i) acc[i] = make_float2(0.f, 0.f);
+  // n <= 1024: a block's side values are loaded before its tables are built
+  // (a load per layer in the sum below would make it a chain of latencies)
+  constexpr bool PRE = QMAX * LB <= 48;
+  float2 sv[PRE ? LB : 1][PRE ? QMAX : 1];
   for (int l0 = 0; l0 < L; l0 += LB) {
     const int nl = min(LB, L - l0);
+    if constexpr (PRE) {
+#pragma unroll
+      for (int lb = 0; lb < LB; ++lb)
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i)
+          sv[lb][i] = side[(size_t)(l0 + min(lb, nl - 1)) * 2 * N + min(first + stride * i, 2 * N - 1)];
+    }
     __syncthreads();  // the previous layers' tables are no longer read
     for (int e = threadIdx.x; e < nl * 4 * (NB + 8); e += 256) {
       const int lb = e / (4 * (NB + 8)), r = e % (4 * (NB + 8));
@@ -381,19 +393,28 @@ __device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp
         t1s[lb][f][ax][j - NB] = v;
     }
     __syncthreads();
-    for (int lb = 0; lb < nl; ++lb) {
-      const float2* sl = side + (size_t)(l0 + lb) * 2 * N;
 #pragma unroll
-      for (int i = 0; i < QMAX; ++i) {
-        const int q = first + stride * i;
-        if (q >= 2 * N || (q >= N && q - N == N / 2)) continue;
-        const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
-        const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
-        const float2 wa = cmul(t8s[lb][0][ax][id >> 3], t1s[lb][0][ax][id & 7]);
-        const float2 wb = cmul(t8s[lb][1][ax][id >> 3], t1s[lb][1][ax][id & 7]);
-        const float2 t = cmul(sl[q], make_float2(wa.x - wb.y, wa.y + wb.x));
-        acc[i].x += t.x;
-        acc[i].y += t.y;
+    for (int lb = 0; lb < LB; ++lb) {
+      if (lb < nl) {
+        const float2* sl = side + (size_t)(l0 + lb) * 2 * N;
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {
+          const int q = first + stride * i;
+          if (q < 2 * N && !(q >= N && q - N == N / 2)) {
+            const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
+            const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
+            const float2 wa = cmul(t8s[lb][0][ax][id >> 3], t1s[lb][0][ax][id & 7]);
+            const float2 wb = cmul(t8s[lb][1][ax][id >> 3], t1s[lb][1][ax][id & 7]);
+            float2 sq;
+            if constexpr (PRE)
+              sq = sv[PRE ? lb : 0][PRE ? i : 0];
+            else
+              sq = sl[q];
+            const float2 t = cmul(sq, make_float2(wa.x - wb.y, wa.y + wb.x));
+            acc[i].x += t.x;
+            acc[i].y += t.y;
+          }
+        }
       }
     }
   }

@@ -79,6 +79,28 @@ __device__ inline void sub_fill_set(const double2* C, int P, int n, int s, SubTa
   const double mean = *s_mean;
   const int cpp = (n + parts - 1) / parts, y0 = part * cpp, y1 = min(n, y0 + cpp);
   float2* H = t.H + (size_t)s * P * n;
+  if ((y1 - y0) * P <= 1024 && y1 - y0 < nthreads) {
+    // a narrow share (a call of a few sets split over many workgroups): one
+    // (level, column) term per thread instead of one column's levels in
+    // sequence, K summed over the levels in the same order afterwards
+    __shared__ double g1s[1024];
+    const int ny = y1 - y0;
+    for (int it = tid; it < ny * P; it += nthreads) {
+      const int p = it / ny, y = y0 + it % ny;
+      double2 Hd;
+      double g1;
+      sh_column(hc + p * 9, basis ? basis[(size_t)p * n + y] : sh_basis_e(p, y, n), Hd, g1);
+      H[(size_t)p * n + y] = make_float2((float)Hd.x, (float)Hd.y);
+      g1s[it] = g1;
+    }
+    __syncthreads();
+    for (int y = tid; y < ny; y += nthreads) {
+      double kacc = -mean;
+      for (int p = 0; p < P; ++p) kacc += g1s[p * ny + y];
+      t.K[(size_t)s * n + y0 + y] = (float)kacc;
+    }
+    return;
+  }
   for (int y = y0 + tid; y < y1; y += nthreads) {
     double kacc = -mean;
     for (int p = 0; p < P; ++p) {
@@ -113,13 +135,24 @@ __device__ inline void frame_sh_blocks(double2* Q, double2* SB, const double* di
   for (int q = tid; q < 9 * P; q += nthreads) {
     const int p = q / 9, k = q % 9, ia = k / 3, ib = k % 3;
     double2 acc = make_double2(0.0, 0.0);
-    for (int l = 0; l < L; ++l) {
-      const double2 qx = Q[2 * (p * L + l) + 0], qy = Q[2 * (p * L + l) + 1];
-      const double2 wx = ia == 0 ? dconj(qx) : (ia == 1 ? make_double2(1.0, 0.0) : qx);
-      const double2 wy = ib == 0 ? dconj(qy) : (ib == 1 ? make_double2(1.0, 0.0) : qy);
-      const double2 t = dcmul(dcmul(coeffs[((size_t)p * L + l) * 9 + k], wx), wy);
-      acc.x += t.x;
-      acc.y += t.y;
+    // the coefficients of 16 layers are loaded before the first is used (a
+    // load per iteration would make the sum a chain of load latencies)
+    for (int l0 = 0; l0 < L; l0 += 16) {
+      double2 cf[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) cf[i] = coeffs[((size_t)p * L + min(l0 + i, L - 1)) * 9 + k];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int l = l0 + i;
+        if (l < L) {
+          const double2 qx = Q[2 * (p * L + l) + 0], qy = Q[2 * (p * L + l) + 1];
+          const double2 wx = ia == 0 ? dconj(qx) : (ia == 1 ? make_double2(1.0, 0.0) : qx);
+          const double2 wy = ib == 0 ? dconj(qy) : (ib == 1 ? make_double2(1.0, 0.0) : qy);
+          const double2 t = dcmul(dcmul(cf[i], wx), wy);
+          acc.x += t.x;
+          acc.y += t.y;
+        }
+      }
     }
     SB[tr ? p * 9 + ib * 3 + ia : q] = acc;
   }
