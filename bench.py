@@ -295,7 +295,7 @@ LGS_ALT = 90e3
 LGS_FRAMES = 512  # spectral LGS-cone frames per timed call (9 per-layer transforms each)
 
 
-def build_extrude(dev, far_points=32):
+def build_extrude(dev, far_points=32, steps=EXT_STEPS):
     """9-layer 1024^2 extruder, 30" field, innovations from seed 3 in HBM;
     stencil of 2 full rows plus ``far_points`` far pixels."""
     import torch
@@ -313,7 +313,7 @@ def build_extrude(dev, far_points=32):
     gen.manual_seed(3)
     # rows each layer extrudes over the run: travel + lookahead, with margin
     speeds = np.array([ly.wind_speed for ly in layers])
-    rows = (speeds * EXT_STEPS * 1e-3 / dx + 8).astype(int)
+    rows = (speeds * steps * 1e-3 / dx + 8).astype(int)
     pool = {}
     used = [0] * len(layers)
     ext = None
@@ -508,6 +508,10 @@ def main(argv=None):
                     help="skip the 16-atmosphere HBM-streamed frames variant")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the one-frame-per-call loop (eager vs HIP-graph replay)")
+    ap.add_argument("--extrude-far", choices=("both", "32", "0"), default="both",
+                    help="extrusion stencils to run: 2 rows + 32 far points, 2 rows, or both")
+    ap.add_argument("--extrude-steps", type=int, default=1000,
+                    help="extrusion steps per stencil (20 of them untimed warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (counter scaling)")
     args = ap.parse_args(argv)
@@ -583,21 +587,26 @@ def main(argv=None):
     if not args.no_extrude:
         # the stated stencil (2 rows + 32 far points), then the reference's
         # full-row stencil alone
-        for far, key in ((32, "extrude"), (0, "extrude_rows_only")):
+        legs = [(32, "extrude"), (0, "extrude_rows_only")]
+        if args.extrude_far != "both":
+            legs = [lg for lg in legs if str(lg[0]) == args.extrude_far]
+        ext_steps = max(21, args.extrude_steps)
+        for far, key in legs:
             print(f"bench: building 9-layer 1024^2 extruder, {far} far points (host FP64 A/B)",
                   file=sys.stderr, flush=True)
             tb = time.perf_counter()
-            ext, thx, thy, mags, eout = build_extrude(dev, far)
+            ext, thx, thy, mags, eout = build_extrude(dev, far, ext_steps)
             extra[f"{key}_setup_s"] = round(time.perf_counter() - tb, 1)
             run_extrude(ext, thx, thy, mags, eout, 20)
             torch.cuda.synchronize()
             te0 = time.perf_counter()
-            run_extrude(ext, thx, thy, mags, eout, EXT_STEPS - 20, t0=20)
+            run_extrude(ext, thx, thy, mags, eout, ext_steps - 20, t0=20)
             torch.cuda.synchronize()
             te = time.perf_counter() - te0
-            extra[f"{key}_steps_per_s"] = round((EXT_STEPS - 20) / te, 1)
-            extra[f"{key}_frames_per_s_7dirs"] = round(7 * (EXT_STEPS - 20) / te, 1)
+            extra[f"{key}_steps_per_s"] = round((ext_steps - 20) / te, 1)
+            extra[f"{key}_frames_per_s_7dirs"] = round(7 * (ext_steps - 20) / te, 1)
             extra["extrude_width"] = ext.width
+            extra["extrude_steps"] = ext_steps
             extra["extrude_far_points"] = 32
             del ext, eout
             torch.cuda.empty_cache()

@@ -294,7 +294,8 @@ int pt_extrude_rows_far(const float* A, int width, int ld, int stencil_rows,
  * aligned, and pitches in floats, multiples of 4; a row is read up to its
  * width rounded up to 4, the floats past the width are ignored), each row's
  * stencil being the m ring rows before it (and the
- * far rows far_rows_back[j] + 1 behind it).  A cooperative launch: each
+ * far rows far_rows_back[j] + 1 behind it).  One launch on `stream` whose
+ * workgroups (one per 16 rows of A) are all resident at once: each
  * workgroup keeps its 16 rows of A in registers for the whole step, and
  * the rounds are separated by a grid barrier on `sync`, PT_EXTRUDE_SYNC_BYTES
  * of device memory (8-byte aligned) that is zero before the first call and

@@ -204,7 +204,7 @@ static int launch_gemv(GemvArgs& a, bool zv4, hipStream_t st) {
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 // ---------------------------------------------------------------------------
-// One extrusion step in one cooperative launch (k_extrude_step)
+// One extrusion step in one launch (k_extrude_step)
 // ---------------------------------------------------------------------------
 // The step's innovation terms and every recurrence round of k_mfma_gemv in a
 // single launch: workgroup g owns matrix rows [16 g, 16 g + 16) of A and B
@@ -934,6 +934,10 @@ __device__ __forceinline__ TileGeo tile_geo(const TileArgs& a, int d, int l, int
 //    box row k once per layer; the loads take it with v_readlane, so a box
 //    row costs one address add.  Every lane loads (columns past the box are
 //    clamped reads of the row), no lane masking.
+//  * The per-layer box geometry (FP64 tile origin, box extents) is worked
+//    out once per wave, lane l for layer l, and broadcast to scalar
+//    registers with v_readlane when the layer comes up: ten instructions per
+//    layer instead of the FP64 arithmetic in every lane.
 template <int TAPS>
 __global__ __launch_bounds__(256) void k_readout_tile(TileArgs a, int tiles_x, int ntiles) {
   using C = TileCfg<TAPS>;
@@ -972,7 +976,22 @@ __global__ __launch_bounds__(256) void k_readout_tile(TileArgs a, int tiles_x, i
       if (wv + 4 * k < g.nr) box[sel][(wave + 4 * k) * BP + lane] = pre[k];
     }
   };
-  TileGeo cur = tile_geo<TAPS>(a, d, 0, i0, j0);
+  // lane l < L: layer l's box; geo(l) reads it back into scalar registers
+  const TileGeo mine = tile_geo<TAPS>(a, d, min(lane, a.L - 1), i0, j0);
+  auto geo = [&](int l) {
+    TileGeo g;
+    g.r0 = __builtin_amdgcn_readlane(mine.r0, l);
+    g.c0 = __builtin_amdgcn_readlane(mine.c0, l);
+    g.nr = __builtin_amdgcn_readlane(mine.nr, l);
+    g.nc = __builtin_amdgcn_readlane(mine.nc, l);
+    g.fr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.fr), l));
+    g.fc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.fc), l));
+    g.p = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.p), l));
+    g.q = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.q), l));
+    g.fits = __builtin_amdgcn_readlane((int)mine.fits, l) != 0;
+    return g;
+  };
+  TileGeo cur = geo(0);
   fetch(0, cur);
   commit(0, cur);
   __syncthreads();
@@ -981,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_readout_tile(TileArgs a, int tiles_x, i
     TileGeo nxt;
     nxt.fits = false;
     if (l + 1 < a.L) {
-      nxt = tile_geo<TAPS>(a, d, l + 1, i0, j0);
+      nxt = geo(l + 1);
       fetch(l + 1, nxt);  // in flight while layer l is interpolated
     }
     // per-thread row terms: row = rj + di p, col = cj - di q
@@ -1370,6 +1389,35 @@ extern "C" int pt_rows_blend(float* dst, long long dst_pitch, int dst_start, int
   return PT_OK;
 }
 
+// Workgroups of k_extrude_step<m> that the current device holds at once
+// (occupancy per CU x CUs), cached per device and stencil; -1 on error.
+//
+// The step is an ordinary launch on the caller's stream, not a cooperative
+// one: a cooperative launch goes to a separate hardware queue, and each hop
+// between it and the stream's queue cost about 9 us per step (the readout
+// runs on the stream).  The grid barrier only needs every workgroup of the
+// launch to become resident eventually, and with grid <= this capacity it
+// does: the stream is in order, so the previous kernel of the stream has
+// finished; no workgroup of this launch holds a slot that another of its
+// workgroups needs; and a CU held by another stream's kernel frees up when
+// that kernel's workgroups end, which does not depend on this launch.  The
+// bounded wait (spin_limit polls, then the abort word) still ends every
+// workgroup if that takes too long.
+static int step_capacity(const void* kern, int m) {
+  static int cache[64][3];  // [device][m]: capacity + 1, 0 = not yet known
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+  if (cache[dev][m] > 0) return cache[dev][m] - 1;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * XS_WAVES, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  cache[dev][m] = per_cu * cus + 1;
+  return per_cu * cus;
+}
+
 extern "C" int pt_extrude_step(const float* A, const float* B, int width, int ld, int stencil_rows, int far_points,
                                const int* far_rows_back, float* buf, long long cap, int layers, const int* first_row,
                                const int* rows, const float* const* beta, const long long* beta_pitch,
@@ -1432,11 +1480,14 @@ extern "C" int pt_extrude_step(const float* A, const float* B, int width, int ld
   if (grid > XS_GRIDMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: width over %d", 16 * XS_GRIDMAX);
   void* args[] = {&a};
   const void* kern = stencil_rows == 1 ? (const void*)k_extrude_step<1> : (const void*)k_extrude_step<2>;
-  hipError_t e = hipLaunchCooperativeKernel(kern, dim3(grid), dim3(64 * XS_WAVES), args, 0, (hipStream_t)stream);
+  const int resident = step_capacity(kern, stencil_rows);
+  if (resident < 0) return fail(PT_ERR_HIP, "pt_extrude_step: occupancy query failed");
+  if (grid > resident)
+    return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: %d workgroups, at most %d resident at once", grid, resident);
+  hipError_t e = hipLaunchKernel(kern, dim3(grid), dim3(64 * XS_WAVES), args, 0, (hipStream_t)stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    return fail(e == hipErrorCooperativeLaunchTooLarge ? PT_ERR_UNSUPPORTED : PT_ERR_HIP,
-                "pt_extrude_step: cooperative launch of %d workgroups: %s", grid, hipGetErrorString(e));
+    return fail(PT_ERR_HIP, "pt_extrude_step: launch of %d workgroups: %s", grid, hipGetErrorString(e));
   }
   return PT_OK;
 }

@@ -6,7 +6,7 @@ exactly (host integers and float64); the data path is on the device:
 * one (layers, cap, W) float32 ring-buffer array in HBM; a compaction is an
   index shift (:class:`pyturb_amd.ring.Ring`), never a copy;
 * ``set_time`` plans every layer's new rows on the host, then one
-  cooperative launch (pt_extrude_step) computes all their innovation terms
+  launch (pt_extrude_step) computes all their innovation terms
   ``B_unit beta`` and runs the sequential ``A z`` recurrence round by round
   with A held in registers and a grid barrier between rounds; shapes it does
   not cover run the innovation GEMM and one launch per round instead (the
@@ -40,7 +40,8 @@ from . import _hip
 from .backend import resolve_device
 from .config import ExtrusionConfig
 from .fourier import PhaseScreen
-from .infinite import (StepState, device_extrusion, device_seed_factor, extrude_rounds, far_rows_back,
+from .infinite import (StepState, device_extrusion, device_seed_factor, extrude_rounds,
+                       extrude_step_or_rounds, far_rows_back,
                        readout, row_pitch)
 from .ring import Ring
 
@@ -147,7 +148,18 @@ class ExtrudedAtmosphere:
             ly.travel = 0.0
             self.layers.append(ly)
         self._wind_cs = [(ly.cos, ly.sin) for ly in self.layers]
-        # each set_time is one cooperative launch where its shape allows
+        self._As = [ly.A for ly in self.layers]
+        self._Bs = [ly.B for ly in self.layers]
+        self._bscales = [ly.bscale for ly in self.layers]
+        # readout arguments kept as arrays (per-layer constants; ring state and
+        # direction geometry refreshed per call)
+        self._alt_pix = np.array([ly.alt_pix for ly in self.layers])
+        self._cos = np.array([ly.cos for ly in self.layers])
+        self._sin = np.array([ly.sin for ly in self.layers])
+        self._mag = np.array([ly.mag for ly in self.layers])
+        self._dir_cache = None
+        self._wind_cs_np = np.array(self._wind_cs, dtype=np.float64)
+        # each set_time is one pt_extrude_step launch where its shape allows
         # (pt_extrude_step), else the innovation GEMM and a launch per round
         self._step = StepState(self._dev, far)
         # boiling (extrude.py:586-654): one stream for every layer; the device
@@ -178,20 +190,22 @@ class ExtrudedAtmosphere:
 
     def set_time(self, t: float) -> None:
         """Advance every layer to time ``t`` [s] (monotonic)."""
-        plans = []
+        spans = []
+        dx = self.dx
         for ly in self.layers:
-            travel = ly.speed * t / self.dx
+            travel = float(ly.speed * t / dx)
             if travel < ly.travel:
                 raise ValueError("wind travel is monotonic; travel cannot decrease")
-            ly.travel = float(travel)
-            top = int(np.ceil(ly.along_max + ly.fov + ly.travel)) + 2
-            plans.append(ly.ring.plan(top, ly.along_min - ly.fov + ly.travel))
-        if any(plans):
-            betas = self._betas([len(p) for p in plans])
-            extrude_rounds([ly.A for ly in self.layers], [ly.B for ly in self.layers],
-                           self._buf, self.capacity, self.width, self.m, plans, betas,
-                           [ly.bscale for ly in self.layers], far_points=self.far_points,
-                           step=self._step)
+            ly.travel = travel
+            top = math.ceil(ly.along_max + ly.fov + travel) + 2
+            spans.append(ly.ring.extend(top, ly.along_min - ly.fov + travel))
+        counts = [k for _, k in spans]
+        if any(counts):
+            betas = self._betas(counts)
+            layers = self.layers
+            extrude_step_or_rounds(self._As, self._Bs, self._buf, self.capacity, self.width, self.m,
+                                   spans, lambda l: layers[l].ring.rows(*spans[l]), betas,
+                                   self._bscales, far_points=self.far_points, step=self._step)
 
     def boil_step(self, dt: float) -> None:
         """One AR(1) blend of every boiling layer's readable window toward a
@@ -268,25 +282,34 @@ class ExtrudedAtmosphere:
         return (self._boil_index * 64 + layer) * 2 + kind
 
     def dir_geometry(self, thx, thy, mags=None) -> np.ndarray:
-        """(D, L, 3) readout geometry: along/perp shifts and magnification."""
+        """(D, L, 3) readout geometry: along/perp shifts and magnification.
+
+        The direction terms depend only on the directions, so the last set is
+        kept; each call adds the layers' current travel - base."""
         thx = np.atleast_1d(np.asarray(thx, dtype=np.float64))
         thy = np.atleast_1d(np.asarray(thy, dtype=np.float64))
-        geom = np.empty((thx.size, self.n_layers, 3))
-        for l, ly in enumerate(self.layers):
-            dxp = ly.alt_pix * thx
-            dyp = ly.alt_pix * thy
-            geom[:, l, 0] = (ly.travel - ly.ring.base) + (dxp * ly.cos + dyp * ly.sin)
-            geom[:, l, 1] = -dxp * ly.sin + dyp * ly.cos
-            geom[:, l, 2] = ly.mag
-        if mags is not None:
-            geom[:, :, 2] = np.broadcast_to(np.asarray(mags, dtype=np.float64), geom.shape[:2])
+        key = (thx.tobytes(), thy.tobytes(),
+               None if mags is None else np.asarray(mags, dtype=np.float64).tobytes())
+        c = self._dir_cache
+        if c is None or c[0] != key:
+            dxp = self._alt_pix[None, :] * thx[:, None]
+            dyp = self._alt_pix[None, :] * thy[:, None]
+            base = np.empty((thx.size, self.n_layers, 3))
+            base[:, :, 0] = dxp * self._cos + dyp * self._sin
+            base[:, :, 1] = -dxp * self._sin + dyp * self._cos
+            base[:, :, 2] = self._mag
+            if mags is not None:
+                base[:, :, 2] = np.broadcast_to(np.asarray(mags, dtype=np.float64), base.shape[:2])
+            c = self._dir_cache = (key, base)
+        geom = c[1].copy()
+        geom[:, :, 0] = np.array([ly.travel - ly.ring.base for ly in self.layers]) + c[1][:, :, 0]
         return geom
 
     def integrate_dirs(self, thx, thy, scale: float = 1.0, mags=None, out=None) -> Any:
         """Summed phase for D directions (tangents) -> (D, n, n) float32."""
         geom = self.dir_geometry(thx, thy, mags)
         return readout(self._buf, self.capacity, self.width, [ly.ring for ly in self.layers],
-                       self._wind_cs, geom, self.n, self.interp, scale=scale, out=out)
+                       self._wind_cs_np, geom, self.n, self.interp, scale=scale, out=out)
 
     def integrate(self, thx: float = 0.0, thy: float = 0.0) -> Any:
         out = self.integrate_dirs([thx], [thy])[0]

@@ -197,71 +197,90 @@ class StepState:
     SPIN_LIMIT = 1 << 22  # barrier polls before a workgroup gives up (seconds)
     SYNC_BYTES = 2048     # PT_EXTRUDE_SYNC_BYTES
 
+    LMAX = 16             # layers per launch
+
     def __init__(self, dev, far=()) -> None:
         t = _hip.torch()
         self.sync = t.zeros((self.SYNC_BYTES // 8,), dtype=t.int64, device=dev)
         self.status = t.zeros((1,), dtype=t.int32, pin_memory=True)
+        self._status = self.status.numpy()  # host view of the pinned word
         self.far = tuple(int(d) for d in far)
+        R = max(1, len(self.far))
+        self.far_arr = (ctypes_int * R)(*(list(self.far) or [0]))
         self.enabled = True
         self.spin_limit = self.SPIN_LIMIT
         self.launches = 0
+        # launch arguments, filled in place per call
+        L = self.LMAX
+        self.first = (ctypes_int * L)()
+        self.rows = (ctypes_int * L)()
+        self.ptrs = (_ct.c_void_p * L)()
+        self.pitch = (ctypes_ll * L)()
+        self.scale = (ctypes_float * L)()
 
     def check(self) -> None:
         """Raise if an earlier step's barrier timed out (the rows it extruded
         are then incomplete); the barrier words are reset first."""
-        if int(self.status[0]) != 0:
+        if self._status[0] != 0:
             self.status.zero_()
             self.sync.zero_()
             raise _hip.HipError("pt_extrude_step: a grid barrier timed out or was aborted; the extrusion state "
                                 "is incomplete")
 
 
-def _extrude_step(step, A, B, buf, cap, width, m, plans, betas, bscales, far_points):
-    """One pt_extrude_step launch for every layer's planned rows, or False when
+def _extrude_step(step, A, B, buf, cap, width, m, spans, betas, bscales, far_points):
+    """One pt_extrude_step launch for every layer's new rows, or False when
     the call shape is outside what the one-launch step implements (then the
-    caller runs the round-by-round path)."""
+    caller runs the round-by-round path).
+
+    ``spans[l]`` = (first_target_phys, count): layer l's new rows are the
+    physical rows (first + i) % cap, i < count."""
     t = _hip.torch()
-    L = len(plans)
-    if L > 16 or m > 2 or any(a is not A[0] for a in A) or any(b is not B[0] for b in B):
+    L = len(spans)
+    if L > StepState.LMAX or m > 2 or any(a is not A[0] for a in A) or any(b is not B[0] for b in B):
         return False
-    first, rows, ptrs, pitch, keep = [], [], [], [], []
+    total = sum(k for _, k in spans)
+    if total == 0:
+        return True
+    if total > 256:
+        return False
+    f32 = t.float32
+    dev = buf.get_device()
+    keep = []
     for l in range(L):
-        p = plans[l]
-        rows.append(len(p))
-        first.append(p[0][0] if p else 0)
-        if any(p[i][0] != (p[0][0] + i) % cap for i in range(len(p))):
-            return False  # rows of a step are consecutive modulo cap
-        beta = betas[l]
-        if not p:
-            ptrs.append(None)
-            pitch.append(width)
+        first, k = spans[l]
+        step.first[l] = first
+        step.rows[l] = k
+        step.scale[l] = bscales[l]
+        if not k:
+            step.ptrs[l] = None
+            step.pitch[l] = width
             continue
-        if beta.dtype != t.float32 or beta.device != buf.device or beta.dim() != 2 \
-                or beta.shape[1] != width or beta.stride(1) != 1:
-            beta = beta.to(device=buf.device, dtype=t.float32).reshape(-1, width).contiguous()
-        if beta.shape[0] != len(p):
-            raise ValueError(f"layer {l}: {beta.shape[0]} innovation rows for {len(p)} new rows")
-        if beta.data_ptr() % 16 or beta.stride(0) % 4:
+        beta = betas[l]
+        st = beta.stride()
+        if beta.dtype is not f32 or beta.get_device() != dev or beta.dim() != 2 \
+                or beta.shape[1] != width or st[1] != 1:
+            beta = beta.to(device=buf.device, dtype=f32).reshape(-1, width).contiguous()
+            st = beta.stride()
+        if beta.shape[0] != k:
+            raise ValueError(f"layer {l}: {beta.shape[0]} innovation rows for {k} new rows")
+        p = beta.data_ptr()
+        if p % 16 or st[0] % 4:
             # the kernel reads rows as float4: a 16-byte aligned copy with a
             # pitch of a multiple of 4 (callers that want no copy supply rows
             # laid out so, e.g. views of a (k, 4 ceil(W / 4)) tensor)
-            padded = t.zeros((beta.shape[0], (width + 3) // 4 * 4), dtype=t.float32, device=buf.device)
+            padded = t.zeros((k, (width + 3) // 4 * 4), dtype=f32, device=buf.device)
             padded[:, :width] = beta
             beta = padded
+            p, st = beta.data_ptr(), beta.stride()
         keep.append(beta)
-        ptrs.append(beta.data_ptr())
-        pitch.append(beta.stride(0))
-    if sum(rows) > 256 or sum(rows) == 0:
-        return sum(rows) == 0
+        step.ptrs[l] = p
+        step.pitch[l] = st[0]
     step.check()
-    lib = _hip.lib()
-    R = far_points // 8
-    far = (ctypes_int * max(R, 1))(*(list(step.far[:R]) or [0]))
-    rc = lib.pt_extrude_step(_hip.ptr(A[0]), _hip.ptr(B[0]), width, int(buf.shape[-1]), m, int(far_points),
-                             far, _hip.ptr(buf), cap, L, (ctypes_int * L)(*first),
-                             (ctypes_int * L)(*rows), (_ct.c_void_p * L)(*ptrs), (ctypes_ll * L)(*pitch),
-                             (ctypes_float * L)(*[float(v) for v in bscales]), _hip.ptr(step.sync),
-                             step.status.data_ptr(), int(step.spin_limit), _hip.stream_ptr())
+    rc = _hip.lib().pt_extrude_step(_hip.ptr(A[0]), _hip.ptr(B[0]), width, int(buf.shape[-1]), m,
+                                    int(far_points), step.far_arr, _hip.ptr(buf), cap, L, step.first,
+                                    step.rows, step.ptrs, step.pitch, step.scale, _hip.ptr(step.sync),
+                                    step.status.data_ptr(), int(step.spin_limit), _hip.stream_ptr())
     if rc == _hip.PT_ERR_UNSUPPORTED:
         _hip.lib().pt_last_error()
         return False
@@ -270,10 +289,24 @@ def _extrude_step(step, A, B, buf, cap, width, m, plans, betas, bscales, far_poi
     return True
 
 
+def extrude_step_or_rounds(A, B, buf, cap, width, m, spans, rows_of, betas, bscales, far_points=0,
+                           step=None):
+    """Layer l's new rows are ``spans[l]`` = (first_target_phys, count):
+    one pt_extrude_step launch when ``step`` is enabled and the shape allows,
+    else the per-row plans ``rows_of(l)`` through :func:`extrude_rounds`."""
+    As = A if isinstance(A, (list, tuple)) else [A] * len(spans)
+    Bs = B if isinstance(B, (list, tuple)) else [B] * len(spans)
+    if step is not None and step.enabled and _extrude_step(step, As, Bs, buf, cap, width, m, spans,
+                                                           betas, bscales, far_points):
+        return
+    plans = [rows_of(l) if spans[l][1] else [] for l in range(len(spans))]
+    extrude_rounds(As, Bs, buf, cap, width, m, plans, betas, bscales, far_points=far_points)
+
+
 def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, far_points=0, step=None):
     """Run the planned rows of several layers on the device.
 
-    With ``step`` (a StepState) the whole step is one cooperative launch
+    With ``step`` (a StepState) the whole step is one launch
     (pt_extrude_step) when its shape allows; otherwise, or without it, the
     innovation GEMM and one launch per round.
 
@@ -290,9 +323,15 @@ def extrude_rounds(A, B, buf, cap, width, m, plans, betas, bscales, far_points=0
     st = _hip.stream_ptr()
     As = A if isinstance(A, (list, tuple)) else [A] * L
     Bs = B if isinstance(B, (list, tuple)) else [B] * L
-    if step is not None and step.enabled and _extrude_step(step, As, Bs, buf, cap, width, m, plans,
-                                                           betas, bscales, far_points):
-        return
+    if step is not None and step.enabled:
+        spans = []
+        for p in plans:
+            if any(p[i][0] != (p[0][0] + i) % cap for i in range(len(p))):
+                break  # rows of a step are consecutive modulo cap
+            spans.append((p[0][0] if p else 0, len(p)))
+        else:
+            if _extrude_step(step, As, Bs, buf, cap, width, m, spans, betas, bscales, far_points):
+                return
     # innovation GEMM per distinct B (one per outer-scale group); the kernel
     # reads every innovation row in place through a host array of pointers
     bb_of = {}
@@ -365,12 +404,14 @@ def readout(buf, cap, width, rings, wind_cs, dir_geom, n, interp, scale=1.0, out
     D = len(dir_geom)
     if out is None:
         out = t.empty((D, n, n), dtype=t.float32, device=buf.device)
-    start = (ctypes_int * L)(*[r.start for r in rings])
-    fill = (ctypes_ll * L)(*[r.fill for r in rings])
-    cs = (ctypes_double * (2 * L))(*[v for c in wind_cs for v in c])
-    geom = (ctypes_double * (3 * D * L))(*np.asarray(dir_geom, dtype=np.float64).ravel())
-    rc = _hip.lib().pt_readout(_hip.ptr(buf), L, cap, width, int(buf.shape[-1]), start, fill,
-                               cs, geom, D, n,
+    start = np.array([r.start for r in rings], dtype=np.int32)
+    fill = np.array([r.fill for r in rings], dtype=np.int64)
+    cs = np.ascontiguousarray(wind_cs, dtype=np.float64)
+    geom = np.ascontiguousarray(dir_geom, dtype=np.float64)
+    if cs.size != 2 * L or geom.size != 3 * D * L:
+        raise ValueError("readout: wind_cs must be (layers, 2) and dir_geom (directions, layers, 3)")
+    rc = _hip.lib().pt_readout(_hip.ptr(buf), L, cap, width, int(buf.shape[-1]), start.ctypes.data,
+                               fill.ctypes.data, cs.ctypes.data, geom.ctypes.data, D, n,
                                _INTERP_CODE[interp], float(scale), int(bool(accumulate)),
                                _hip.ptr(out), _hip.stream_ptr())
     _hip.check(rc, "pt_readout")

@@ -53,17 +53,43 @@ class Ring:
         the bookkeeping (compacting when full, as the reference does before
         each extrusion).
         """
-        rows = []
-        while self.base + self.fill - 1 < top_virtual:
+        first, count = self.extend(top_virtual, first_readable_row)
+        return self.rows(first, count)
+
+    def extend(self, top_virtual: int, first_readable_row: float) -> Tuple[int, int]:
+        """The bookkeeping of :meth:`plan` without the per-row lists.
+
+        Returns ``(first_target_phys, count)``: the new rows are physical
+        rows ``(first + i) % cap``, i < count.  They are consecutive modulo
+        ``cap`` because a compaction moves ``start`` and ``fill`` by the same
+        amount, which leaves the physical row of the next new row in place.
+        """
+        need = int(top_virtual) - (self.base + self.fill - 1)
+        first = (self.start + self.fill) % self.cap
+        count = 0
+        while need > 0:
             if self.fill == self.cap:
                 self.compact(first_readable_row)
-            f = self.fill
-            z = tuple(self.phys(f - self.m + k) for k in range(self.m))
+            if self.far and self.fill - 1 - self.far[-1] < 0:
+                raise RuntimeError("far stencil row behind the ring's oldest row")
+            k = min(need, self.cap - self.fill)
+            self.fill += k
+            count += k
+            need -= k
+        return first, count
+
+    def rows(self, first: int, count: int) -> List[Tuple[int, Tuple[int, ...]]]:
+        """Per-row ``(target_phys, (z_phys...)[, (far_phys...)])`` of ``count``
+        consecutive new rows from physical row ``first`` (what :meth:`plan`
+        returns): a row's stencil rows are the m physical rows before it and
+        its far rows lie ``d + 1`` rows behind it, modulo ``cap``."""
+        cap, m = self.cap, self.m
+        out = []
+        for i in range(count):
+            t = (first + i) % cap
+            z = tuple((t - m + k) % cap for k in range(m))
             if self.far:
-                if f - 1 - self.far[-1] < 0:
-                    raise RuntimeError("far stencil row behind the ring's oldest row")
-                rows.append((self.phys(f), z, tuple(self.phys(f - 1 - d) for d in self.far)))
+                out.append((t, z, tuple((t - 1 - d) % cap for d in self.far)))
             else:
-                rows.append((self.phys(f), z))
-            self.fill = f + 1
-        return rows
+                out.append((t, z))
+        return out

@@ -1,5 +1,5 @@
 """The one-launch extrusion step (pt_extrude_step): the innovation terms and
-every recurrence round of a set_time call in one cooperative launch, with a
+every recurrence round of a set_time call in one launch, with a
 bounded grid barrier between rounds.  It replaces the innovation GEMM plus
 one launch per round (src/pyturb/extrude.py:424-430 for every layer) and
 must give the same rows bit for bit:

@@ -184,3 +184,50 @@ def test_tune_state_is_host_side_and_validated():
         tune.set_variant("gen_rows", 256, 9)
     with pytest.raises(ValueError):
         tune.variants("fft", 256)
+
+
+def test_ring_rows_hold_their_stencil_and_far_rows():
+    """The index-shift ring (pyturb_amd.ring) against a brute-force model of
+    the reference's copying ring (src/pyturb/ring.py:9-29, extrude.py:424-449):
+    every planned row's stencil and far rows must hold the virtual rows the
+    reference would read, through compactions and wraps; extend() and rows()
+    must give plan()'s rows; and base/fill must follow the reference rule."""
+    import math
+    import random
+
+    from pyturb_amd.ring import Ring
+
+    rng = random.Random(5)
+    for trial in range(200):
+        cap = rng.randint(12, 60)
+        m = rng.choice([1, 2, 3])
+        far = tuple(sorted(rng.sample(range(m, m + 8), rng.randint(0, 3))))
+        base = rng.randint(-20, 5)
+        fill = rng.randint(max(m, far[-1] + 2 if far else m), min(cap, 30))
+        ring = Ring(cap, m, base, fill, far)
+        twin = Ring(cap, m, base, fill, far)
+        mem = [None] * cap  # virtual row held by each physical row
+        for i in range(fill):
+            mem[ring.phys(i)] = base + i
+        ref_base, ref_fill = base, fill  # the reference's copying ring
+        top, first = base + fill - 1, float(base)
+        for _ in range(40):
+            top += rng.randint(0, 7)
+            first += rng.uniform(0, 5)
+            rows = ring.plan(top, first)
+            f0, k = twin.extend(top, first)
+            assert twin.rows(f0, k) == rows
+            for row in rows:
+                if ref_fill == cap:  # ring.py:23-29, before the extrusion
+                    stencil = ref_base + ref_fill - m
+                    if far:
+                        stencil = min(stencil, ref_base + ref_fill - 1 - far[-1])
+                    keep = max(1, min(math.floor(first) - 1, stencil) - ref_base)
+                    ref_base, ref_fill = ref_base + keep, ref_fill - keep
+                v = ref_base + ref_fill  # virtual row being extruded
+                assert [mem[z] for z in row[1]] == [v - m + q for q in range(m)]
+                if far:
+                    assert [mem[z] for z in row[2]] == [v - 1 - d for d in far]
+                mem[row[0]] = v
+                ref_fill += 1
+            assert (ring.base, ring.fill) == (ref_base, ref_fill) == (twin.base, twin.fill)
