@@ -579,11 +579,6 @@ def main(argv=None):
     par = parity(atm, out)
 
     extra = {}
-    # The extrusion leg runs before anything is captured into a HIP graph:
-    # kernels launched for the first time after a capture of this library's
-    # kernels run about 25 % slower for the rest of the process on ROCm 7
-    # (measured: 68 -> 90 us and 82 -> 100 us for the step and readout
-    # kernels), while kernels first launched before it keep their speed.
     if not args.no_extrude:
         # the stated stencil (2 rows + 32 far points), then the reference's
         # full-row stencil alone
