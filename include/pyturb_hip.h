@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PT_ABI_VERSION 5
+#define PT_ABI_VERSION 6
 
 #define PT_OK 0
 #define PT_ERR_ARG 1         /* invalid argument (shape, count, NULL)      */
@@ -479,25 +479,28 @@ int pt_rows_blend(float* dst, long long dst_pitch, int dst_start, int dst_cap,
                   float a, float b, void* stream);
 
 /*
- * Autotuner state: the launch shape of the generate row pass
+ * Autotuner state: the launch shape and FFT plan of the generate row pass
  * (PT_TUNE_GEN_ROWS) and of the column pass shared by pt_generate* and the
- * frame engine's pass B (PT_TUNE_GEN_COLS), selected per power-of-two grid
- * size n in 32..2048.  No counterpart in the reference, whose transforms are
- * library ifft2 calls (fourier.py:259, atmosphere.py:918).  Variant 0 is
- * the built-in default; every variant gives bit-identical results.  The
- * selection is process-wide host state read when a call launches (not
- * synchronised with calls from other threads).
+ * frame engine's pass B (PT_TUNE_GEN_COLS), and the frame engine's transform
+ * order (PT_TUNE_FRAMES), selected per power-of-two grid size n in 32..2048.
+ * No counterpart in the reference, whose transforms are library ifft2 calls
+ * (fourier.py:259, atmosphere.py:918).  Variant 0 is the built-in default.
+ * Variants that differ only in launch shape give bit-identical results;
+ * the radix-4 plans and the two transform orders round differently (within
+ * 1e-6 relative RMS).  The selection is process-wide host state read when a
+ * call launches (not synchronised with calls from other threads).
  *
  * pt_tune_variants: number of variants of a knob (negative error code on a
  * bad knob).  pt_tune_info: effective threads per workgroup, rows or
- * columns per workgroup (tile) and XCD-contiguous order (0/1) of a variant
- * at size n; any output pointer may be NULL.  pt_tune_set / pt_tune_get:
+ * columns per workgroup (tile), XCD-contiguous order (0/1) and the plan's
+ * largest radix (8 or 4) of a variant at size n; any output pointer may be
+ * NULL.  pt_tune_set / pt_tune_get:
  * select / read the variant for size n (pt_tune_get returns it, or a
  * negative error code).
  */
 int pt_tune_variants(int knob);
 int pt_tune_info(int knob, int n, int variant, int* threads, int* tile,
-                 int* xcd_order);
+                 int* xcd_order, int* radix);
 int pt_tune_set(int knob, int n, int variant);
 int pt_tune_get(int knob, int n);
 

@@ -27,7 +27,7 @@ PT_ERR_UNSUPPORTED = 3
 PT_TUNE_GEN_ROWS = 0
 PT_TUNE_GEN_COLS = 1
 PT_TUNE_FRAMES = 2
-ABI_VERSION = 5  # PT_ABI_VERSION of include/pyturb_hip.h
+ABI_VERSION = 6  # PT_ABI_VERSION of include/pyturb_hip.h
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -76,7 +76,8 @@ SIGNATURES = {
                               _vp, ctypes.c_ulonglong, ctypes.c_ulonglong, _i, _i, _f, _vp, _vp,
                               _vp]),
     "pt_tune_variants": (_i, [_i]),
-    "pt_tune_info": (_i, [_i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "pt_tune_info": (_i, [_i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                          ctypes.POINTER(_i)]),
     "pt_tune_set": (_i, [_i, _i, _i]),
     "pt_tune_get": (_i, [_i, _i]),
 }

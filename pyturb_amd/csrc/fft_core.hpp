@@ -3,7 +3,9 @@
 // One workgroup transforms M independent complex sequences of length N
 // (N a power of two, 32 <= N <= 2048).  Every thread owns exactly eight
 // complex values per stage: one radix-8 butterfly, two radix-4 or four
-// radix-2 butterflies, so M * N == 8 * T for a T-thread workgroup.
+// radix-2 butterflies, so M * N == 8 * T for a T-thread workgroup.  The
+// plan's largest radix RX is 8 (the default: log2(N)/3 LDS round trips) or
+// 4 (log2(N)/2 round trips of simpler butterflies), a tuning choice.
 //
 // The first stage reads its inputs through a caller-supplied prologue
 // functor straight from HBM (so layer sums, PSD filtering and noise
@@ -109,19 +111,22 @@ __device__ __forceinline__ void dftR(float2* v) {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
-// Stage plan: radix-8 stages plus one radix-4 or radix-2 stage when log2 N is
-// not a multiple of 3.  The small stage is last by default; SF (small
-// first) puts it first, so that the last stage is radix 8 for every N and a
+// Stage plan: radix-RX stages plus one smaller stage when log2 N is not a
+// multiple of log2 RX (RX = 8: one radix-4 or radix-2 stage; RX = 4: one
+// radix-2 stage).  The small stage is last by default; SF (small first) puts
+// it first, so that with RX = 8 the last stage is radix 8 for every N and a
 // thread's eight outputs lie in one sequence (what a per-row epilogue needs).
-template <int N, bool SF = false>
+template <int N, bool SF = false, int RX = 8>
 struct Plan {
   static_assert((N & (N - 1)) == 0 && N >= 32 && N <= 2048, "power-of-two 32..2048");
+  static_assert(RX == 8 || RX == 4, "largest radix 8 or 4");
   static constexpr int LOG = ilog2(N);
-  static constexpr int NFULL = LOG / 3;
-  static constexpr int REM = LOG % 3;
+  static constexpr int LR = ilog2(RX);
+  static constexpr int NFULL = LOG / LR;
+  static constexpr int REM = LOG % LR;
   static constexpr int NST = NFULL + (REM ? 1 : 0);
   static constexpr int radix(int s) {
-    return SF ? ((REM && s == 0) ? (1 << REM) : 8) : (s < NFULL ? 8 : (1 << REM));
+    return SF ? ((REM && s == 0) ? (1 << REM) : RX) : (s < NFULL ? RX : (1 << REM));
   }
   static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * radix(s - 1); }
 };
@@ -227,9 +232,9 @@ struct Layout {
 
 // One Stockham stage ST.  For ST == 0 inputs come from `load(m, i)`; for the
 // last stage outputs go to `store(m, i, v)`; otherwise LDS.
-template <int N, int T, int M, bool COLS, int S, int ST, bool SF, class Load, class Store>
+template <int N, int T, int M, bool COLS, int S, int ST, bool SF, int RX, class Load, class Store>
 __device__ __forceinline__ void fft_stage(float2* lds, const float2* tw, int tid, Load& load, Store& store) {
-  using P = Plan<N, SF>;
+  using P = Plan<N, SF, RX>;
   using Lo = Layout<N, T, M, COLS>;
   constexpr int R = P::radix(ST);
   constexpr int NS = P::ns(ST);
@@ -281,13 +286,13 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* tw, int tid
   }
   if constexpr (!LAST) {
     __syncthreads();
-    fft_stage<N, T, M, COLS, S, ST + 1, SF>(lds, tw, tid, load, store);
+    fft_stage<N, T, M, COLS, S, ST + 1, SF, RX>(lds, tw, tid, load, store);
   }
 }
 
-template <int N, int T, int M, bool COLS, int S, bool SF = false, class Load, class Store>
+template <int N, int T, int M, bool COLS, int S, bool SF = false, int RX = 8, class Load, class Store>
 __device__ __forceinline__ void wg_fft(float2* lds, const float2* tw, int tid, Load& load, Store& store) {
-  fft_stage<N, T, M, COLS, S, 0, SF>(lds, tw, tid, load, store);
+  fft_stage<N, T, M, COLS, S, 0, SF, RX>(lds, tw, tid, load, store);
 }
 
 // ---------------------------------------------------------------------------

@@ -8,8 +8,10 @@
 //            columns, workgroups in XCD-contiguous (XCD) or flat order.
 // Every shape runs the same per-element arithmetic (the Stockham stages of
 // one row or column do not depend on how many rows or columns share a
-// workgroup), so all variants give bit-identical results.  The defaults
-// are the measured best on MI355X; tune_variants.hip instantiates the rest.
+// workgroup), so all shapes give bit-identical results.  The plan's largest
+// radix RX (8 or 4) is a template parameter too; the radix-4 plan rounds
+// differently (within 1e-6 relative RMS).  The defaults are the measured
+// best on MI355X; tune_variants.hip instantiates the rest.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,7 +47,7 @@ struct ColsCfg {
 //   spectrum = sum_g (re_g + i im_g) * amp_g,
 // where group g's planes start at nre + g * gstride (real) and
 // nre + g * gstride + im_off (imaginary), and amp_g = amp + g * N * N.
-template <int N, int TR = kRowsT>
+template <int N, int TR = kRowsT, int RX = 8>
 __global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* __restrict__ nre, ptrdiff_t im_off,
                                                                  size_t gstride, int G, const float* __restrict__ amp,
                                                                  float2* __restrict__ work) {
@@ -76,7 +78,7 @@ __global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* _
     return acc;
   };
   auto store = [&](int m, int i, float2 v) { store_nt(out + (size_t)m * N + i, v); };
-  wg_fft<N, T, M, false, 1>(lds, tw, tid, load, store);
+  wg_fft<N, T, M, false, 1, false, RX>(lds, tw, tid, load, store);
 }
 
 // Column pass with a two-output epilogue.  Transform b yields the sets
@@ -108,7 +110,7 @@ constexpr size_t gen_cols_smem() {
   return gen_cols_alias<N>() ? (tile > sub ? tile : sub) : tile + sub;
 }
 
-template <int N, bool ACC, int TC = kColsT, bool XCD = true>
+template <int N, bool ACC, int TC = kColsT, bool XCD = true, int RX = 8>
 __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
   constexpr int T = ColsCfg<N, TC>::T, M = ColsCfg<N, TC>::M;
   constexpr bool ALIAS = gen_cols_alias<N>();
@@ -143,13 +145,13 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
     v1[cnt] = v.y;
     ++cnt;
   };
-  wg_fft<N, T, M, true, 1>(lds, tw, tid, load, store);
+  wg_fft<N, T, M, true, 1, false, RX>(lds, tw, tid, load, store);
   if (P > 0) {
     if (ALIAS) __syncthreads();  // every lane has read its last-stage inputs from the tile
     sub.commit(pre, P, tid);
   }
   __syncthreads();
-  constexpr int RL = Plan<N>::radix(Plan<N>::NST - 1);
+  constexpr int RL = Plan<N, false, RX>::radix(Plan<N, false, RX>::NST - 1);
   const int m = tid % M, x0 = tid / M;
   const int y = c0 + m - a.off;
   if ((unsigned)y >= (unsigned)a.n_out) return;  // no barrier follows
@@ -191,9 +193,9 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
 
 // Writing column pass: the epilogue fits 64 VGPRs and the LDS stays under
 // 80 KiB for N <= 1024, so two 1024-thread workgroups share a CU.
-template <int N, int TC = kColsT, bool XCD = true>
+template <int N, int TC = kColsT, bool XCD = true, int RX = 8>
 __global__ __launch_bounds__((ColsCfg<N, TC>::T)) __attribute__((amdgpu_waves_per_eu(8))) void k_gen_cols_set(GenArgs a) {
-  gen_cols_body<N, false, TC, XCD>(a);
+  gen_cols_body<N, false, TC, XCD, RX>(a);
 }
 // Accumulating variant (adds into out); its read-modify-write needs a few
 // more registers, so no occupancy target (it would spill).  Default shape

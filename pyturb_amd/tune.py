@@ -3,11 +3,12 @@
 Three knobs, set per power-of-two grid size n (32..2048):
 
 * ``gen_rows``: threads per workgroup of the row pass of ``ops.generate*``
-  (128, 256 or 512; 8 T / n rows per workgroup);
+  (128, 256 or 512; 8 T / n rows per workgroup) and the FFT plan's largest
+  radix (8, or 4 at 256 and 512 threads);
 * ``gen_cols``: threads per workgroup (256, 512 or 1024; a tile of
-  8 T / n adjacent columns) and workgroup order (XCD-contiguous or flat)
-  of the column pass shared by ``ops.generate*`` and the row-first frame
-  engine's pass B;
+  8 T / n adjacent columns), workgroup order (XCD-contiguous or flat) and
+  largest radix (8, or 4 at 1024 threads) of the column pass shared by
+  ``ops.generate*`` and the row-first frame engine's pass B;
 * ``frames_order``: the transform order of the fused frame engine
   (``ops.spectral_frames``): column-first (variant 0: layer sum and column
   transform in 16-column tiles, then rows with the epilogue, whole rows
@@ -17,8 +18,9 @@ Three knobs, set per power-of-two grid size n (32..2048):
 ``autotune`` times every distinct shape of each knob on a synthetic
 workload of the given size with HIP events on the launch stream, checks
 the output against the default's (bit for bit for the launch shapes, which
-run the same per-element arithmetic; within 1e-6 relative RMS for the two
-transform orders, which round differently), keeps the fastest and, given a path,
+run the same per-element arithmetic; within 1e-6 relative RMS for the
+radix-4 plans and the two transform orders, which round differently), keeps
+the fastest and, given a path,
 stores the choice in a JSON cache keyed by the device (architecture and
 compute-unit count).  ``load_cache`` applies a stored choice; setting the
 environment variable ``PYTURB_AMD_TUNE_CACHE`` to such a file applies it
@@ -50,35 +52,38 @@ def _knob(name: str) -> int:
 
 def variants(knob: str, n: int) -> List[Dict[str, int]]:
     """Distinct launch shapes of ``knob`` at size ``n``: one dict per shape
-    (``variant``, ``threads``, ``tile``, ``xcd``), lowest variant first."""
+    (``variant``, ``threads``, ``tile``, ``xcd``, ``radix``), lowest variant
+    first."""
     lib = _hip.lib()
     k = _knob(knob)
     count = lib.pt_tune_variants(k)
     seen, out = set(), []
     for v in range(count):
-        t, m, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _hip.check(lib.pt_tune_info(k, int(n), v, ctypes.byref(t), ctypes.byref(m), ctypes.byref(x)),
-                   "pt_tune_info")
-        key = (t.value, m.value, x.value)
+        t, m, x, r = _info(k, n, v)
+        key = (t, m, x, r)
         if key in seen:
             continue
         seen.add(key)
-        out.append({"variant": v, "threads": t.value, "tile": m.value, "xcd": x.value})
+        out.append({"variant": v, "threads": t, "tile": m, "xcd": x, "radix": r})
     return out
+
+
+def _info(k: int, n: int, v: int):
+    t, m, x, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _hip.check(_hip.lib().pt_tune_info(k, int(n), int(v), ctypes.byref(t), ctypes.byref(m),
+                                       ctypes.byref(x), ctypes.byref(r)), "pt_tune_info")
+    return t.value, m.value, x.value, r.value
 
 
 def describe(knob: str, n: int, variant: Optional[int] = None) -> str:
     """Readable shape of a variant (the selected one by default)."""
     v = get(knob, n) if variant is None else int(variant)
-    lib = _hip.lib()
-    t, m, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    _hip.check(lib.pt_tune_info(_knob(knob), int(n), v, ctypes.byref(t), ctypes.byref(m),
-                                ctypes.byref(x)), "pt_tune_info")
+    t, m, x, r = _info(_knob(knob), n, v)
     if knob == "frames_order":
-        return ("column-first" if v == 0 else "row-first") + f", pass A {t.value} threads"
+        return ("column-first" if v == 0 else "row-first") + f", pass A {t} threads"
     unit = "rows" if knob == "gen_rows" else "cols"
-    order = "" if knob == "gen_rows" else (" xcd" if x.value else " flat")
-    return f"{t.value} threads x {m.value} {unit}{order}"
+    order = "" if knob == "gen_rows" else (" xcd" if x else " flat")
+    return f"{t} threads x {m} {unit}{order}, radix {r}"
 
 
 def get(knob: str, n: int) -> int:
@@ -202,7 +207,7 @@ def autotune(sizes: Iterable[int] = (256, 512, 1024), *, screens: Optional[int] 
                 warnings.warn(f"autotune: {knob} variant {var} at n={n} failed ({exc}); skipped",
                               RuntimeWarning, stacklevel=3)
                 continue
-            if exact:
+            if exact and var.get("radix", 8) == 8:
                 ok = bool(t.equal(out, ref))
             else:
                 ok = float(((out.double() - ref.double()).norm() / ref.double().norm()).item()) < 1e-6

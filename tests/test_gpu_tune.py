@@ -1,5 +1,6 @@
-"""Autotuned launch shapes (pyturb_amd.tune): every shape of the row and
-column passes reproduces the default shape's output bit for bit, for the
+"""Autotuned launch shapes (pyturb_amd.tune): every radix-8 shape of the row
+and column passes reproduces the default shape's output bit for bit, and
+every radix-4 plan agrees with it within 1e-6 relative RMS, for the
 Monte-Carlo generate path and for the frame engine's pass B; the frame
 engine's two transform orders agree with each other and with the FP64
 oracle at every size; the tuner selects, stores and re-applies its
@@ -43,8 +44,16 @@ def test_every_generate_shape_is_bit_identical(hip, tuned_reset, n, count):
             out.fill_(float("nan"))
             run()
             torch.cuda.synchronize()
-            assert torch.equal(out, ref), (knob, var)
+            if var["radix"] == 8:
+                assert torch.equal(out, ref), (knob, var)
+            else:  # another plan: other rounding
+                assert _rel_rms(out, ref) < 1e-6, (knob, var, _rel_rms(out, ref))
         tune.set_variant(knob, n, 0)
+
+
+def _rel_rms(a, b):
+    a, b = a.double(), b.double()
+    return float(((a - b).norm() / b.norm()).item())
 
 
 def test_every_column_shape_is_bit_identical_for_frames(hip, tuned_reset):
@@ -60,7 +69,10 @@ def test_every_column_shape_is_bit_identical_for_frames(hip, tuned_reset):
     for var in tune.variants("gen_cols", atm.n_screen):
         tune.set_variant("gen_cols", atm.n_screen, var["variant"])
         got = atm.opd_batch(times)
-        assert got.equal(ref), var
+        if var["radix"] == 8:
+            assert got.equal(ref), var
+        else:
+            assert _rel_rms(got, ref) < 1e-6, var
 
 
 @pytest.mark.parametrize("n,fov,frames", [(32, 0.0, 5), (64, 8.0, 7), (128, 0.0, 4),

@@ -60,7 +60,7 @@ def test_library_loads_and_exports_every_header_symbol():
     for name in syms:
         assert hasattr(so, name), name
         assert name in _hip.SIGNATURES, name
-    assert _hip.lib().pt_abi_version() == _hip.ABI_VERSION == 5
+    assert _hip.lib().pt_abi_version() == _hip.ABI_VERSION == 6
     assert "pt_work_elems" in syms
     # host-only sizing: power-of-two fused path vs general path
     # frames: pairs of frames share one transform, plus the folded stack, its
@@ -152,8 +152,8 @@ def test_tune_state_is_host_side_and_validated():
     from pyturb_amd import _hip, tune
 
     lib = _hip.lib()
-    assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_ROWS) == 3
-    assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_COLS) == 6
+    assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_ROWS) == 5
+    assert lib.pt_tune_variants(_hip.PT_TUNE_GEN_COLS) == 8
     assert lib.pt_tune_variants(_hip.PT_TUNE_FRAMES) == 2
     assert [(v["threads"], v["tile"]) for v in tune.variants("frames_order", 512)] == [(1024, 16),
                                                                                       (256, 4)]
@@ -163,17 +163,20 @@ def test_tune_state_is_host_side_and_validated():
     assert lib.pt_tune_set(7, 512, 0) == _hip.PT_ERR_ARG
     assert lib.pt_tune_set(0, 500, 0) == _hip.PT_ERR_ARG      # not a power of two
     assert lib.pt_tune_set(0, 4096, 0) == _hip.PT_ERR_ARG     # above 2048
-    assert lib.pt_tune_set(0, 512, 3) == _hip.PT_ERR_ARG      # rows has variants 0..2
+    assert lib.pt_tune_set(0, 512, 5) == _hip.PT_ERR_ARG      # rows has variants 0..4
     assert lib.pt_tune_get(1, 100) == -_hip.PT_ERR_ARG
-    # shapes: (threads, tile, xcd order)
-    shapes = {(v["threads"], v["tile"], v["xcd"]) for v in tune.variants("gen_cols", 512)}
-    assert shapes == {(1024, 16, 1), (512, 8, 1), (256, 4, 1), (1024, 16, 0), (512, 8, 0),
-                      (256, 4, 0)}
-    assert [(v["threads"], v["tile"]) for v in tune.variants("gen_cols", 2048)
-            if v["xcd"]] == [(1024, 4), (512, 2), (256, 1)]
-    assert [(v["threads"], v["tile"]) for v in tune.variants("gen_rows", 32)] == [(128, 32)]
-    assert [(v["threads"], v["tile"]) for v in tune.variants("gen_rows", 2048)] == [(256, 1),
-                                                                                    (512, 2)]
+    # shapes: (threads, tile, xcd order, largest radix)
+    shapes = [(v["threads"], v["tile"], v["xcd"], v["radix"]) for v in tune.variants("gen_cols", 512)]
+    assert shapes == [(1024, 16, 1, 8), (512, 8, 1, 8), (256, 4, 1, 8), (1024, 16, 0, 8),
+                      (512, 8, 0, 8), (256, 4, 0, 8), (1024, 16, 1, 4), (1024, 16, 0, 4)]
+    assert [(v["threads"], v["tile"], v["radix"]) for v in tune.variants("gen_cols", 2048)
+            if v["xcd"]] == [(1024, 4, 8), (512, 2, 8), (256, 1, 8), (1024, 4, 4)]
+    assert [(v["threads"], v["tile"], v["radix"]) for v in tune.variants("gen_rows", 32)] == [
+        (128, 32, 8), (128, 32, 4)]
+    assert [(v["threads"], v["tile"], v["radix"]) for v in tune.variants("gen_rows", 2048)] == [
+        (256, 1, 8), (512, 2, 8), (256, 1, 4), (512, 2, 4)]
+    assert tune.describe("gen_cols", 512, 6) == "1024 threads x 16 cols xcd, radix 4"
+    assert [v["radix"] for v in tune.variants("frames_order", 512)] == [8, 8]
     try:
         tune.set_variant("gen_cols", 256, 4)
         assert tune.get("gen_cols", 256) == 4 and tune.get("gen_cols", 512) == 0
