@@ -121,6 +121,34 @@ int pt_generate_groups(const float* noise, const float* amplitude,
                        int accumulate, float* out, void* work, void* stream);
 
 /*
+ * A Monte-Carlo ensemble of several grid sizes (Atmosphere.sample or
+ * PhaseScreen.generate once per size, src/pyturb/atmosphere.py:1191-1219,
+ * fourier.py:233-282) in three launches shared by every size class: a
+ * persistent row-pass kernel over all classes' (transform, row block) items,
+ * one subharmonic prep, and a persistent column-pass kernel over all
+ * classes' (transform, column tile) items.  Class k is pt_generate_groups
+ * with crop_offset = 0, n_out = n[k], accumulate = 0 and the default launch
+ * shapes, and its screens are bit-identical to that call's:
+ *   n[k]            : power of two, 128..2048 (else PT_ERR_UNSUPPORTED)
+ *   count[k]        : screens of class k
+ *   noise[k], amplitude[k], sh_noise[k], sh_amp[k], out[k]: as
+ *                     pt_generate_groups' arrays of class k (HOST arrays of
+ *                     device pointers); sh_noise / sh_amp may be NULL when
+ *                     levels == 0
+ *   screen_scale    : NULL, or a host array of per-class device pointers
+ *                     (each NULL or (count[k]) f32)
+ *   work[k]         : pt_work_elems(0, n[k], count[k], levels) c64 each
+ * 1 <= classes <= 8, 1 <= groups <= 64, levels <= 12.
+ */
+int pt_generate_ensemble(int classes, const int* n, const int* count,
+                         const float* const* noise,
+                         const float* const* amplitude,
+                         const float* const* sh_noise,
+                         const double* const* sh_amp, int groups, int levels,
+                         const float* const* screen_scale, float scale,
+                         float* const* out, void* const* work, void* stream);
+
+/*
  * Spectral frozen-flow frames (src/pyturb/atmosphere.py:882-938 on-axis,
  * :940-998 for directions; src/pyturb/flow.py:96-129 for one layer).
  *

@@ -47,19 +47,17 @@ struct ColsCfg {
 //   spectrum = sum_g (re_g + i im_g) * amp_g,
 // where group g's planes start at nre + g * gstride (real) and
 // nre + g * gstride + im_off (imaginary), and amp_g = amp + g * N * N.
-template <int N, int TR = kRowsT, int RX = 8>
-__global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* __restrict__ nre, ptrdiff_t im_off,
-                                                                 size_t gstride, int G, const float* __restrict__ amp,
-                                                                 float2* __restrict__ work) {
-  constexpr int T = RowsCfg<N, TR>::T, M = RowsCfg<N, TR>::M;
-  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
-  __shared__ float2 tw[N];
-  const int tid = threadIdx.x;
-  init_twiddles<N, 1>(tw, tid, T);
-  __syncthreads();
+//
+// gen_rows_item transforms row block rb (M rows) of transform b; the
+// twiddle table tw holds size N and a barrier separates it from the
+// previous use of lds.
+template <int N, int T, int M, int RX>
+__device__ __forceinline__ void gen_rows_item(const float* __restrict__ nre, ptrdiff_t im_off, size_t gstride, int G,
+                                              const float* __restrict__ amp, float2* __restrict__ work, int rb,
+                                              int bi, float2* lds, const float2* tw, int tid) {
   const size_t plane = (size_t)N * N;
-  const size_t b = blockIdx.y;
-  const size_t r0 = (size_t)blockIdx.x * M * N;
+  const size_t b = bi;
+  const size_t r0 = (size_t)rb * M * N;
   const float* re = nre + b * plane + r0;
   const float* am = amp + r0;
   float2* out = work + b * plane + r0;
@@ -79,6 +77,44 @@ __global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* _
   };
   auto store = [&](int m, int i, float2 v) { store_nt(out + (size_t)m * N + i, v); };
   wg_fft<N, T, M, false, 1, false, RX>(lds, tw, tid, load, store);
+}
+
+template <int N, int TR = kRowsT, int RX = 8>
+__global__ __launch_bounds__((RowsCfg<N, TR>::T)) void k_gen_rows(const float* __restrict__ nre, ptrdiff_t im_off,
+                                                                 size_t gstride, int G, const float* __restrict__ amp,
+                                                                 float2* __restrict__ work) {
+  constexpr int T = RowsCfg<N, TR>::T, M = RowsCfg<N, TR>::M;
+  __shared__ float2 lds[Layout<N, T, M, false>::SIZE];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  init_twiddles<N, 1>(tw, tid, T);
+  __syncthreads();
+  gen_rows_item<N, T, M, RX>(nre, im_off, gstride, G, amp, work, blockIdx.x, blockIdx.y, lds, tw, tid);
+}
+
+// Subharmonic tables of screen s (one 256-thread block per screen); with G
+// groups the coefficient blocks of the groups are summed (the low-frequency
+// field is linear in them).  Group g's noise is sh_noise + g * P * 2 * count
+// * 9, its amplitudes sh_amp + g * P * 9.  With `ubasis` the finished basis
+// table t.U is read, else screen 0 also writes it.
+__device__ __forceinline__ void gen_prep_set(const float* __restrict__ sh_noise, const double* __restrict__ sh_amp,
+                                             int count, int P, int n, int G, SubTables t, int ubasis, int s) {
+  __shared__ double2 C[PMAX * 9];
+  __shared__ double s_mean;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < P * 9; q += 256) {
+    const int p = q / 9, k = q % 9;
+    double2 c = make_double2(0.0, 0.0);
+    for (int g = 0; g < G; ++g) {
+      const float* sn = sh_noise + (size_t)g * P * 2 * count * 9;
+      const double am = sh_amp[(size_t)g * P * 9 + q];
+      c.x += (double)sn[((size_t)(p * 2 + 0) * count + s) * 9 + k] * am;
+      c.y += (double)sn[((size_t)(p * 2 + 1) * count + s) * 9 + k] * am;
+    }
+    C[q] = c;
+  }
+  __syncthreads();
+  sub_fill_set(C, P, n, s, t, !ubasis && s == 0, &s_mean, tid, 256, 0, 1, ubasis ? t.U : nullptr);
 }
 
 // Column pass with a two-output epilogue.  Transform b yields the sets
@@ -110,26 +146,24 @@ constexpr size_t gen_cols_smem() {
   return gen_cols_alias<N>() ? (tile > sub ? tile : sub) : tile + sub;
 }
 
-template <int N, bool ACC, int TC = kColsT, bool XCD = true, int RX = 8>
-__device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
-  constexpr int T = ColsCfg<N, TC>::T, M = ColsCfg<N, TC>::M;
-  constexpr bool ALIAS = gen_cols_alias<N>();
+// gen_cols_item transforms the column tile at c0 (M columns) of transform
+// b and writes its two screens; smem holds the tile and the subharmonic
+// staging block (beside the tile, or in it once the transform has read it
+// when ALIAS), tw the twiddles of size N, and a barrier separates both from
+// their previous use.  A tile outside the crop returns at once
+// (workgroup-uniform).
+template <int N, bool ACC, int T, int M, int RX, bool ALIAS = gen_cols_alias<N>()>
+__device__ __forceinline__ void gen_cols_item(const GenArgs& a, unsigned char* smem, const float2* tw, int c0, int b,
+                                              int tid) {
   constexpr size_t TILE_B = (size_t)Layout<N, T, M, true>::SIZE * sizeof(float2);
   using Sub = GenColSub<N, T, M>;
-  __shared__ __align__(16) unsigned char smem[gen_cols_smem<N, T, M>()];
-  __shared__ float2 tw[N];
   float2* lds = reinterpret_cast<float2*>(smem);
   Sub& sub = *reinterpret_cast<Sub*>(smem + (ALIAS ? 0 : TILE_B));
-  const int tid = threadIdx.x;
-  const int b = blockIdx.y;
-  const int c0 = (XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * M;
   if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;  // tile entirely outside the crop
   const int P = a.levels;
   const int s0 = b * a.s_mul, s1 = s0 + a.s_add;
   const bool has1 = s1 < a.count;
   const int sets[2] = {s0, has1 ? s1 : -1};
-  init_twiddles<N, 1>(tw, tid, T);
-  __syncthreads();
   // issued after the barrier: the loads complete under the tile load
   typename Sub::Regs pre;
   if (P > 0) pre = Sub::prefetch(a.tab, sets, c0, P, tid);
@@ -189,6 +223,19 @@ __device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
     o0[(size_t)x * a.n_out] = prev0[c] + v0[c] * f0;
     if (has1) o1[(size_t)x * a.n_out] = prev1[c] + v1[c] * f1;
   }
+}
+
+template <int N, bool ACC, int TC = kColsT, bool XCD = true, int RX = 8>
+__device__ __forceinline__ void gen_cols_body(const GenArgs& a) {
+  constexpr int T = ColsCfg<N, TC>::T, M = ColsCfg<N, TC>::M;
+  __shared__ __align__(16) unsigned char smem[gen_cols_smem<N, T, M>()];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  const int c0 = (XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * M;
+  if (c0 + M <= a.off || c0 >= a.off + a.n_out) return;  // tile entirely outside the crop
+  init_twiddles<N, 1>(tw, tid, T);
+  __syncthreads();
+  gen_cols_item<N, ACC, T, M, RX>(a, smem, tw, c0, blockIdx.y, tid);
 }
 
 // Writing column pass: the epilogue fits 64 VGPRs and the LDS stays under

@@ -42,22 +42,7 @@ __global__ __launch_bounds__(256) void k_sub_basis(float2* __restrict__ U, int P
 __global__ __launch_bounds__(256) void k_gen_prep(const float* __restrict__ sh_noise,
                                                   const double* __restrict__ sh_amp, int count, int P, int n,
                                                   int G, SubTables t, int ubasis) {
-  __shared__ double2 C[PMAX * 9];
-  __shared__ double s_mean;
-  const int s = blockIdx.x, tid = threadIdx.x;
-  for (int q = tid; q < P * 9; q += 256) {
-    const int p = q / 9, k = q % 9;
-    double2 c = make_double2(0.0, 0.0);
-    for (int g = 0; g < G; ++g) {
-      const float* sn = sh_noise + (size_t)g * P * 2 * count * 9;
-      const double am = sh_amp[(size_t)g * P * 9 + q];
-      c.x += (double)sn[((size_t)(p * 2 + 0) * count + s) * 9 + k] * am;
-      c.y += (double)sn[((size_t)(p * 2 + 1) * count + s) * 9 + k] * am;
-    }
-    C[q] = c;
-  }
-  __syncthreads();
-  sub_fill_set(C, P, n, s, t, !ubasis && s == 0, &s_mean, tid, 256, 0, 1, ubasis ? t.U : nullptr);
+  gen_prep_set(sh_noise, sh_amp, count, P, n, G, t, ubasis, blockIdx.x);
 }
 
 // One workgroup's share (`part` of `parts`) of frame f's subharmonic tables;

@@ -152,6 +152,70 @@ def generate_groups(noise, sh_noise, amplitude, sh_amp, count: int, *, crop_offs
     return out
 
 
+class EnsembleClass:
+    """One size class of :func:`generate_ensemble`: the arrays of a
+    :func:`generate_groups` call without crop (``screen_scale``, ``out`` and
+    ``work`` optional)."""
+
+    __slots__ = ("noise", "sh_noise", "amplitude", "sh_amp", "count", "screen_scale", "out", "work")
+
+    def __init__(self, noise, sh_noise, amplitude, sh_amp, count: int, screen_scale=None,
+                 out=None, work=None) -> None:
+        self.noise, self.sh_noise, self.amplitude, self.sh_amp = noise, sh_noise, amplitude, sh_amp
+        self.count, self.screen_scale, self.out, self.work = int(count), screen_scale, out, work
+
+
+def generate_ensemble(classes, *, scale: float = 1.0):
+    """Screens of several grid sizes in three launches shared by every class
+    (pt_generate_ensemble): class k gives the screens of
+    ``generate_groups(noise, sh_noise, amplitude, sh_amp, count,
+    screen_scale=..., scale=scale)`` bit for bit.  ``classes`` is a list of
+    :class:`EnsembleClass` with the same group count and subharmonic depth;
+    n must be a power of two in 128..2048.  Returns the list of outputs."""
+    import ctypes
+
+    t = _t()
+    K = len(classes)
+    if not 1 <= K <= 8:
+        raise ValueError("generate_ensemble: 1..8 size classes")
+    G = int(classes[0].amplitude.shape[0])
+    P = 0 if classes[0].sh_amp is None else int(classes[0].sh_amp.shape[1])
+    ns, outs, works = [], [], []
+    for c in classes:
+        n, count = int(c.amplitude.shape[-1]), c.count
+        if int(c.amplitude.shape[0]) != G or (0 if c.sh_amp is None else int(c.sh_amp.shape[1])) != P:
+            raise ValueError("generate_ensemble: every class needs the same groups and levels")
+        _require(c.noise, t.float32, (G, 2, (count + 1) // 2, n, n), "noise")
+        _require(c.amplitude, t.float32, (G, n, n), "amplitude")
+        if P:
+            _require(c.sh_amp, t.float64, (G, P, 3, 3), "sh_amp")
+            _require(c.sh_noise, t.float32, (G, P, 2, count, 3, 3), "sh_noise")
+        if c.screen_scale is not None:
+            _require(c.screen_scale, t.float32, (count,), "screen_scale")
+        out = c.out if c.out is not None else t.empty((count, n, n), dtype=t.float32,
+                                                      device=c.noise.device)
+        _require(out, t.float32, (count, n, n), "out")
+        need = work_elems(0, n, count, P)
+        work = c.work if c.work is not None else t.empty((need,), dtype=t.complex64,
+                                                         device=c.noise.device)
+        if work.numel() < need or work.dtype != t.complex64:
+            raise ValueError(f"work: need {need} complex64 elements")
+        ns.append(n)
+        outs.append(out)
+        works.append(work)
+    vp = ctypes.c_void_p * K
+    ci = ctypes.c_int * K
+    rc = _hip.lib().pt_generate_ensemble(
+        K, ci(*ns), ci(*[c.count for c in classes]), vp(*[_hip.ptr(c.noise) for c in classes]),
+        vp(*[_hip.ptr(c.amplitude) for c in classes]),
+        vp(*[_hip.ptr(c.sh_noise) for c in classes]) if P else None,
+        vp(*[_hip.ptr(c.sh_amp) for c in classes]) if P else None, G, P,
+        vp(*[_hip.ptr(c.screen_scale) for c in classes]), float(scale),
+        vp(*[_hip.ptr(o) for o in outs]), vp(*[_hip.ptr(w) for w in works]), _hip.stream_ptr())
+    _hip.check(rc, "pt_generate_ensemble")
+    return outs
+
+
 def spectral_fold(spectra, frames: int, work):
     """Fold the layer stack into ``work`` for calls of ``frames`` frames
     (pt_spectral_fold); then ``spectral_frames(..., folded=True)`` skips it."""
