@@ -123,12 +123,15 @@ int pt_generate_groups(const float* noise, const float* amplitude,
 /*
  * A Monte-Carlo ensemble of several grid sizes (Atmosphere.sample or
  * PhaseScreen.generate once per size, src/pyturb/atmosphere.py:1191-1219,
- * fourier.py:233-282) in three launches shared by every size class: a
- * persistent row-pass kernel over all classes' (transform, row block) items,
- * one subharmonic prep, and a persistent column-pass kernel over all
- * classes' (transform, column tile) items.  Class k is pt_generate_groups
- * with crop_offset = 0, n_out = n[k], accumulate = 0 and the default launch
- * shapes, and its screens are bit-identical to that call's:
+ * fourier.py:233-282) with launches shared by every size class: a
+ * persistent row-pass kernel over all classes' (transform, row block) items
+ * and one subharmonic prep; then, with persistent_cols, a persistent
+ * column-pass kernel over all classes' (transform, column tile) items, else
+ * each class's column launch of pt_generate_groups.  Class k is
+ * pt_generate_groups with crop_offset = 0, n_out = n[k], accumulate = 0,
+ * and its screens are bit-identical to that call's under the default
+ * launch shapes (the persistent passes run the default shapes and radix-8
+ * plans; a tuned radix-4 plan of pt_generate_groups rounds differently):
  *   n[k]            : power of two, 128..2048 (else PT_ERR_UNSUPPORTED)
  *   count[k]        : screens of class k
  *   noise[k], amplitude[k], sh_noise[k], sh_amp[k], out[k]: as
@@ -146,7 +149,8 @@ int pt_generate_ensemble(int classes, const int* n, const int* count,
                          const float* const* sh_noise,
                          const double* const* sh_amp, int groups, int levels,
                          const float* const* screen_scale, float scale,
-                         float* const* out, void* const* work, void* stream);
+                         float* const* out, void* const* work,
+                         int persistent_cols, void* stream);
 
 /*
  * Spectral frozen-flow frames (src/pyturb/atmosphere.py:882-938 on-axis,

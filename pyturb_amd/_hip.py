@@ -44,7 +44,8 @@ SIGNATURES = {
     "pt_generate": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp, _vp]),
     "pt_generate_groups": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp,
                                 _vp]),
-    "pt_generate_ensemble": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp]),
+    "pt_generate_ensemble": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _i,
+                                  _vp]),
     "pt_spectral_frames": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp]),
     "pt_spectral_frames_noalloc": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp]),
     "pt_spectral_rows": (_i, [_vp, _i, _i, _d, _vp, _i, _vp, _vp]),

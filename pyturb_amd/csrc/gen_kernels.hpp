@@ -262,5 +262,7 @@ int tune_choice(int knob, int n);
 int launch_gen_rows_variant(int n, int variant, const float* nre, ptrdiff_t im_off, size_t gstride, int G,
                             const float* amp, float2* work, int n_fft, hipStream_t st);
 int launch_gen_cols_variant(int n, int variant, const GenArgs& a, hipStream_t st);
+// The column pass of pt_generate* for a power-of-two n (the selected shape).
+int launch_gen_cols(int n, const GenArgs& a, hipStream_t st);
 
 }  // namespace pt

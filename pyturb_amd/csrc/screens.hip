@@ -1087,6 +1087,13 @@ static bool fused_ok(int n, int levels) {
   return false;
 }
 
+int launch_gen_cols(int n, const GenArgs& a, hipStream_t st) {
+#define CALL(N) return gen_cols_pow2<N>(a, st);
+  PT_POW2_SWITCH(n, CALL)
+#undef CALL
+  return fail(PT_ERR_UNSUPPORTED, "gen_cols: n=%d", n);
+}
+
 }  // namespace pt
 
 using namespace pt;

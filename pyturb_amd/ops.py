@@ -165,13 +165,16 @@ class EnsembleClass:
         self.count, self.screen_scale, self.out, self.work = int(count), screen_scale, out, work
 
 
-def generate_ensemble(classes, *, scale: float = 1.0):
+def generate_ensemble(classes, *, scale: float = 1.0, persistent_cols: bool = False):
     """Screens of several grid sizes in three launches shared by every class
     (pt_generate_ensemble): class k gives the screens of
     ``generate_groups(noise, sh_noise, amplitude, sh_amp, count,
     screen_scale=..., scale=scale)`` bit for bit.  ``classes`` is a list of
     :class:`EnsembleClass` with the same group count and subharmonic depth;
-    n must be a power of two in 128..2048.  Returns the list of outputs."""
+    n must be a power of two in 128..2048.  The row pass and the subharmonic
+    prep are persistent launches over every class; the column pass runs per
+    class, or, with ``persistent_cols``, as one persistent launch too.
+    Returns the list of outputs."""
     import ctypes
 
     t = _t()
@@ -211,7 +214,8 @@ def generate_ensemble(classes, *, scale: float = 1.0):
         vp(*[_hip.ptr(c.sh_noise) for c in classes]) if P else None,
         vp(*[_hip.ptr(c.sh_amp) for c in classes]) if P else None, G, P,
         vp(*[_hip.ptr(c.screen_scale) for c in classes]), float(scale),
-        vp(*[_hip.ptr(o) for o in outs]), vp(*[_hip.ptr(w) for w in works]), _hip.stream_ptr())
+        vp(*[_hip.ptr(o) for o in outs]), vp(*[_hip.ptr(w) for w in works]), int(bool(persistent_cols)),
+        _hip.stream_ptr())
     _hip.check(rc, "pt_generate_ensemble")
     return outs
 
