@@ -210,6 +210,42 @@ def test_uniform_step_frames_match_oracle(hip, order, n, frames):
             assert O.rel_rms(got["uniform"][t], got["off"][t]) < 1e-6, t
 
 
+@pytest.mark.parametrize("n", [64, 512])
+def test_few_frame_calls_equal_batched_calls(hip, n):
+    """Frames do not depend on how many other frames share their call: a call
+    of 1 to 4 frame pairs gives, bit for bit, the same frames as a larger
+    call (column-first order, 9 layers), with equally spaced times (uniform
+    step) and with jittered ones, 2 to 8 frames."""
+    import torch
+    from pyturb_amd import ops, tune
+
+    print(f"few-frame calls n={n}", flush=True)
+    rng = np.random.default_rng(n + 5)
+    L, dx = 9, 8.0 / n
+    spectra = (rng.standard_normal((L, n, n)) + 1j * rng.standard_normal((L, n, n))).astype(np.complex64)
+    wind = rng.uniform(-30.0, 30.0, (L, 2))
+    times = 0.4 + 1e-3 * np.arange(12)
+    dev = torch.device("cuda")
+    sp = torch.from_numpy(spectra).to(dev)
+    try:
+        tune.set_variant("frames_order", n, 0)
+        for jitter in (0.0, 0.3):
+            t = times + jitter * 1e-3 * rng.uniform(-1.0, 1.0, times.size)
+            disp = torch.from_numpy(t[:, None, None] * wind[None]).to(dev)
+            big = ops.spectral_frames(sp, disp, None, dx)  # 6 pairs: the batched pass A
+            # 1..4 pairs: the few-pair pass A.  The two calls must take the same
+            # path: a single pair always passes the uniform-step check, so it is
+            # compared on equally spaced times only; and a frame alone in its
+            # pair shares its transform with no partner (unless the uniform step
+            # supplies one), so jittered calls compare their complete pairs
+            for F in ((7, 8) if jitter else (2, 7, 8)):
+                few = ops.spectral_frames(sp, disp[:F].contiguous(), None, dx)
+                m = F if not jitter else F - F % 2
+                assert torch.equal(few[:m], big[:m]), (jitter, F)
+    finally:
+        tune.reset()
+
+
 @pytest.mark.parametrize("order", [0, 1])
 def test_spectral_frames_many_frames_match_oracle(hip, order):
     """20 equally spaced frames with 8 subharmonic levels: a call of 16 or
