@@ -10,13 +10,21 @@ launches); inputs (layer spectra, displacement table) are resident in HBM.
 Also reported on the same line:
   * the Monte-Carlo ensemble: 2048 x 256^2 + 1536 x 512^2 + 512 x 1024^2
     screens, two outer-scale groups (L0 = 25 m / 50 m, half the Cn2 each),
-    per-screen r0 ~ U(0.08, 0.25) m (seed 4), white noise resident in HBM;
+    per-screen r0 ~ U(0.08, 0.25) m (seed 4), white noise resident in HBM,
+    through per-class pt_generate_groups calls and through the one-call
+    pt_generate_ensemble (the faster is mc_screens_per_s, both are listed),
+    with per-pass times and HBM fractions in roofline.passes;
+  * the 1024^2 9-layer extrusion with 7 readout directions (one extrusion
+    step and one tiled readout per set_time), with and without the
+    32-point far stencil;
+  * boiled frames (tau_boil = 0.2 s) and LGS-cone frames (90 km beacon);
   * the plain GPU form of both (torch ops, one kernel chain per layer, FP32
     everywhere -- the reference's array-module algorithm on ROCm);
   * roofline of the dominant kernel (pass A of the frame engine, in the
     transform order the autotuner picked) from HIP events on the launch
     stream;
-  * the CPU baseline: the FP64 NumPy oracle timed on this host (1 thread);
+  * the CPU baseline: the FP64 NumPy oracle timed on this host (1 thread,
+    and every available core in cpu_baseline.rows);
   * a closed-loop call pattern, one frame per call, eager against a
     HIP-graph replay (pyturb_amd.graph.CapturedFrames), and the ensemble
     replayed from a HIP graph;
