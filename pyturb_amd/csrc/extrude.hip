@@ -235,6 +235,7 @@ constexpr int XS_NG = (XS_CWMAX + XS_GRP - 1) / XS_GRP;  // rounds: groups per v
 constexpr int XS_NBUF = 4;               // rounds: prefetch ring (loads run XS_NBUF - 1 groups ahead)
 constexpr int XS_NT = 2;                 // innovation vector tiles per pass (32 vectors)
 constexpr int XS_GRIDMAX = 144;          // workgroups (row pitch <= 2304)
+static_assert(XS_GRIDMAX <= 3 * 64, "a barrier poll reads three flags per lane");
 constexpr int XS_FLAGS = 16;             // sync word of workgroup g's flag: XS_FLAGS + g
 static_assert((XS_FLAGS + XS_GRIDMAX) * 8 <= PT_EXTRUDE_SYNC_BYTES, "sync words");
 
@@ -276,7 +277,7 @@ __device__ __forceinline__ int nth_layer(unsigned live, int e) {
 // thread; the last wave polls, so polling starts while wave 0 still writes)
 // returns false when the barrier timed out here or anywhere in the grid.
 __device__ __forceinline__ void step_arrive(const StepArgs& a, unsigned long long target) {
-  __threadfence();  // this wave's row stores reach memory before the flag
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's row stores reach memory before the flag
   if (threadIdx.x == 0)
     __hip_atomic_store(a.sync + XS_FLAGS + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -285,14 +286,21 @@ __device__ bool step_wait(const StepArgs& a, unsigned long long target, int poll
   __shared__ int ok;
   const int lane = threadIdx.x & 63;
   if ((int)(threadIdx.x >> 6) == poll_wave) {
+    // lane q polls the flags of workgroups q, q + 64 and q + 128 (clamped to
+    // the grid: a repeated flag changes nothing); the four loads of a poll
+    // are independent, so a poll costs one memory round trip
     const unsigned long long* flags = a.sync + XS_FLAGS;
+    const int g1 = (int)gridDim.x - 1;
+    const unsigned long long *f0 = flags + min(lane, g1), *f1 = flags + min(lane + 64, g1),
+                             *f2 = flags + min(lane + 128, g1);
     int good = 1;
     unsigned it = 0;
     for (;;) {
       const unsigned long long abort = __hip_atomic_load(a.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int done = 1;
-      for (int q = lane; q < (int)gridDim.x; q += 64)
-        done &= __hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+      const unsigned long long v0 = __hip_atomic_load(f0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long v1 = __hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long v2 = __hip_atomic_load(f2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const bool done = v0 >= target && v1 >= target && v2 >= target;
       if (__any(abort != 0) || ++it > a.spin_limit) {  // the abort word is checked first
         good = 0;
         break;
@@ -300,7 +308,7 @@ __device__ bool step_wait(const StepArgs& a, unsigned long long target, int poll
       if (__all(done)) break;
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();  // the other workgroups' rows are read fresh after this
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other workgroups' rows are read fresh after this
     if (lane == 0) {
       if (!good) {
         __hip_atomic_store(a.sync + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -480,8 +488,12 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
   __syncthreads();
   const unsigned long long base = base_s;
   // --- the rounds: round r extrudes row r of every layer that has one.
-  // Loads of the older stencil rows and the far points of round r + 1 do not
-  // depend on round r, so they are issued before the barrier that ends it.
+  // Virtual wave vw sums its chunks of stencil segment 0, ..., M - 1 and
+  // then its far points (k_mfma_gemv's order) in acc[h].  Segments
+  // 0 .. M - 2 and the far rows lie two or more rows behind round r's
+  // target row, so every workgroup wrote them before the barrier that ends
+  // round r - 1: their loads and MFMAs run before that barrier is waited
+  // for, and only segment M - 1 (the rows of round r - 1) follows it.
   unsigned live = 0;
   int zo[M], fo[FAR_RMAX];
   float zf[2][4];
@@ -519,16 +531,17 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
       }
     }
   };
-  // stencil rows in groups g = (h, s, gi): chunks XS_GRP gi + [0, XS_GRP), loaded XS_NBUF - 1 groups ahead
-  constexpr int G = 2 * M * XS_NG;
-  // groups 0 .. EARLY - 1 read segment 0, an older stencil row when M >= 2
-  constexpr int EARLY = M >= 2 ? (XS_NBUF - 1 < XS_NG ? XS_NBUF - 1 : XS_NG) : 0;
+  // chunks in groups of XS_GRP, loaded XS_NBUF - 1 groups ahead of their
+  // MFMAs: GO groups (h, s, gi) of the old segments s < M - 1, then GN
+  // groups (h, gi) of segment M - 1
+  constexpr int GO = 2 * (M - 1) * XS_NG, GN = 2 * XS_NG;
   float4 zb[XS_NBUF][XS_GRP];
-  auto load_group = [&](int g, float4 (&z)[XS_GRP]) {
-    const int h = g / (M * XS_NG), s = (g / XS_NG) % M, vw = wave + XS_WAVES * h;
+  mfma_v4f acc[2];
+  auto load_chunks = [&](int h, int s, int gi, float4 (&z)[XS_GRP]) {
+    const int vw = wave + XS_WAVES * h;
 #pragma unroll
     for (int u = 0; u < XS_GRP; ++u) {
-      const int i = XS_GRP * (g % XS_NG) + u, c = vw + MF_WAVES * i;
+      const int i = XS_GRP * gi + u, c = vw + MF_WAVES * i;
       // unconditional loads (the compiler drains loads in flight where
       // branches around loads join): a chunk that does not exist is never
       // multiplied, and a column without a layer (le < 0, zo = 0) only
@@ -537,44 +550,67 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
       z[u] = *(const float4*)(a.buf + (unsigned)(zo[s] + k));
     }
   };
-  setup(0);
+  auto mfma_chunks = [&](int h, int s, int gi, const float4 (&z)[XS_GRP]) {
+    const int vw = wave + XS_WAVES * h;
 #pragma unroll
-  for (int g = 0; g < EARLY; ++g) load_group(g, zb[g]);
-  for (int r = 0; r < a.rounds; ++r) {
-#pragma unroll
-    for (int g = EARLY; g < XS_NBUF - 1; ++g)
-      if (g < G) load_group(g, zb[g]);
-    mfma_v4f acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      if (g + XS_NBUF - 1 < G) load_group(g + XS_NBUF - 1, zb[(g + XS_NBUF - 1) % XS_NBUF]);
-      const int h = g / (M * XS_NG), s = (g / XS_NG) % M, vw = wave + XS_WAVES * h;
-#pragma unroll
-      for (int u = 0; u < XS_GRP; ++u) {
-        const int i = XS_GRP * (g % XS_NG) + u, c = vw + MF_WAVES * i;
-        if (i < XS_CWMAX && i < a.cw && c < cps) {  // wave-uniform
-          const float4 x = i < XS_CWMAX - 1 ? areg[h][s][i] : alast[h][s][wave][lane], z = zb[g % XS_NBUF][u];
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z.x, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z.y, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z.z, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z.w, acc, 0, 0, 0);
-        }
+    for (int u = 0; u < XS_GRP; ++u) {
+      const int i = XS_GRP * gi + u, c = vw + MF_WAVES * i;
+      if (i < XS_CWMAX && i < a.cw && c < cps) {  // wave-uniform
+        const float4 x = i < XS_CWMAX - 1 ? areg[h][s][i] : alast[h][s][wave][lane];
+        acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, z[u].x, acc[h], 0, 0, 0);
+        acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, z[u].y, acc[h], 0, 0, 0);
+        acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, z[u].z, acc[h], 0, 0, 0);
+        acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, z[u].w, acc[h], 0, 0, 0);
       }
-      if (g % (M * XS_NG) == M * XS_NG - 1) {  // the last group of virtual wave vw: far points, then its partial tile
+    }
+  };
+  // round r's old segments (before the barrier that ends round r - 1)
+  auto old_part = [&]() {
+    acc[0] = acc[1] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+    if constexpr (GO > 0) {
+      constexpr int PS = (M > 1 ? M - 1 : 1) * XS_NG;  // groups per virtual wave
+#pragma unroll
+      for (int g = 0; g < XS_NBUF - 1; ++g)
+        if (g < GO) load_chunks(g / PS, (g / XS_NG) % (M - 1), g % XS_NG, zb[g]);
+#pragma unroll
+      for (int g = 0; g < GO; ++g) {
+        const int gp = g + XS_NBUF - 1;
+        if (gp < GO) load_chunks(gp / PS, (gp / XS_NG) % (M - 1), gp % XS_NG, zb[gp % XS_NBUF]);
+        mfma_chunks(g / PS, (g / XS_NG) % (M - 1), g % XS_NG, zb[g % XS_NBUF]);
+      }
+    }
+  };
+  // segment M - 1 and the far points, then each virtual wave's partial tile into LDS
+  auto new_part = [&]() {
+#pragma unroll
+    for (int g = 0; g < XS_NBUF - 1; ++g)
+      if (g < GN) load_chunks(g / XS_NG, M - 1, g % XS_NG, zb[g]);
+#pragma unroll
+    for (int g = 0; g < GN; ++g) {
+      const int gp = g + XS_NBUF - 1;
+      if (gp < GN) load_chunks(gp / XS_NG, M - 1, gp % XS_NG, zb[gp % XS_NBUF]);
+      const int h = g / XS_NG, vw = wave + XS_WAVES * h;
+      mfma_chunks(h, M - 1, g % XS_NG, zb[g % XS_NBUF]);
+      if (g % XS_NG == XS_NG - 1) {  // the last group of virtual wave vw: far points, then its partial tile
         if (vw < fch) {
           // points past nfar meet A's zero padding: zero them so they stay finite
           const bool fin = 16 * vw + 4 * kq < a.nfar;
           const float4 x = afar[vw][lane];
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, fin ? zf[h][0] : 0.f, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, fin ? zf[h][1] : 0.f, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, fin ? zf[h][2] : 0.f, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, fin ? zf[h][3] : 0.f, acc, 0, 0, 0);
+          acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, fin ? zf[h][0] : 0.f, acc[h], 0, 0, 0);
+          acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, fin ? zf[h][1] : 0.f, acc[h], 0, 0, 0);
+          acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, fin ? zf[h][2] : 0.f, acc[h], 0, 0, 0);
+          acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, fin ? zf[h][3] : 0.f, acc[h], 0, 0, 0);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) red[0][vw][4 * kq + q][ri] = acc[q];
-        acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+        for (int q = 0; q < 4; ++q) red[0][vw][4 * kq + q][ri] = acc[h][q];
       }
     }
+  };
+  setup(0);
+  for (int r = 0; r < a.rounds; ++r) {
+    old_part();
+    if (r > 0 && !step_wait(a, base + r, XS_WAVES - 1)) return;
+    new_part();
     __syncthreads();
     if (threadIdx.x < 256) {
       const int rr = threadIdx.x >> 4, e = threadIdx.x & 15;
@@ -601,15 +637,7 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
       }
       if (more) step_arrive(a, base + r + 1);
     }
-    if (more) {
-      // round r + 1's rows that round r does not write: issued now, used
-      // after the barrier (wave 0 issues them after its flag store, whose
-      // fence would wait for them)
-      setup(r + 1);
-#pragma unroll
-      for (int g = 0; g < EARLY; ++g) load_group(g, zb[g]);
-      if (!step_wait(a, base + r + 1, XS_WAVES - 1)) return;
-    }
+    if (more) setup(r + 1);  // round r + 1's rows, far-row loads issued before its barrier
   }
   // the barrier count for the next call (every workgroup read it before barrier 0)
   if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -1456,6 +1484,11 @@ extern "C" int pt_extrude_step(const float* A, const float* B, int width, int ld
                 "pt_extrude_step: more than %d layers, more than 2 stencil rows or a row too wide for the register slice",
                 XS_LMAX);
   for (int j = 0; j < FAR_RMAX; ++j) a.far_d[j] = far_points && j < far_points / FAR_COLS ? far_rows_back[j] : 0;
+  // round r reads its far rows before the barrier that ends round r - 1, so
+  // they must lie behind the newest stencil row (written in round r - 1)
+  for (int j = 0; j < far_points / FAR_COLS; ++j)
+    if (far_rows_back[j] < 1)
+      return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: far row %d is not behind the newest stencil row", j);
   int nvec = 0, rounds = 0;
   for (int l = 0; l < layers; ++l) {
     if (rows[l] < 0 || first_row[l] < 0 || first_row[l] >= cap) return fail(PT_ERR_ARG, "pt_extrude_step: bad rows");
