@@ -337,7 +337,7 @@ int pt_extrude_rows_far(const float* A, int width, int ld, int stencil_rows,
  * nonzero, every workgroup leaves the kernel, *status (host-visible, e.g.
  * pinned) is set to 1 and the rows of the step are incomplete: zero sync
  * again before the next call.  PT_ERR_UNSUPPORTED (nothing launched) for
- * more than 16 layers, more than 256 new rows, stencil_rows > 2, a row
+ * more than 16 layers, more than 256 new rows or 32 of one layer, stencil_rows > 2, a row
  * pitch ld above 2304 or not a multiple of 16, a far_rows_back entry below
  * 1 (a round reads its far rows before the previous round's barrier), or a
  * GPU that cannot hold the grid; the caller then uses the

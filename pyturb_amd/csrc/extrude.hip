@@ -227,6 +227,7 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 // too (the word is checked before the flags), so every workgroup ends.
 constexpr int XS_LMAX = 16;              // layers per launch
 constexpr int XS_VMAX = 256;             // innovation rows per step
+constexpr int XS_RMAX = 32;              // rounds per step (rows of one layer)
 constexpr int XS_CWMAX = 9;              // A chunks per segment and virtual wave in registers (row pitch <= 2304)
 constexpr int XI_GRP = 3;                // innovation: chunks per prefetch group
 constexpr int XI_NG = (XS_CWMAX + XI_GRP - 1) / XI_GRP;  // innovation: groups per virtual wave
@@ -443,6 +444,12 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
   __shared__ unsigned long long base_s;
   __shared__ int lay_cnt[XS_LMAX], lay_t0[XS_LMAX], lay_boff[XS_LMAX];
   __shared__ float lay_sc[XS_LMAX];
+  // per round r and column e (the e-th layer with cnt > r, or none): the
+  // float offsets in buf of its stencil rows (RT_Z + s), far rows
+  // (RT_F + j) and target row (RT_T), and the layer (RT_L, -1 for none)
+  constexpr int RT_Z = 0, RT_F = 2, RT_T = RT_F + FAR_RMAX, RT_L = RT_T + 1;
+  static_assert(M <= RT_F && RT_L < 8, "round table entry");
+  __shared__ int rtab[XS_RMAX][16][8];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row0 = blockIdx.x * 16;
   const int ri = lane & 15, kq = lane >> 4;
@@ -485,6 +492,29 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
       }
   }
   if (wave < fch) afar[wave][lane] = *(const float4*)(arow + (size_t)M * a.ld + 16 * wave + 4 * kq);
+  // the round table (its integer work runs while the A loads are in flight;
+  // ring rows as 32-bit float offsets: the rings hold fewer than 2^31 floats)
+  for (int q = threadIdx.x; q < a.rounds * 16; q += 64 * XS_WAVES) {
+    const int r = q >> 4, e = q & 15;
+    unsigned live = 0;
+#pragma unroll
+    for (int l = 0; l < XS_LMAX; ++l) live |= (lay_cnt[l] > r ? 1u : 0u) << l;
+    const int le = nth_layer(live, e);
+    int* t = rtab[r][e];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = 0;
+    t[RT_L] = le;
+    if (le >= 0) {
+      const int cap = (int)a.cap, slab = le * cap;
+      const int target = ring_row32(lay_t0[le] + r, cap);
+#pragma unroll
+      for (int s = 0; s < M; ++s) t[RT_Z + s] = (slab + ring_row32(target - M + s, cap)) * a.ld;
+#pragma unroll
+      for (int j = 0; j < FAR_RMAX; ++j)
+        if (8 * j < a.nfar) t[RT_F + j] = (slab + ring_row32(target - 1 - a.far_d[j], cap)) * a.ld;
+      t[RT_T] = (slab + target) * a.ld;
+    }
+  }
   __syncthreads();
   const unsigned long long base = base_s;
   // --- the rounds: round r extrudes row r of every layer that has one.
@@ -494,29 +524,16 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
   // target row, so every workgroup wrote them before the barrier that ends
   // round r - 1: their loads and MFMAs run before that barrier is waited
   // for, and only segment M - 1 (the rows of round r - 1) follows it.
-  unsigned live = 0;
   int zo[M], fo[FAR_RMAX];
   float zf[2][4];
-  // entries of round r: column e is the e-th layer with cnt > r; ring rows as
-  // 32-bit float offsets from buf (the rings hold fewer than 2^31 floats)
+  // entries of round r (column ri of the round table): a column without a
+  // layer reads row offset 0 and feeds only its own unused output column
   auto setup = [&](int r) {
-    live = 0;
+    const int* t = rtab[r][ri];
 #pragma unroll
-    for (int q = 0; q < XS_LMAX; ++q) live |= (lay_cnt[q] > r ? 1u : 0u) << q;
-    const int le = nth_layer(live, ri);
+    for (int s = 0; s < M; ++s) zo[s] = t[RT_Z + s];
 #pragma unroll
-    for (int s = 0; s < M; ++s) zo[s] = 0;
-#pragma unroll
-    for (int j = 0; j < FAR_RMAX; ++j) fo[j] = 0;
-    if (le >= 0) {
-      const int cap = (int)a.cap, slab = le * cap;  // 32-bit: the rings hold fewer than 2^31 floats
-      const int target = ring_row32(lay_t0[le] + r, cap);
-#pragma unroll
-      for (int s = 0; s < M; ++s) zo[s] = (slab + ring_row32(target - M + s, cap)) * a.ld;
-#pragma unroll
-      for (int j = 0; j < FAR_RMAX; ++j)
-        if (8 * j < a.nfar) fo[j] = (slab + ring_row32(target - 1 - a.far_d[j], cap)) * a.ld;
-    }
+    for (int j = 0; j < FAR_RMAX; ++j) fo[j] = t[RT_F + j];
     // far points of both virtual waves (k_mfma_gemv's far segment)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -614,7 +631,7 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
     __syncthreads();
     if (threadIdx.x < 256) {
       const int rr = threadIdx.x >> 4, e = threadIdx.x & 15;
-      const int l = nth_layer(live, e);
+      const int l = rtab[r][e][RT_L];
       if (l >= 0) {
         float sum = 0.f;
 #pragma unroll
@@ -626,9 +643,8 @@ __global__ __launch_bounds__(64 * XS_WAVES) void k_extrude_step(StepArgs a) {
     const bool more = r + 1 < a.rounds;
     if (wave == 0) {  // one wave stores the rows, so one write-back publishes them
       const int e = lane & 15;
-      const int l = nth_layer(live, e);
-      if (l >= 0) {
-        float* dst = a.buf + (unsigned)((l * (int)a.cap + ring_row32(lay_t0[l] + r, (int)a.cap)) * a.ld + row0);
+      if (rtab[r][e][RT_L] >= 0) {
+        float* dst = a.buf + (unsigned)(rtab[r][e][RT_T] + row0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int rr = (lane >> 4) + 4 * q;
@@ -1506,6 +1522,8 @@ extern "C" int pt_extrude_step(const float* A, const float* B, int width, int ld
     rounds = rows[l] > rounds ? rows[l] : rounds;
   }
   if (nvec > XS_VMAX) return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: more than %d new rows per step", XS_VMAX);
+  if (rounds > XS_RMAX)
+    return fail(PT_ERR_UNSUPPORTED, "pt_extrude_step: more than %d new rows of one layer per step", XS_RMAX);
   if (rounds == 0) return PT_OK;
   a.nvec = nvec;
   a.rounds = rounds;
