@@ -307,6 +307,48 @@ __host__ __device__ constexpr int pair_tab_stride() { return N / 8 + 9; }
 // each pair at phase u(2p) + d, so the Nyquist sums use that phase too, and
 // the pair stays Hermitian in both parts.  The last pair of an odd frame
 // count includes that second frame as well (it is not written).
+// Per-layer phasor tables of the Nyquist sums for LB layers at a time,
+// T8[a] and T1[b] per frame and varying axis (see pair_tables_body), and one
+// term of a sum; shared by k_frame_pair_tables and pass A' with TAB, so the
+// two build bit-identical Nyquist terms.
+template <int N>
+struct NyqTabs {
+  static constexpr int LB = N <= 512 ? FOLD_LMAX : 4;
+  float2 t8s[LB][2][2][N / 8], t1s[LB][2][2][8];  // [layer][frame][varying axis]
+};
+template <int N>
+__device__ __forceinline__ void nyq_build(NyqTabs<N>& tb, const uint32_t (&u)[2][FOLD_LMAX][2], int l0, int nl,
+                                          bool has_b, int tid, int nthreads) {
+  constexpr int NB = N / 8;
+  for (int e = tid; e < nl * 4 * (NB + 8); e += nthreads) {
+    const int lb = e / (4 * (NB + 8)), r = e % (4 * (NB + 8));
+    const int fa = r / (NB + 8), j = r % (NB + 8);
+    const int f = fa >> 1, ax = fa & 1, l = l0 + lb;
+    float2 v = make_float2(0.f, 0.f);
+    if (f == 0 || has_b) {
+      const uint32_t uv = u[f][l][ax], uo = u[f][l][ax ^ 1];
+      v = j < NB ? phasor_fixed((uint32_t)(8 * j - N / 2 + 1) * uv + (uint32_t)(N / 2) * uo)
+                 : phasor_fixed((uint32_t)(j - NB) * uv);
+    }
+    if (j < NB)
+      tb.t8s[lb][f][ax][j] = v;
+    else
+      tb.t1s[lb][f][ax][j - NB] = v;
+  }
+}
+// acc += the term of layer block entry lb for Nyquist entry q (q < 2N, not
+// the Nyquist column's own N/2 entry), side value sq
+template <int N>
+__device__ __forceinline__ void nyq_add(float2& acc, const NyqTabs<N>& tb, int lb, int q, float2 sq) {
+  const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
+  const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
+  const float2 wa = cmul(tb.t8s[lb][0][ax][id >> 3], tb.t1s[lb][0][ax][id & 7]);
+  const float2 wb = cmul(tb.t8s[lb][1][ax][id >> 3], tb.t1s[lb][1][ax][id & 7]);
+  const float2 t = cmul(sq, make_float2(wa.x - wb.y, wa.y + wb.x));
+  acc.x += t.x;
+  acc.y += t.y;
+}
+
 template <int N, bool TR = false>
 __device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp, const float2* __restrict__ side,
                                                  int L, int F, double inv_period, const uint32_t* __restrict__ step,
@@ -343,8 +385,8 @@ __device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp
   // Tables of LB layers are built per pair of barriers (at most 37 KiB):
   // every layer at once up to n = 512, blocks of 4 above.
   constexpr int QMAX = (2 * N + 255) / 256;
-  constexpr int LB = N <= 512 ? FOLD_LMAX : 4;
-  __shared__ float2 t8s[LB][2][2][NB], t1s[LB][2][2][8];  // [layer][frame][varying axis]
+  constexpr int LB = NyqTabs<N>::LB;
+  __shared__ NyqTabs<N> nt;
   float2 acc[QMAX];
 #pragma unroll
   for (int i = 0; i < QMAX; ++i) acc[i] = make_float2(0.f, 0.f);
@@ -362,21 +404,7 @@ __device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp
           sv[lb][i] = side[(size_t)(l0 + min(lb, nl - 1)) * 2 * N + min(first + stride * i, 2 * N - 1)];
     }
     __syncthreads();  // the previous layers' tables are no longer read
-    for (int e = threadIdx.x; e < nl * 4 * (NB + 8); e += 256) {
-      const int lb = e / (4 * (NB + 8)), r = e % (4 * (NB + 8));
-      const int fa = r / (NB + 8), j = r % (NB + 8);
-      const int f = fa >> 1, ax = fa & 1, l = l0 + lb;
-      float2 v = make_float2(0.f, 0.f);
-      if (f == 0 || has_b) {
-        const uint32_t uv = u[f][l][ax], uo = u[f][l][ax ^ 1];
-        v = j < NB ? phasor_fixed((uint32_t)(8 * j - N / 2 + 1) * uv + (uint32_t)(N / 2) * uo)
-                   : phasor_fixed((uint32_t)(j - NB) * uv);
-      }
-      if (j < NB)
-        t8s[lb][f][ax][j] = v;
-      else
-        t1s[lb][f][ax][j - NB] = v;
-    }
+    nyq_build<N>(nt, u, l0, nl, has_b, threadIdx.x, 256);
     __syncthreads();
 #pragma unroll
     for (int lb = 0; lb < LB; ++lb) {
@@ -386,18 +414,12 @@ __device__ __forceinline__ void pair_tables_body(const double* __restrict__ disp
         for (int i = 0; i < QMAX; ++i) {
           const int q = first + stride * i;
           if (q < 2 * N && !(q >= N && q - N == N / 2)) {
-            const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
-            const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
-            const float2 wa = cmul(t8s[lb][0][ax][id >> 3], t1s[lb][0][ax][id & 7]);
-            const float2 wb = cmul(t8s[lb][1][ax][id >> 3], t1s[lb][1][ax][id & 7]);
             float2 sq;
             if constexpr (PRE)
               sq = sv[PRE ? lb : 0][PRE ? i : 0];
             else
               sq = sl[q];
-            const float2 t = cmul(sq, make_float2(wa.x - wb.y, wa.y + wb.x));
-            acc[i].x += t.x;
-            acc[i].y += t.y;
+            nyq_add<N>(acc[i], nt, lb, q, sq);
           }
         }
       }
@@ -603,6 +625,24 @@ struct PairColsCfg {
   static constexpr int W = N >= 64 ? 8 : 1;
 };
 
+// Pass A' for calls of a few frame pairs (FEW_PAIRS at most): 4-column
+// tiles (4x the workgroups of PairColsCfg at n <= 512), read from the fold's
+// PairColsCfg tiles, with the fold values of PF layers loaded before any is
+// summed, so a call is neither one long chain of per-layer load latencies
+// nor bound by the L2 bandwidth of a few CUs.  Per element the arithmetic is
+// PairColsCfg's (same layer order, same transform), so the intermediates are
+// bit-identical.  n >= 128 (one wave of rows per column at least).
+constexpr int FEW_PAIRS = 4;
+template <int N>
+struct FewColsCfg {
+  static constexpr int MT = 4;
+  static constexpr int T = MT * N / 8;
+  // layers per prefetch block, PF x 4 float4 per thread: nine (a 9-layer
+  // profile in one block) at one wave per SIMD, four at two or more; the
+  // first block loads before the phasor tables are staged
+  static constexpr int PF = T <= 256 ? 9 : 4;
+};
+
 // Folded stack for pass A' (same values as k_frame_fold): tile t = k1 / mt
 // of mt columns; with k0 = j + r * n/8, lane (j, m = k1 % mt) reads its
 // values r = 2h, 2h + 1 as float4 number ((h * (n/mt) + t) * n/8 + j) * mt + m
@@ -637,10 +677,9 @@ __global__ __launch_bounds__(256) void k_frame_fold_tiles(const float2* __restri
 // (t8, t1 per layer and frame over the row index, the per-column factors
 // and the column phases) aliased at its start; they are dead before the
 // first stage writes the tile.
-template <int N>
+template <int N, int MT = PairColsCfg<N>::MT>
 __host__ __device__ constexpr size_t pair_cols_lds(int L) {
-  constexpr int MT = PairColsCfg<N>::MT;
-  const size_t tile = (size_t)Layout<N, PairColsCfg<N>::T, MT, true>::SIZE * 8;
+  const size_t tile = (size_t)Layout<N, MT * N / 8, MT, true>::SIZE * 8;
   const size_t tabs = (size_t)2 * L * (N / 8 + 8 + MT) * 8 + (size_t)2 * L * 4;
   return tile > tabs ? tile : tabs;
 }
@@ -650,13 +689,34 @@ __host__ __device__ constexpr size_t pair_cols_lds(int L) {
 // = 8a + b); the packed sum in FMA form as in pass A; the Nyquist terms from
 // k_frame_pair_tables; then the column transform, written as MT-column
 // runs of the c64 intermediate.
-template <int N>
-__global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(PairColsCfg<N>::W))) void k_frame_pair_cols(const float2* __restrict__ fold,
-                                                                      const float2* __restrict__ corr, int L,
-                                                                      const float2* __restrict__ ptab, int F,
-                                                                      const uint32_t* __restrict__ step,
-                                                                      float2* __restrict__ work) {
-  constexpr int T = PairColsCfg<N>::T, MT = PairColsCfg<N>::MT, NB = N / 8;
+// What pass A' needs to build its own per-call tables (TAB): the
+// displacements, the fold's side values and marker, and for the subharmonic
+// tables of the same frames (P > 0) the coefficients and table set.
+struct PairTabArgs {
+  const double* disp;
+  const float2* side;
+  const uint32_t* guard;
+  double inv_period;
+  const double2* coeffs;
+  int P;
+  SubTables tab;
+};
+
+// MT: columns of this launch's tiles; the fold is laid out in tiles of
+// PairColsCfg<N>::MT columns (MF).  PF > 1: the fold values of PF layers are
+// loaded before the first of them is summed.  TAB: the workgroup builds the
+// pair tables of its tile itself (t8, t1, the column phases and its
+// Nyquist terms, with k_frame_pair_tables' arithmetic) and its share of the
+// pair's subharmonic tables (k_frame_prep's), while its fold loads are in
+// flight, instead of reading ptab and corr (calls of a reused fold, no
+// uniform step).
+template <int N, int MT, int PF, bool TAB = false>
+__device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, const float2* __restrict__ corr, int L,
+                                               const float2* __restrict__ ptab, int F,
+                                               const uint32_t* __restrict__ step, float2* __restrict__ work,
+                                               const PairTabArgs& ta = PairTabArgs{}) {
+  constexpr int T = MT * N / 8, NB = N / 8, MF = PairColsCfg<N>::MT;
+  static_assert(MF % MT == 0, "launch tiles inside fold tiles");
   static_assert(T == MT * NB, "one column position per thread in the first stage");
   extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ float2 tw[N];
@@ -670,26 +730,118 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
   const int tile = blockIdx.x, c0 = tile * MT;
   const bool two = 2 * pr + 1 < F;
   constexpr int S = pair_tab_stride<N>();
-  const float2* tp = ptab + (size_t)pr * 2 * L * S;
-  for (int q = tid; q < 2 * L * S; q += T) {
-    const int fl = q / S, e = q % S;
-    const float2 v = tp[q];
-    if (e < NB)
-      t8[fl * NB + e] = v;
-    else if (e < NB + 8)
-      t1[fl * 8 + e - NB] = v;
-    else
-      U[fl] = __float_as_uint(v.x);
-  }
-  init_twiddles<N, 1>(tw, tid, T);
-  __syncthreads();
-  for (int q = tid; q < 2 * L * MT; q += T) py[q] = phasor_fixed((uint32_t)signed_freq(c0 + q % MT, N) * U[q / MT]);
-  __syncthreads();
   // first-stage positions of this thread: column c0 + m, rows j + r * NB (r < 8)
   const int m = tid % MT, j = tid / MT;
   const int k1 = c0 + m;
   const size_t plane = (size_t)N * N;
-  const float4* fp = (const float4*)fold + ((size_t)tile * NB + j) * MT + m;  // + h * N * NB per chunk
+  // fold tile c0 / MF, column c0 % MF + m of it
+  const float4* fp = (const float4*)fold + ((size_t)(c0 / MF) * NB + j) * MF + c0 % MF + m;  // + h * N * NB per chunk
+  // PF > 1: the fold values of layers l0 .. l0 + PF - 1 (clamped to the
+  // last layer: unconditional loads, the clamped ones never summed)
+  constexpr int PB = PF > 1 ? PF : 1;
+  float4 gb[PB][4];
+  auto load_block = [&](int l0) {
+#pragma unroll
+    for (int lb = 0; lb < PB; ++lb)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) gb[lb][h] = fp[(size_t)min(l0 + lb, L - 1) * (plane / 2) + (size_t)h * N * NB];
+  };
+  // Nyquist terms of this tile: cl[m] for column c0 + m on the Nyquist row,
+  // cl[MT + k0] for the Nyquist column when the tile holds column N/2
+  __shared__ float2 cl[TAB ? MT + N : 1];
+  const bool nyq_col = c0 <= N / 2 && N / 2 < c0 + MT;
+  __shared__ uint32_t u[TAB ? 2 : 1][FOLD_LMAX][2];
+  __shared__ float2 srow[TAB ? MT : 1][FOLD_LMAX];  // side values of the Nyquist row entries
+  // TAB: the displacements and the row entries' side values load before the
+  // fold, so waiting for them does not wait for the fold's stream
+  double dv = 0.0;
+  float2 sv = make_float2(0.f, 0.f);
+  uint32_t gv = FOLD_PLAIN;  // the fold marker (a reused fold must be the plain one)
+  if constexpr (TAB) {
+    if (ta.guard) gv = *ta.guard;
+    if (tid < 4 * L) {
+      const int f = tid / (2 * L), t = 2 * pr + f;
+      dv = ta.disp[(size_t)min(t, F - 1) * 2 * L + tid % (2 * L)];
+    }
+    if (tid < MT * L) sv = ta.side[(size_t)(tid / MT) * 2 * N + c0 + tid % MT];
+  }
+  // PF > 1: the twiddles and (without TAB) this pair's tables load into
+  // registers ahead of the fold (the load counter drains in order, so a
+  // load issued after the fold's would wait for all of it)
+  constexpr int TWR = PF > 1 ? (N + T - 1) / T : 1, PTR = PF > 1 && !TAB ? (2 * FOLD_LMAX * S + T - 1) / T : 1;
+  float2 twv[TWR], ptv[PTR];
+  const float2* tp = TAB ? nullptr : ptab + (size_t)pr * 2 * L * S;
+  if constexpr (PF > 1) {
+#pragma unroll
+    for (int r = 0; r < TWR; ++r) {
+      const int t = min(tid + r * T, N - 1);
+      const TwEntry e = kTw.e[t * (2048 / N)];
+      twv[r] = make_float2(e.c, e.s);
+    }
+    if constexpr (!TAB) {
+#pragma unroll
+      for (int r = 0; r < PTR; ++r) ptv[r] = tp[min(tid + r * T, 2 * L * S - 1)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    load_block(0);
+  }
+  if constexpr (TAB) {
+    // fixed-point phases of the pair's frames (a frame past the end is absent)
+    if (tid < 4 * L) {
+      const int f = tid / (2 * L), e = tid % (2 * L), t = 2 * pr + f;
+      u[f][e >> 1][e & 1] = t < F ? cycles_fixed(dv * ta.inv_period) : 0u;
+    }
+    if (tid < MT * L) srow[tid % MT][tid / MT] = sv;
+    __syncthreads();
+    const bool poison = gv != FOLD_PLAIN;
+    const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+    for (int q = tid; q < 2 * L * S; q += T) {
+      const int fl = q / S, e = q % S;
+      const int fb = fl / L, l = fl % L;
+      const uint32_t ut = u[fb][l][0], uo = u[fb][l][1];  // column-first: the table axis is x
+      float2 v;
+      if (e < NB)
+        v = phasor_fixed((uint32_t)(8 * e - N / 2) * ut);
+      else if (e < NB + 8)
+        v = phasor_fixed((uint32_t)(e - NB) * ut);
+      else
+        v = make_float2(__uint_as_float(uo), 0.f);
+      if (poison && e < NB + 8) v = nan2;
+      if (e < NB)
+        t8[fl * NB + e] = v;
+      else if (e < NB + 8)
+        t1[fl * 8 + e - NB] = v;
+      else
+        U[fl] = __float_as_uint(v.x);
+    }
+  } else {
+    auto put = [&](int q, float2 v) {
+      const int fl = q / S, e = q % S;
+      if (e < NB)
+        t8[fl * NB + e] = v;
+      else if (e < NB + 8)
+        t1[fl * 8 + e - NB] = v;
+      else
+        U[fl] = __float_as_uint(v.x);
+    };
+    if constexpr (PF > 1) {
+#pragma unroll
+      for (int r = 0; r < PTR; ++r)
+        if (tid + r * T < 2 * L * S) put(tid + r * T, ptv[r]);
+    } else {
+      for (int q = tid; q < 2 * L * S; q += T) put(q, tp[q]);
+    }
+  }
+  if constexpr (PF > 1) {
+#pragma unroll
+    for (int r = 0; r < TWR; ++r)
+      if (tid + r * T < N) tw[tid + r * T] = twv[r];
+  } else {
+    init_twiddles<N, 1>(tw, tid, T);
+  }
+  __syncthreads();
+  for (int q = tid; q < 2 * L * MT; q += T) py[q] = phasor_fixed((uint32_t)signed_freq(c0 + q % MT, N) * U[q / MT]);
+  __syncthreads();
   float2 z[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) z[r] = make_float2(0.f, 0.f);
@@ -702,14 +854,21 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
   auto layer_sum = [&](auto uni_tag) {
     constexpr bool UNI = decltype(uni_tag)::value;
     const bool two2 = !UNI && two;
-    for (int l = 0; l < L; ++l) {
+    for (int l0 = 0; l0 < L; l0 += PB) {
+      if constexpr (PF > 1) {
+        if (l0 > 0) load_block(l0);
+      }
+#pragma unroll
+      for (int lb = 0; lb < PB; ++lb) {
+      const int l = l0 + lb;
+      if (PF > 1 && l >= L) break;  // workgroup-uniform
       const float2 pa = py[l * MT + m], pb = py[(L + l) * MT + m];
       const float2 ca0 = cmul(pa, t1[l * 8 + b0]);
       const float2 cb0 = two2 ? cmul(pb, t1[(L + l) * 8 + b0]) : make_float2(0.f, 0.f);
       float2 g[8];
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        const float4 v = fp[(size_t)l * (plane / 2) + (size_t)h * N * NB];
+        const float4 v = PF > 1 ? gb[lb][h] : fp[(size_t)l * (plane / 2) + (size_t)h * N * NB];
         g[2 * h] = make_float2(v.x, v.y);
         g[2 * h + 1] = make_float2(v.z, v.w);
       }
@@ -739,6 +898,7 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
         z[r].y = fmaf(g[r].x, qy, z[r].y);
         z[r].y = fmaf(g[r].y, qx, z[r].y);
       }
+      }
     }
   };
   if (step_uniform(step))
@@ -748,10 +908,59 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
   // Nyquist row / column terms of this pair: corr[k1] on the Nyquist row
   // (k0 = N/2, every k1), corr[N + k0] on the Nyquist column (k1 = N/2,
   // k0 != N/2)
-  const float2* cp = corr + (size_t)pr * 2 * N;
+  if constexpr (TAB) {
+    // Nyquist sums (k_frame_pair_tables' tables and terms), once the layer
+    // sum has freed its registers: row entry q < N has k0 = N/2, k1 = q;
+    // column entry N + k0 has k1 = N/2; each entry's side values are loaded
+    // before its sum, and the layers are summed in order, LB per pair of
+    // barriers
+    constexpr int LB = NyqTabs<N>::LB, QT = (MT + N + T - 1) / T;
+    __shared__ NyqTabs<N> nt;
+    const bool has_b = 2 * pr + 1 < F;
+    const bool poison = gv != FOLD_PLAIN;
+    const int nq = MT + (nyq_col ? N : 0);
+    float2 acc[QT];
+#pragma unroll
+    for (int i = 0; i < QT; ++i) acc[i] = make_float2(0.f, 0.f);
+    for (int l0 = 0; l0 < L; l0 += LB) {
+      const int nl = min(LB, L - l0);
+      float2 sq[QT][LB];  // row entries from srow, Nyquist column entries from the side table
+#pragma unroll
+      for (int i = 0; i < QT; ++i) {
+        const int idx = min(tid + i * T, nq - 1);
+        const int q = idx < MT ? c0 + idx : N + (idx - MT);
+#pragma unroll
+        for (int lb = 0; lb < LB; ++lb) {
+          const int l = l0 + min(lb, nl - 1);
+          sq[i][lb] = idx < MT ? srow[idx][l] : ta.side[(size_t)l * 2 * N + q];
+        }
+      }
+      __syncthreads();  // the previous layers' tables are no longer read
+      nyq_build<N>(nt, u, l0, nl, has_b, tid, T);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < QT; ++i) {
+        const int idx = tid + i * T;
+        const int q = idx < MT ? c0 + idx : N + (idx - MT);
+        if (idx < nq && !(q >= N && q - N == N / 2)) {
+#pragma unroll
+          for (int lb = 0; lb < LB; ++lb)
+            if (lb < nl) nyq_add<N>(acc[i], nt, lb, q, sq[i][lb]);
+        }
+      }
+    }
+    const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+#pragma unroll
+    for (int i = 0; i < QT; ++i) {
+      const int idx = tid + i * T;
+      if (idx < nq) cl[idx] = poison ? nan2 : acc[i];
+    }
+    __syncthreads();
+  }
+  const float2* cp = TAB ? nullptr : corr + (size_t)pr * 2 * N;
   constexpr int RN = (N / 2) / NB;  // element r holding row N/2 (for j == (N/2) % NB)
   if (j == (N / 2) % NB) {
-    const float2 c = cp[k1];
+    const float2 c = TAB ? cl[m] : cp[k1];
     z[RN].x += c.x;
     z[RN].y += c.y;
   }
@@ -760,7 +969,7 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
     for (int r = 0; r < 8; ++r) {
       const int k0 = j + r * NB;
       if (k0 == N / 2) continue;
-      const float2 c = cp[N + k0];
+      const float2 c = TAB ? cl[MT + k0] : cp[N + k0];
       z[r].x += c.x;
       z[r].y += c.y;
     }
@@ -771,6 +980,49 @@ __global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_
   float2* out = work + (size_t)pr * plane + c0;
   auto store = [&](int mm, int i, float2 v) { store_nt(out + (size_t)i * N + mm, v); };
   wg_fft<N, T, MT, true, 1>(lds, tw, tid, load, store);
+}
+
+template <int N>
+__global__ __launch_bounds__(PairColsCfg<N>::T) __attribute__((amdgpu_waves_per_eu(PairColsCfg<N>::W))) void k_frame_pair_cols(const float2* __restrict__ fold,
+                                                                      const float2* __restrict__ corr, int L,
+                                                                      const float2* __restrict__ ptab, int F,
+                                                                      const uint32_t* __restrict__ step,
+                                                                      float2* __restrict__ work) {
+  pair_cols_body<N, PairColsCfg<N>::MT, 1>(fold, corr, L, ptab, F, step, work);
+}
+
+// Calls of at most FEW_PAIRS pairs (n >= 128).
+template <int N>
+__global__ __launch_bounds__(FewColsCfg<N>::T) void k_frame_pair_cols_few(const float2* __restrict__ fold,
+                                                                        const float2* __restrict__ corr, int L,
+                                                                        const float2* __restrict__ ptab, int F,
+                                                                        const uint32_t* __restrict__ step,
+                                                                        float2* __restrict__ work) {
+  pair_cols_body<N, FewColsCfg<N>::MT, FewColsCfg<N>::PF>(fold, corr, L, ptab, F, step, work);
+}
+
+// The same for a reused fold with the call's tables built in the same
+// launch: blocks x < N / MT are pass A' tiles that build their own pair
+// tables (TAB); blocks beyond build the pair's subharmonic tables for pass
+// B' (tb per frame, k_frame_prep's arithmetic), on other CUs at the same
+// time.  One launch instead of two per call.
+template <int N>
+__global__ __launch_bounds__(FewColsCfg<N>::T) void k_frame_pair_cols_tab(const float2* __restrict__ fold, int L, int F,
+                                                                        float2* __restrict__ work, PairTabArgs ta,
+                                                                        int tb) {
+  constexpr int T = FewColsCfg<N>::T, NT = N / FewColsCfg<N>::MT;
+  if ((int)blockIdx.x >= NT) {  // workgroup-uniform
+    const int i = blockIdx.x - NT, f = 2 * blockIdx.y + i / tb;
+    if (f >= F || ta.P == 0) return;
+    extern __shared__ __align__(16) unsigned char dyn[];
+    __shared__ double s_mean;
+    double2* Q = (double2*)dyn;
+    double2* SB = Q + 2 * (size_t)ta.P * L;
+    frame_sh_blocks(Q, SB, ta.disp, ta.coeffs, f, L, ta.P, ta.inv_period, threadIdx.x, T, true);
+    sub_fill_set(SB, ta.P, N, f, ta.tab, f == 0, &s_mean, threadIdx.x, T, i % tb, tb, nullptr);
+    return;
+  }
+  pair_cols_body<N, FewColsCfg<N>::MT, FewColsCfg<N>::PF, true>(fold, nullptr, L, nullptr, F, nullptr, work, ta);
 }
 
 // Pass B' of the column-first engine: the row transform of pair b (a plan
@@ -1051,9 +1303,45 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
     PT_LAUNCH_CHECK();
   }
   if (!(stages & STAGE_PAIRS)) return PT_OK;
+  if constexpr (N >= 128 && N <= 1024) {
+    // a reused fold and a few pairs: the tables are built inside pass A'
+    // (n = 2048 would spill at 1024 threads)
+    if (stages == STAGE_PAIRS && (F + 1) / 2 <= FEW_PAIRS) {
+      constexpr int MTF = FewColsCfg<N>::MT;
+      PairTabArgs ta;
+      ta.disp = disp;
+      ta.side = side;
+      ta.guard = mark;
+      ta.inv_period = inv_period;
+      ta.coeffs = mp ? mp->coeffs : nullptr;
+      ta.P = mp ? mp->P : 0;
+      ta.tab = mp ? mp->tab : SubTables{};
+      const size_t prep = (size_t)(2 * ta.P * L + 9 * ta.P) * sizeof(double2);
+      const size_t dyn_t = pair_cols_lds<N, MTF>(L) > prep ? pair_cols_lds<N, MTF>(L) : prep;
+      rc = set_dyn_lds(k_frame_pair_cols_tab<N>, dyn_t);
+      if (rc) return rc;
+      const int tb = ta.P > 0 ? 16 : 0;  // subharmonic table blocks per frame
+      k_frame_pair_cols_tab<N><<<dim3(N / MTF + 2 * tb, (F + 1) / 2), FewColsCfg<N>::T, dyn_t, st>>>(fold, L, F, work,
+                                                                                                  ta, tb);
+      PT_LAUNCH_CHECK();
+      return PT_OK;
+    }
+  }
   rc = launch_pair_tables<N, true>(disp, side, L, F, inv_period, step, ptab, corr,
                                    stages == STAGE_PAIRS ? mark : nullptr, mp, st);
   if (rc) return rc;
+  if constexpr (N >= 128) {
+    if ((F + 1) / 2 <= FEW_PAIRS) {
+      constexpr int MTF = FewColsCfg<N>::MT;
+      const size_t dyn_f = pair_cols_lds<N, MTF>(L);
+      rc = set_dyn_lds(k_frame_pair_cols_few<N>, dyn_f);
+      if (rc) return rc;
+      k_frame_pair_cols_few<N><<<dim3(N / MTF, (F + 1) / 2), FewColsCfg<N>::T, dyn_f, st>>>(fold, corr, L, ptab, F,
+                                                                                          step, work);
+      PT_LAUNCH_CHECK();
+      return PT_OK;
+    }
+  }
   k_frame_pair_cols<N><<<dim3(N / MT, (F + 1) / 2), PairColsCfg<N>::T, dyn, st>>>(
       fold, corr, L, ptab, F, step, work);
   PT_LAUNCH_CHECK();

@@ -141,9 +141,13 @@ class CapturedFrames:
 class FoldedFrames:
     """``atm.opd_batch(times, directions)`` for a fixed number of times and
     directions, as one ``pt_spectral_frames_folded`` call per step on a stack
-    folded once.  The displacement tables stay in pinned host memory (two,
-    alternating, each reused once the call that read it has finished).  The
+    folded once.  The displacement tables stay in pinned host memory: a ring
+    of 2 B tables, an event recorded after every B calls, and a table
+    rewritten only once the batch that last read it has finished, so the
+    host waits on an event once per B calls instead of every call.  The
     returned tensor is reused by the next call: copy it if it must be kept."""
+
+    BATCH = 16  # calls per event
 
     def __init__(self, atm, frames: int, directions: Optional[Sequence] = None,
                  wavelength: Optional[float] = None) -> None:
@@ -171,9 +175,11 @@ class FoldedFrames:
         self._vel = np.stack([atm._vx, atm._vy], axis=-1)
         self._base = np.stack([atm._alt[None, :] * ox[:, None], atm._alt[None, :] * oy[:, None]],
                               axis=-1)
-        self._host = [t.zeros((F, L, 2), dtype=t.float64, pin_memory=True) for _ in range(2)]
+        nt = 2 * self.BATCH
+        self._host = [t.zeros((F, L, 2), dtype=t.float64, pin_memory=True) for _ in range(nt)]
         self._views = [h.numpy().reshape(self.frames, self.D, L, 2) for h in self._host]
-        self._done = [t.cuda.Event(), t.cuda.Event()]
+        self._done = [t.cuda.Event(), t.cuda.Event()]  # end of the last batch on each half of the ring
+        self._armed = [False, False]
         self.out = t.empty((F, atm.n, atm.n), dtype=t.float32, device=dev)
         self.work = t.empty((ops.frames_work_elems(ns, F, L, P),), dtype=t.complex64, device=dev)
         ops.spectral_fold(atm._spectra, F, self.work)
@@ -193,15 +199,19 @@ class FoldedFrames:
         if times.size != self.frames:
             raise ValueError(f"expected {self.frames} times, got {times.size}")
         k = self._next
-        self._next ^= 1
-        self._done[k].synchronize()  # the call that last read table k has finished
+        self._next = (k + 1) % len(self._host)
+        half, pos = divmod(k, self.BATCH)
+        if pos == 0 and self._armed[half]:
+            self._done[half].synchronize()  # the batch that last read these tables has finished
         v = self._views[k]
         np.multiply(times[:, None, None, None], self._vel[None, None], out=v)
         np.add(v, self._base[None], out=v)
         if not a._wrap_warned:
             a._check_wrap(v[..., 0], v[..., 1])
         _hip.check(self._fn(*self._args[k], _hip.stream_ptr()), "pt_spectral_frames_folded")
-        self._done[k].record()
+        if pos == self.BATCH - 1:
+            self._done[half].record()
+            self._armed[half] = True
         out = a._cast(self.out)
         if self.D > 1:
             out = out.view(self.frames, self.D, a.n, a.n)
