@@ -249,11 +249,12 @@ int pt_spectral_rows_folded(int layers, int n, double pixel_scale,
 /*
  * pt_spectral_rows_folded + pt_spectral_cols in one call, on the fold that
  * pt_spectral_fold left in `work` (same rules).  For calls of at most 8
- * frames the pair tables and the subharmonic tables are built in one launch
- * (three kernels per call instead of four), which is what a loop of a few
- * frames per call, replayed from a HIP graph, waits on.  `disp` may point to
- * pinned host memory (hipHostMalloc): only the table kernels read it.
- * Same results as the two calls.  No counterpart in the reference (its
+ * frames the per-call tables are built with pass A (n = 128..1024: pass A
+ * builds its own pair tables and further workgroups of the same launch the
+ * subharmonic tables; two kernels per call), which is what a loop of a few
+ * frames per call waits on.  `disp` may point to pinned host memory
+ * (hipHostMalloc): the kernels read it during the call, so it must not be
+ * rewritten before the call has finished.  Same results as the two calls.  No counterpart in the reference (its
  * per-frame layer sum is atmosphere.py:910-917).  Power-of-two n only.
  */
 int pt_spectral_frames_folded(int layers, int n, double pixel_scale,
