@@ -210,7 +210,7 @@ def test_uniform_step_frames_match_oracle(hip, order, n, frames):
             assert O.rel_rms(got["uniform"][t], got["off"][t]) < 1e-6, t
 
 
-@pytest.mark.parametrize("n", [64, 512])
+@pytest.mark.parametrize("n", [64, 256, 512, 1024])
 def test_few_frame_calls_equal_batched_calls(hip, n):
     """Frames do not depend on how many other frames share their call: a call
     of 1 to 4 frame pairs gives, bit for bit, the same frames as a larger
