@@ -316,37 +316,73 @@ struct NyqTabs {
   static constexpr int LB = N <= 512 ? FOLD_LMAX : 4;
   float2 t8s[LB][2][2][N / 8], t1s[LB][2][2][8];  // [layer][frame][varying axis]
 };
+// Entry j of frame f's table along axis ax for layer l: T8[j] (j < N/8)
+// or T1[j - N/8]; zero for an absent second frame.
+template <int N>
+__device__ __forceinline__ float2 nyq_tab_value(const uint32_t (&u)[2][FOLD_LMAX][2], int f, int ax, int l, int j,
+                                                bool has_b) {
+  constexpr int NB = N / 8;
+  const uint32_t uv = u[f][l][ax], uo = u[f][l][ax ^ 1];
+  // one phasor on either branch (straight-line code, so that entries
+  // computed together overlap)
+  const float2 v = phasor_fixed(j < NB ? (uint32_t)(8 * j - N / 2 + 1) * uv + (uint32_t)(N / 2) * uo
+                                       : (uint32_t)(j - NB) * uv);
+  return f != 0 && !has_b ? make_float2(0.f, 0.f) : v;
+}
+// Both axes' tables (ax_only < 0), or one axis' only.
 template <int N>
 __device__ __forceinline__ void nyq_build(NyqTabs<N>& tb, const uint32_t (&u)[2][FOLD_LMAX][2], int l0, int nl,
-                                          bool has_b, int tid, int nthreads) {
-  constexpr int NB = N / 8;
-  for (int e = tid; e < nl * 4 * (NB + 8); e += nthreads) {
-    const int lb = e / (4 * (NB + 8)), r = e % (4 * (NB + 8));
-    const int fa = r / (NB + 8), j = r % (NB + 8);
-    const int f = fa >> 1, ax = fa & 1, l = l0 + lb;
-    float2 v = make_float2(0.f, 0.f);
-    if (f == 0 || has_b) {
-      const uint32_t uv = u[f][l][ax], uo = u[f][l][ax ^ 1];
-      v = j < NB ? phasor_fixed((uint32_t)(8 * j - N / 2 + 1) * uv + (uint32_t)(N / 2) * uo)
-                 : phasor_fixed((uint32_t)(j - NB) * uv);
+                                          bool has_b, int tid, int nthreads, int ax_only = -1) {
+  constexpr int NB = N / 8, RU = 3;  // RU entries per thread at a time (independent chains)
+  const int nax = ax_only < 0 ? 2 : 1, total = nl * 2 * nax * (NB + 8);
+  for (int e0 = tid; e0 < total; e0 += RU * nthreads) {
+    float2 v[RU];
+#pragma unroll
+    for (int h = 0; h < RU; ++h) {  // an index past the end recomputes the last entry
+      const int e = min(e0 + h * nthreads, total - 1);
+      const int lb = e / (2 * nax * (NB + 8)), r = e % (2 * nax * (NB + 8));
+      const int fa = r / (NB + 8), j = r % (NB + 8);
+      const int f = ax_only < 0 ? fa >> 1 : fa, ax = ax_only < 0 ? fa & 1 : ax_only;
+      v[h] = nyq_tab_value<N>(u, f, ax, l0 + lb, j, has_b);
     }
-    if (j < NB)
-      tb.t8s[lb][f][ax][j] = v;
-    else
-      tb.t1s[lb][f][ax][j - NB] = v;
+#pragma unroll
+    for (int h = 0; h < RU; ++h) {
+      const int e = e0 + h * nthreads;
+      if (e < total) {
+        const int lb = e / (2 * nax * (NB + 8)), r = e % (2 * nax * (NB + 8));
+        const int fa = r / (NB + 8), j = r % (NB + 8);
+        const int f = ax_only < 0 ? fa >> 1 : fa, ax = ax_only < 0 ? fa & 1 : ax_only;
+        if (j < NB)
+          tb.t8s[lb][f][ax][j] = v[h];
+        else
+          tb.t1s[lb][f][ax][j - NB] = v[h];
+      }
+    }
   }
+}
+// Table position of Nyquist entry k along its varying axis: T8[id >> 3] and
+// T1[id & 7].
+template <int N>
+__device__ __forceinline__ int nyq_index(int k) {
+  return k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
+}
+// acc += one layer's term from its table values (both frames) and side
+// value sq.
+__device__ __forceinline__ void nyq_term(float2& acc, float2 t8a, float2 t1a, float2 t8b, float2 t1b, float2 sq) {
+  const float2 wa = cmul(t8a, t1a);
+  const float2 wb = cmul(t8b, t1b);
+  const float2 t = cmul(sq, make_float2(wa.x - wb.y, wa.y + wb.x));
+  acc.x += t.x;
+  acc.y += t.y;
 }
 // acc += the term of layer block entry lb for Nyquist entry q (q < 2N, not
 // the Nyquist column's own N/2 entry), side value sq
 template <int N>
 __device__ __forceinline__ void nyq_add(float2& acc, const NyqTabs<N>& tb, int lb, int q, float2 sq) {
   const int ax = q < N ? 1 : 0, k = q < N ? q : q - N;
-  const int id = k <= N / 2 ? k + N / 2 - 1 : k - N / 2 - 1;
-  const float2 wa = cmul(tb.t8s[lb][0][ax][id >> 3], tb.t1s[lb][0][ax][id & 7]);
-  const float2 wb = cmul(tb.t8s[lb][1][ax][id >> 3], tb.t1s[lb][1][ax][id & 7]);
-  const float2 t = cmul(sq, make_float2(wa.x - wb.y, wa.y + wb.x));
-  acc.x += t.x;
-  acc.y += t.y;
+  const int id = nyq_index<N>(k);
+  nyq_term(acc, tb.t8s[lb][0][ax][id >> 3], tb.t1s[lb][0][ax][id & 7], tb.t8s[lb][1][ax][id >> 3],
+           tb.t1s[lb][1][ax][id & 7], sq);
 }
 
 template <int N, bool TR = false>
@@ -641,6 +677,12 @@ struct FewColsCfg {
   // profile in one block) at one wave per SIMD, four at two or more; the
   // first block loads before the phasor tables are staged
   static constexpr int PF = T <= 256 ? 9 : 4;
+  // the same when the launch builds its own tables (k_frame_pair_cols_tab):
+  // the first block is requested just before the phasor tables are built,
+  // and a wave cannot issue past a long run of loads until most of them
+  // have left, so a short block lets the tables start sooner (n = 512, one
+  // frame per call: 22.8 us per call with 3 layers, 24.1 us with 9)
+  static constexpr int PF_TAB = T <= 256 ? 3 : 4;
 };
 
 // Folded stack for pass A' (same values as k_frame_fold): tile t = k1 / mt
@@ -746,25 +788,40 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
 #pragma unroll
       for (int h = 0; h < 4; ++h) gb[lb][h] = fp[(size_t)min(l0 + lb, L - 1) * (plane / 2) + (size_t)h * N * NB];
   };
-  // Nyquist terms of this tile: cl[m] for column c0 + m on the Nyquist row,
-  // cl[MT + k0] for the Nyquist column when the tile holds column N/2
-  __shared__ float2 cl[TAB ? MT + N : 1];
+  // TAB: the Nyquist column's terms cl[k0] (the tile that holds column N/2),
+  // the phases, the row entries' side values and table values rt[m][l][f]
+  // (T8, T1 of entry c0 + m)
+  __shared__ float2 cl[TAB ? N : 1];
   const bool nyq_col = c0 <= N / 2 && N / 2 < c0 + MT;
   __shared__ uint32_t u[TAB ? 2 : 1][FOLD_LMAX][2];
-  __shared__ float2 srow[TAB ? MT : 1][FOLD_LMAX];  // side values of the Nyquist row entries
-  // TAB: the displacements and the row entries' side values load before the
-  // fold, so waiting for them does not wait for the fold's stream
+  __shared__ float2 srow[TAB ? MT : 1][FOLD_LMAX];
+  __shared__ float2 rt[TAB ? MT : 1][FOLD_LMAX][2][2];
+  // TAB: the displacements and the side values (the row entries', and the
+  // Nyquist column's in the tile that holds it) load first, before any
+  // tile's fold traffic is queued
   double dv = 0.0;
   float2 sv = make_float2(0.f, 0.f);
   uint32_t gv = FOLD_PLAIN;  // the fold marker (a reused fold must be the plain one)
+  constexpr int CQ = TAB ? 2 : 1;  // Nyquist column entries per thread (T = N / 2)
+  float2 sc[CQ][TAB ? FOLD_LMAX : 1];
   if constexpr (TAB) {
+    static_assert(2 * T == N, "two Nyquist column entries per thread");
     if (ta.guard) gv = *ta.guard;
     if (tid < 4 * L) {
       const int f = tid / (2 * L), t = 2 * pr + f;
       dv = ta.disp[(size_t)min(t, F - 1) * 2 * L + tid % (2 * L)];
     }
     if (tid < MT * L) sv = ta.side[(size_t)(tid / MT) * 2 * N + c0 + tid % MT];
+    if (nyq_col) {
+#pragma unroll
+      for (int i = 0; i < CQ; ++i)
+#pragma unroll
+        for (int l = 0; l < FOLD_LMAX; ++l)
+          sc[i][l] = l < L ? ta.side[(size_t)l * 2 * N + N + tid + i * T] : make_float2(0.f, 0.f);
+    }
   }
+  const bool poison = TAB && gv != FOLD_PLAIN;
+  const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
   // PF > 1: the twiddles and (without TAB) this pair's tables load into
   // registers ahead of the fold (the load counter drains in order, so a
   // load issued after the fold's would wait for all of it)
@@ -783,7 +840,7 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
       for (int r = 0; r < PTR; ++r) ptv[r] = tp[min(tid + r * T, 2 * L * S - 1)];
     }
     __builtin_amdgcn_sched_barrier(0);
-    load_block(0);
+    if constexpr (!TAB) load_block(0);
   }
   if constexpr (TAB) {
     // fixed-point phases of the pair's frames (a frame past the end is absent)
@@ -793,26 +850,75 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
     }
     if (tid < MT * L) srow[tid % MT][tid / MT] = sv;
     __syncthreads();
-    const bool poison = gv != FOLD_PLAIN;
-    const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
-    for (int q = tid; q < 2 * L * S; q += T) {
-      const int fl = q / S, e = q % S;
-      const int fb = fl / L, l = fl % L;
-      const uint32_t ut = u[fb][l][0], uo = u[fb][l][1];  // column-first: the table axis is x
-      float2 v;
-      if (e < NB)
-        v = phasor_fixed((uint32_t)(8 * e - N / 2) * ut);
-      else if (e < NB + 8)
-        v = phasor_fixed((uint32_t)(e - NB) * ut);
-      else
-        v = make_float2(__uint_as_float(uo), 0.f);
-      if (poison && e < NB + 8) v = nan2;
-      if (e < NB)
-        t8[fl * NB + e] = v;
-      else if (e < NB + 8)
-        t1[fl * 8 + e - NB] = v;
-      else
-        U[fl] = __float_as_uint(v.x);
+    // The Nyquist column's sums (k1 = N/2, every k0 but N/2) in the tile
+    // that holds column N/2, k_frame_pair_tables' tables and terms, layers in
+    // order, LB per pair of barriers.  They come before this tile's fold is
+    // requested: its loads would otherwise wait behind its fold, and the
+    // other tiles' fold traffic is under way meanwhile.
+    if (nyq_col) {  // workgroup-uniform
+      constexpr int LB = NyqTabs<N>::LB;
+      __shared__ NyqTabs<N> nt;
+      const bool has_b = 2 * pr + 1 < F;
+      float2 acc[CQ];
+#pragma unroll
+      for (int i = 0; i < CQ; ++i) acc[i] = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int bl = 0; bl < FOLD_LMAX / LB; ++bl) {
+        const int l0 = bl * LB;
+        if (l0 >= L) break;  // workgroup-uniform
+        const int nl = min(LB, L - l0);
+        if (bl > 0) __syncthreads();  // the previous layers' tables are no longer read
+        nyq_build<N>(nt, u, l0, nl, has_b, tid, T, 0);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < CQ; ++i) {
+          const int k0 = tid + i * T;
+          if (k0 != N / 2) {
+#pragma unroll
+            for (int lb = 0; lb < LB; ++lb)
+              if (lb < nl) nyq_add<N>(acc[i], nt, lb, N + k0, sc[i][l0 + lb]);
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < CQ; ++i) cl[tid + i * T] = poison ? nan2 : acc[i];
+    }
+    // the fold streams in once the displacements and side values are here:
+    // issued earlier, it queues those few loads behind every tile's fold
+    // traffic (about 4 us of the table phase at n = 512), and the tables
+    // below are built while it arrives
+    __builtin_amdgcn_sched_barrier(0);
+    load_block(0);
+    // RU entries per thread at a time, as independent chains (one wave per
+    // SIMD hides no latency otherwise); an index past the end recomputes the
+    // last entry and is not stored
+    constexpr int RU = 3;
+    const int ntab = 2 * L * S;
+    for (int q0 = tid; q0 < ntab; q0 += RU * T) {
+      float2 v[RU];
+#pragma unroll
+      for (int h = 0; h < RU; ++h) {
+        const int q = min(q0 + h * T, ntab - 1);
+        const int fl = q / S, e = q % S;
+        const int fb = fl / L, l = fl % L;
+        const uint32_t ut = u[fb][l][0], uo = u[fb][l][1];  // column-first: the table axis is x
+        v[h] = phasor_fixed(e < NB ? (uint32_t)(8 * e - N / 2) * ut : (uint32_t)(e - NB) * ut);
+        if (e >= NB + 8) v[h] = make_float2(__uint_as_float(uo), 0.f);
+        if (poison && e < NB + 8) v[h] = nan2;
+      }
+#pragma unroll
+      for (int h = 0; h < RU; ++h) {
+        const int q = q0 + h * T;
+        if (q < ntab) {
+          const int fl = q / S, e = q % S;
+          if (e < NB)
+            t8[fl * NB + e] = v[h];
+          else if (e < NB + 8)
+            t1[fl * 8 + e - NB] = v[h];
+          else
+            U[fl] = __float_as_uint(v[h].x);
+        }
+      }
     }
   } else {
     auto put = [&](int q, float2 v) {
@@ -840,7 +946,17 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
     init_twiddles<N, 1>(tw, tid, T);
   }
   __syncthreads();
-  for (int q = tid; q < 2 * L * MT; q += T) py[q] = phasor_fixed((uint32_t)signed_freq(c0 + q % MT, N) * U[q / MT]);
+  // and (TAB) the row entries' table values, one phasor each
+  for (int q = tid; q < (TAB ? 6 : 2) * L * MT; q += T) {
+    if (q < 2 * L * MT) {
+      py[q] = phasor_fixed((uint32_t)signed_freq(c0 + q % MT, N) * U[q / MT]);
+    } else if constexpr (TAB) {
+      const int r = q - 2 * L * MT;
+      const int mm = r / (4 * L), l = (r / 4) % L, f = (r >> 1) & 1, w = r & 1;
+      const int id = nyq_index<N>(c0 + mm);
+      rt[mm][l][f][w] = nyq_tab_value<N>(u, f, 1, l, w == 0 ? id >> 3 : NB + (id & 7), 2 * pr + 1 < F);
+    }
+  }
   __syncthreads();
   float2 z[8];
 #pragma unroll
@@ -907,60 +1023,19 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
     layer_sum(std::false_type{});
   // Nyquist row / column terms of this pair: corr[k1] on the Nyquist row
   // (k0 = N/2, every k1), corr[N + k0] on the Nyquist column (k1 = N/2,
-  // k0 != N/2)
-  if constexpr (TAB) {
-    // Nyquist sums (k_frame_pair_tables' tables and terms), once the layer
-    // sum has freed its registers: row entry q < N has k0 = N/2, k1 = q;
-    // column entry N + k0 has k1 = N/2; each entry's side values are loaded
-    // before its sum, and the layers are summed in order, LB per pair of
-    // barriers
-    constexpr int LB = NyqTabs<N>::LB, QT = (MT + N + T - 1) / T;
-    __shared__ NyqTabs<N> nt;
-    const bool has_b = 2 * pr + 1 < F;
-    const bool poison = gv != FOLD_PLAIN;
-    const int nq = MT + (nyq_col ? N : 0);
-    float2 acc[QT];
-#pragma unroll
-    for (int i = 0; i < QT; ++i) acc[i] = make_float2(0.f, 0.f);
-    for (int l0 = 0; l0 < L; l0 += LB) {
-      const int nl = min(LB, L - l0);
-      float2 sq[QT][LB];  // row entries from srow, Nyquist column entries from the side table
-#pragma unroll
-      for (int i = 0; i < QT; ++i) {
-        const int idx = min(tid + i * T, nq - 1);
-        const int q = idx < MT ? c0 + idx : N + (idx - MT);
-#pragma unroll
-        for (int lb = 0; lb < LB; ++lb) {
-          const int l = l0 + min(lb, nl - 1);
-          sq[i][lb] = idx < MT ? srow[idx][l] : ta.side[(size_t)l * 2 * N + q];
-        }
-      }
-      __syncthreads();  // the previous layers' tables are no longer read
-      nyq_build<N>(nt, u, l0, nl, has_b, tid, T);
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < QT; ++i) {
-        const int idx = tid + i * T;
-        const int q = idx < MT ? c0 + idx : N + (idx - MT);
-        if (idx < nq && !(q >= N && q - N == N / 2)) {
-#pragma unroll
-          for (int lb = 0; lb < LB; ++lb)
-            if (lb < nl) nyq_add<N>(acc[i], nt, lb, q, sq[i][lb]);
-        }
-      }
-    }
-    const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
-#pragma unroll
-    for (int i = 0; i < QT; ++i) {
-      const int idx = tid + i * T;
-      if (idx < nq) cl[idx] = poison ? nan2 : acc[i];
-    }
-    __syncthreads();
-  }
+  // k0 != N/2); TAB: the row entry's sum here, the column's from cl
   const float2* cp = TAB ? nullptr : corr + (size_t)pr * 2 * N;
   constexpr int RN = (N / 2) / NB;  // element r holding row N/2 (for j == (N/2) % NB)
   if (j == (N / 2) % NB) {
-    const float2 c = TAB ? cl[m] : cp[k1];
+    float2 c;
+    if constexpr (TAB) {
+      float2 acc = make_float2(0.f, 0.f);  // k_frame_pair_tables' terms, layers in order
+      for (int l = 0; l < L; ++l)
+        nyq_term(acc, rt[m][l][0][0], rt[m][l][0][1], rt[m][l][1][0], rt[m][l][1][1], srow[m][l]);
+      c = poison ? nan2 : acc;
+    } else {
+      c = cp[k1];
+    }
     z[RN].x += c.x;
     z[RN].y += c.y;
   }
@@ -969,7 +1044,7 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
     for (int r = 0; r < 8; ++r) {
       const int k0 = j + r * NB;
       if (k0 == N / 2) continue;
-      const float2 c = TAB ? cl[MT + k0] : cp[N + k0];
+      const float2 c = TAB ? cl[k0] : cp[N + k0];
       z[r].x += c.x;
       z[r].y += c.y;
     }
@@ -1022,7 +1097,7 @@ __global__ __launch_bounds__(FewColsCfg<N>::T) void k_frame_pair_cols_tab(const 
     sub_fill_set(SB, ta.P, N, f, ta.tab, f == 0, &s_mean, threadIdx.x, T, i % tb, tb, nullptr);
     return;
   }
-  pair_cols_body<N, FewColsCfg<N>::MT, FewColsCfg<N>::PF, true>(fold, nullptr, L, nullptr, F, nullptr, work, ta);
+  pair_cols_body<N, FewColsCfg<N>::MT, FewColsCfg<N>::PF_TAB, true>(fold, nullptr, L, nullptr, F, nullptr, work, ta);
 }
 
 // Pass B' of the column-first engine: the row transform of pair b (a plan
