@@ -15,6 +15,8 @@
 // is the only extra HBM traffic.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <chrono>
 #include <type_traits>
 
 #include "common.hpp"
@@ -742,7 +744,81 @@ struct PairTabArgs {
   const double2* coeffs;
   int P;
   SubTables tab;
+  // the Nyquist column's sums, N per pair, written by the launch's producer
+  // workgroup of each pair, which then sets nflag[pair] to token; the tile
+  // that holds column N/2 waits for it (at most spin_limit polls, NaN
+  // frames past that) and clears it
+  float2* ncol;
+  unsigned long long* nflag;
+  unsigned long long token;
+  unsigned spin_limit;
 };
+
+// The Nyquist column's sums of pair pr (k1 = N/2, every k0 but N/2) with
+// k_frame_pair_tables' tables and terms, layers in order, LB per pair of
+// barriers: thread tid (of T = N / 2) owns k0 = tid + i T, sc its side
+// values.  Every thread of the workgroup calls it.
+template <int N, int T>
+__device__ __forceinline__ void nyq_col_sums(const uint32_t (&u)[2][FOLD_LMAX][2], const float2 (&sc)[2][FOLD_LMAX],
+                                             int L, bool has_b, int tid, float2 (&acc)[2]) {
+  static_assert(2 * T == N, "two Nyquist column entries per thread");
+  constexpr int LB = NyqTabs<N>::LB;
+  __shared__ NyqTabs<N> nt;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) acc[i] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int bl = 0; bl < FOLD_LMAX / LB; ++bl) {
+    const int l0 = bl * LB;
+    if (l0 >= L) break;  // workgroup-uniform
+    const int nl = min(LB, L - l0);
+    if (bl > 0) __syncthreads();  // the previous layers' tables are no longer read
+    nyq_build<N>(nt, u, l0, nl, has_b, tid, T, 0);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int k0 = tid + i * T;
+      if (k0 != N / 2) {
+#pragma unroll
+        for (int lb = 0; lb < LB; ++lb)
+          if (lb < nl) nyq_add<N>(acc[i], nt, lb, N + k0, sc[i][l0 + lb]);
+      }
+    }
+  }
+}
+
+// Producer workgroup of pair blockIdx.y: the pair's phases (as the tiles
+// work them out), its Nyquist column sums into ta.ncol, then the flag.
+template <int N, int T>
+__device__ void nyq_col_producer(int L, int F, const PairTabArgs& ta) {
+  __shared__ uint32_t u[2][FOLD_LMAX][2];
+  const int tid = threadIdx.x, pr = blockIdx.y;
+  const uint32_t gv = ta.guard ? *ta.guard : FOLD_PLAIN;
+  double dv = 0.0;
+  if (tid < 4 * L) {
+    const int f = tid / (2 * L), t = 2 * pr + f;
+    dv = ta.disp[(size_t)min(t, F - 1) * 2 * L + tid % (2 * L)];
+  }
+  float2 sc[2][FOLD_LMAX];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int l = 0; l < FOLD_LMAX; ++l)
+      sc[i][l] = l < L ? ta.side[(size_t)l * 2 * N + N + tid + i * T] : make_float2(0.f, 0.f);
+  if (tid < 4 * L) {
+    const int f = tid / (2 * L), e = tid % (2 * L), t = 2 * pr + f;
+    u[f][e >> 1][e & 1] = t < F ? cycles_fixed(dv * ta.inv_period) : 0u;
+  }
+  __syncthreads();
+  float2 acc[2];
+  nyq_col_sums<N, T>(u, sc, L, 2 * pr + 1 < F, tid, acc);
+  const bool poison = gv != FOLD_PLAIN;
+  const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+#pragma unroll
+  for (int i = 0; i < 2; ++i) ta.ncol[(size_t)pr * N + tid + i * T] = poison ? nan2 : acc[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's sums reach memory before the flag
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(ta.nflag + pr, ta.token, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // MT: columns of this launch's tiles; the fold is laid out in tiles of
 // PairColsCfg<N>::MT columns (MF).  PF > 1: the fold values of PF layers are
@@ -788,37 +864,24 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
 #pragma unroll
       for (int h = 0; h < 4; ++h) gb[lb][h] = fp[(size_t)min(l0 + lb, L - 1) * (plane / 2) + (size_t)h * N * NB];
   };
-  // TAB: the Nyquist column's terms cl[k0] (the tile that holds column N/2),
-  // the phases, the row entries' side values and table values rt[m][l][f]
-  // (T8, T1 of entry c0 + m)
-  __shared__ float2 cl[TAB ? N : 1];
+  // TAB: the phases, the row entries' side values and table values
+  // rt[m][l][f] (T8, T1 of entry c0 + m)
   const bool nyq_col = c0 <= N / 2 && N / 2 < c0 + MT;
   __shared__ uint32_t u[TAB ? 2 : 1][FOLD_LMAX][2];
   __shared__ float2 srow[TAB ? MT : 1][FOLD_LMAX];
   __shared__ float2 rt[TAB ? MT : 1][FOLD_LMAX][2][2];
-  // TAB: the displacements and the side values (the row entries', and the
-  // Nyquist column's in the tile that holds it) load first, before any
-  // tile's fold traffic is queued
+  // TAB: the displacements and the row entries' side values load first,
+  // before any tile's fold traffic is queued
   double dv = 0.0;
   float2 sv = make_float2(0.f, 0.f);
   uint32_t gv = FOLD_PLAIN;  // the fold marker (a reused fold must be the plain one)
-  constexpr int CQ = TAB ? 2 : 1;  // Nyquist column entries per thread (T = N / 2)
-  float2 sc[CQ][TAB ? FOLD_LMAX : 1];
   if constexpr (TAB) {
-    static_assert(2 * T == N, "two Nyquist column entries per thread");
     if (ta.guard) gv = *ta.guard;
     if (tid < 4 * L) {
       const int f = tid / (2 * L), t = 2 * pr + f;
       dv = ta.disp[(size_t)min(t, F - 1) * 2 * L + tid % (2 * L)];
     }
     if (tid < MT * L) sv = ta.side[(size_t)(tid / MT) * 2 * N + c0 + tid % MT];
-    if (nyq_col) {
-#pragma unroll
-      for (int i = 0; i < CQ; ++i)
-#pragma unroll
-        for (int l = 0; l < FOLD_LMAX; ++l)
-          sc[i][l] = l < L ? ta.side[(size_t)l * 2 * N + N + tid + i * T] : make_float2(0.f, 0.f);
-    }
   }
   const bool poison = TAB && gv != FOLD_PLAIN;
   const float2 nan2 = make_float2(__builtin_nanf(""), __builtin_nanf(""));
@@ -850,39 +913,6 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
     }
     if (tid < MT * L) srow[tid % MT][tid / MT] = sv;
     __syncthreads();
-    // The Nyquist column's sums (k1 = N/2, every k0 but N/2) in the tile
-    // that holds column N/2, k_frame_pair_tables' tables and terms, layers in
-    // order, LB per pair of barriers.  They come before this tile's fold is
-    // requested: its loads would otherwise wait behind its fold, and the
-    // other tiles' fold traffic is under way meanwhile.
-    if (nyq_col) {  // workgroup-uniform
-      constexpr int LB = NyqTabs<N>::LB;
-      __shared__ NyqTabs<N> nt;
-      const bool has_b = 2 * pr + 1 < F;
-      float2 acc[CQ];
-#pragma unroll
-      for (int i = 0; i < CQ; ++i) acc[i] = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int bl = 0; bl < FOLD_LMAX / LB; ++bl) {
-        const int l0 = bl * LB;
-        if (l0 >= L) break;  // workgroup-uniform
-        const int nl = min(LB, L - l0);
-        if (bl > 0) __syncthreads();  // the previous layers' tables are no longer read
-        nyq_build<N>(nt, u, l0, nl, has_b, tid, T, 0);
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < CQ; ++i) {
-          const int k0 = tid + i * T;
-          if (k0 != N / 2) {
-#pragma unroll
-            for (int lb = 0; lb < LB; ++lb)
-              if (lb < nl) nyq_add<N>(acc[i], nt, lb, N + k0, sc[i][l0 + lb]);
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < CQ; ++i) cl[tid + i * T] = poison ? nan2 : acc[i];
-    }
     // the fold streams in once the displacements and side values are here:
     // issued earlier, it queues those few loads behind every tile's fold
     // traffic (about 4 us of the table phase at n = 512), and the tables
@@ -1025,6 +1055,29 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
   // (k0 = N/2, every k1), corr[N + k0] on the Nyquist column (k1 = N/2,
   // k0 != N/2); TAB: the row entry's sum here, the column's from cl
   const float2* cp = TAB ? nullptr : corr + (size_t)pr * 2 * N;
+  bool col_ok = true;
+  if constexpr (TAB) {
+    if (nyq_col) {  // workgroup-uniform: this pair's producer has written its sums
+      __shared__ int s_ok;
+      if (tid < 64) {
+        int good = 1;
+        for (unsigned it = 0;; ++it) {
+          const unsigned long long f = __hip_atomic_load(ta.nflag + pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (f == ta.token) break;
+          if (it >= ta.spin_limit) {
+            good = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the sums are read fresh after this
+        if (tid == 0) s_ok = good;
+      }
+      __syncthreads();
+      col_ok = s_ok != 0;
+      if (tid == 0) __hip_atomic_store(ta.nflag + pr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
   constexpr int RN = (N / 2) / NB;  // element r holding row N/2 (for j == (N/2) % NB)
   if (j == (N / 2) % NB) {
     float2 c;
@@ -1044,7 +1097,7 @@ __device__ __forceinline__ void pair_cols_body(const float2* __restrict__ fold, 
     for (int r = 0; r < 8; ++r) {
       const int k0 = j + r * NB;
       if (k0 == N / 2) continue;
-      const float2 c = TAB ? cl[k0] : cp[N + k0];
+      const float2 c = TAB ? (col_ok ? ta.ncol[(size_t)pr * N + k0] : nan2) : cp[N + k0];
       z[r].x += c.x;
       z[r].y += c.y;
     }
@@ -1079,13 +1132,21 @@ __global__ __launch_bounds__(FewColsCfg<N>::T) void k_frame_pair_cols_few(const 
 // The same for a reused fold with the call's tables built in the same
 // launch: blocks x < N / MT are pass A' tiles that build their own pair
 // tables (TAB); blocks beyond build the pair's subharmonic tables for pass
-// B' (tb per frame, k_frame_prep's arithmetic), on other CUs at the same
-// time.  One launch instead of two per call.
+// B' (tb per frame, k_frame_prep's arithmetic), and the last one the pair's
+// Nyquist column sums for the tile that holds column N/2, on other CUs at
+// the same time.  One launch instead of two per call.  Only that tile waits,
+// and only for a workgroup of its own launch: every other workgroup runs to
+// its end without waiting, so the producer finds a free slot whatever the
+// dispatch order and occupancy.
 template <int N>
 __global__ __launch_bounds__(FewColsCfg<N>::T) void k_frame_pair_cols_tab(const float2* __restrict__ fold, int L, int F,
                                                                         float2* __restrict__ work, PairTabArgs ta,
                                                                         int tb) {
   constexpr int T = FewColsCfg<N>::T, NT = N / FewColsCfg<N>::MT;
+  if ((int)blockIdx.x == NT + 2 * tb) {  // workgroup-uniform: the pair's Nyquist column sums
+    nyq_col_producer<N, T>(L, F, ta);
+    return;
+  }
   if ((int)blockIdx.x >= NT) {  // workgroup-uniform
     const int i = blockIdx.x - NT, f = 2 * blockIdx.y + i / tb;
     if (f >= F || ta.P == 0) return;
@@ -1262,6 +1323,15 @@ static int generate_pow2(const float* nre, ptrdiff_t im_off, size_t gstride, int
 struct FramesLayout {
   size_t fold, side, ptab, corr, step, tab, total;
 };
+
+// A value no earlier launch of the process used (the producer/consumer
+// flags of k_frame_pair_cols_tab): a clock-seeded odd start, then +1.
+static unsigned long long next_launch_token() {
+  static std::atomic<unsigned long long> next{
+      ((unsigned long long)std::chrono::steady_clock::now().time_since_epoch().count() * 0x9E3779B97F4A7C15ull) | 1ull};
+  unsigned long long t = next.fetch_add(1);
+  return t ? t : next.fetch_add(1);
+}
 static inline size_t r32(size_t v) { return (v + 31) & ~(size_t)31; }
 static FramesLayout frames_layout(int n, int F, int L, int P) {
   FramesLayout w;
@@ -1396,8 +1466,15 @@ static int frames_cf_rows_pow2(const float2* spectra, int L, const double* disp,
       rc = set_dyn_lds(k_frame_pair_cols_tab<N>, dyn_t);
       if (rc) return rc;
       const int tb = ta.P > 0 ? 16 : 0;  // subharmonic table blocks per frame
-      k_frame_pair_cols_tab<N><<<dim3(N / MTF + 2 * tb, (F + 1) / 2), FewColsCfg<N>::T, dyn_t, st>>>(fold, L, F, work,
-                                                                                                  ta, tb);
+      // the Nyquist column sums go where the unfolded call keeps its
+      // correction terms, and the flags after the fold marker
+      static_assert((STEP_WORDS + 2) / 2 + FEW_PAIRS <= 32, "flags inside the step words' padding");
+      ta.ncol = corr;
+      ta.nflag = reinterpret_cast<unsigned long long*>(work + w.step + (STEP_WORDS + 2) / 2);
+      ta.token = next_launch_token();
+      ta.spin_limit = 1u << 20;
+      k_frame_pair_cols_tab<N><<<dim3(N / MTF + 2 * tb + 1, (F + 1) / 2), FewColsCfg<N>::T, dyn_t, st>>>(
+          fold, L, F, work, ta, tb);
       PT_LAUNCH_CHECK();
       return PT_OK;
     }
