@@ -251,8 +251,11 @@ int pt_spectral_rows_folded(int layers, int n, double pixel_scale,
  * pt_spectral_fold left in `work` (same rules).  For calls of at most 8
  * frames the per-call tables are built with pass A (n = 128..1024: pass A
  * builds its own pair tables and further workgroups of the same launch the
- * subharmonic tables; two kernels per call), which is what a loop of a few
- * frames per call waits on.  `disp` may point to pinned host memory
+ * subharmonic tables and the Nyquist column's sums; two kernels per call),
+ * which is what a loop of a few frames per call waits on.  One workgroup
+ * per frame pair waits (bounded) for a producer workgroup of its own launch
+ * through a flag in `work`: calls on one `work` must not overlap (as for
+ * every call that writes it).  `disp` may point to pinned host memory
  * (hipHostMalloc): the kernels read it during the call, so it must not be
  * rewritten before the call has finished.  Same results as the two calls.  No counterpart in the reference (its
  * per-frame layer sum is atmosphere.py:910-917).  Power-of-two n only.
